@@ -1,0 +1,262 @@
+// Adjoint of the randomisation-method sum (gradients for calibration).
+//
+// Forward (src/gstools/field/generator.py:193-199):
+//   u_i = sum_j z1_j cos(t_ij) + z2_j sin(t_ij),   t_ij = k_j . x_i
+// For an upstream gradient g_i = dL/du_i:
+//   dL/dz1_j   = sum_i g_i cos t_ij
+//   dL/dz2_j   = sum_i g_i sin t_ij
+//   dL/dk_jd   = sum_i g_i x_id (z2_j cos t_ij - z1_j sin t_ij)
+//              = z2_j Cx_jd - z1_j Sx_jd,  Cx_jd = sum_i (g_i x_id) cos t_ij, ...
+//
+// Schedule: a reduction over points per mode.  Stage 1 runs a grid of
+// (mode tile of kThreads modes) x (point chunk); each thread owns one mode and
+// walks the chunk's points, which are staged through LDS with g_i x_id
+// precomputed (every lane reads the same point: LDS broadcast).  It writes
+// the 2 + 2D partial sums of its (chunk, mode) to the workspace.  Stage 2
+// sums the chunks of each mode in a fixed order, so results are bitwise
+// reproducible run to run (no atomics).
+//
+// FP64: exact reference phase and sincos_fast (two-term reduction) with a
+// library fallback for waves whose phases may exceed 2^30.  FP32: phase in
+// revolutions in FP64, v_fract_f64, hardware v_sin_f32 / v_cos_f32, FP32
+// partial sums per LDS chunk folded into FP64 accumulators.
+#include "gsx_internal.h"
+
+namespace gsx {
+
+constexpr int kAdjPts = 256;         // points per LDS stage
+constexpr int64_t kAdjTargetBlocks = 2048;
+
+struct AdjPlan {
+    int64_t mode_tiles, chunks, chunk_pts;
+};
+
+inline AdjPlan adj_plan(int64_t N, int64_t P) {
+    AdjPlan pl;
+    pl.mode_tiles = ceil_div(N, kThreads);
+    const int64_t max_chunks = ceil_div(P, kAdjPts);
+    int64_t chunks = ceil_div(kAdjTargetBlocks, pl.mode_tiles);
+    if (chunks > max_chunks) chunks = max_chunks;
+    if (chunks < 1) chunks = 1;
+    pl.chunk_pts = ceil_div(ceil_div(P, chunks), kAdjPts) * kAdjPts;
+    pl.chunks = ceil_div(P, pl.chunk_pts);
+    return pl;
+}
+
+template <int D, typename GT>
+__global__ __launch_bounds__(kThreads) void k_adjoint_stage1(int64_t N, int64_t P, int64_t chunk_pts,
+                                                             const double* __restrict__ cov,
+                                                             const double* __restrict__ pos,
+                                                             const GT* __restrict__ g,
+                                                             double* __restrict__ part) {
+    constexpr bool F64 = sizeof(GT) == 8;
+    constexpr int Q = 2 + 2 * D;
+    __shared__ double s_x[kAdjPts * D];
+    __shared__ GT s_g[kAdjPts * (D + 1)];  // g, g x_0 .. g x_{D-1}
+    __shared__ unsigned long long s_xmax;
+
+    const int64_t j = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+    const bool active = j < N;
+    double k[D];
+    double k1 = 0.0;
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+        k[d] = active ? cov[d * N + j] : 0.0;
+        if (!F64) k[d] *= kInv2Pi;
+        k1 += fabs(k[d]);
+    }
+    double acc[Q];
+#pragma unroll
+    for (int q = 0; q < Q; ++q) acc[q] = 0.0;
+
+    const int64_t p0 = (int64_t)blockIdx.y * chunk_pts;
+    const int64_t p1 = (p0 + chunk_pts < P) ? p0 + chunk_pts : P;
+    for (int64_t s0 = p0; s0 < p1; s0 += kAdjPts) {
+        const int cnt = (int)((p1 - s0) < kAdjPts ? (p1 - s0) : kAdjPts);
+        __syncthreads();
+        if (threadIdx.x == 0) s_xmax = 0ull;
+        __syncthreads();
+        if ((int)threadIdx.x < cnt) {
+            const int64_t i = s0 + threadIdx.x;
+            const GT gi = g[i];
+            double xm = 0.0;
+            s_g[threadIdx.x * (D + 1)] = gi;
+#pragma unroll
+            for (int d = 0; d < D; ++d) {
+                const double xv = pos[d * P + i];
+                s_x[threadIdx.x * D + d] = xv;
+                s_g[threadIdx.x * (D + 1) + 1 + d] = (GT)(gi * xv);
+                xm = fmax(xm, fabs(xv));
+            }
+            atomicMax(&s_xmax, (unsigned long long)__double_as_longlong(xm));
+        }
+        __syncthreads();
+        if constexpr (F64) {
+            const double xmax = __longlong_as_double((long long)s_xmax);
+            if (__all(k1 * xmax <= 0x1.0p+29)) {
+                for (int m = 0; m < cnt; ++m) {
+                    double x[D];
+#pragma unroll
+                    for (int d = 0; d < D; ++d) x[d] = s_x[m * D + d];
+                    double sn, cs;
+                    sincos_fast(phase_exact<D>(k, x), &sn, &cs);
+                    const GT* gg = s_g + m * (D + 1);
+                    acc[0] = fma_(gg[0], cs, acc[0]);
+                    acc[1] = fma_(gg[0], sn, acc[1]);
+#pragma unroll
+                    for (int d = 0; d < D; ++d) {
+                        acc[2 + d] = fma_(gg[1 + d], cs, acc[2 + d]);
+                        acc[2 + D + d] = fma_(gg[1 + d], sn, acc[2 + D + d]);
+                    }
+                }
+            } else {
+                for (int m = 0; m < cnt; ++m) {
+                    double x[D];
+                    for (int d = 0; d < D; ++d) x[d] = s_x[m * D + d];
+                    const double th = phase_exact<D>(k, x);
+                    double sn, cs;
+                    if (fabs(th) < kFastPhaseMax) {
+                        sincos_fast(th, &sn, &cs);
+                    } else {
+                        sincos(th, &sn, &cs);
+                    }
+                    const GT* gg = s_g + m * (D + 1);
+                    acc[0] = fma_(gg[0], cs, acc[0]);
+                    acc[1] = fma_(gg[0], sn, acc[1]);
+                    for (int d = 0; d < D; ++d) {
+                        acc[2 + d] = fma_(gg[1 + d], cs, acc[2 + d]);
+                        acc[2 + D + d] = fma_(gg[1 + d], sn, acc[2 + D + d]);
+                    }
+                }
+            }
+        } else {
+            float pa[Q];
+#pragma unroll
+            for (int q = 0; q < Q; ++q) pa[q] = 0.0f;
+            for (int m = 0; m < cnt; ++m) {
+                double rev = 0.0;
+#pragma unroll
+                for (int d = 0; d < D; ++d) rev = fma_(k[d], s_x[m * D + d], rev);
+                const float f = (float)__builtin_amdgcn_fract(rev);
+                const float cs = __builtin_amdgcn_cosf(f);
+                const float sn = __builtin_amdgcn_sinf(f);
+                const GT* gg = s_g + m * (D + 1);
+                pa[0] = __builtin_fmaf(gg[0], cs, pa[0]);
+                pa[1] = __builtin_fmaf(gg[0], sn, pa[1]);
+#pragma unroll
+                for (int d = 0; d < D; ++d) {
+                    pa[2 + d] = __builtin_fmaf(gg[1 + d], cs, pa[2 + d]);
+                    pa[2 + D + d] = __builtin_fmaf(gg[1 + d], sn, pa[2 + D + d]);
+                }
+            }
+#pragma unroll
+            for (int q = 0; q < Q; ++q) acc[q] += (double)pa[q];
+        }
+    }
+    if (active) {
+        double* dst = part + (int64_t)blockIdx.y * Q * N;
+#pragma unroll
+        for (int q = 0; q < Q; ++q) dst[q * N + j] = acc[q];
+    }
+}
+
+template <int D>
+__global__ __launch_bounds__(kThreads) void k_adjoint_stage2(int64_t N, int64_t chunks,
+                                                             const double* __restrict__ z1,
+                                                             const double* __restrict__ z2,
+                                                             const double* __restrict__ part,
+                                                             double* __restrict__ gz1,
+                                                             double* __restrict__ gz2,
+                                                             double* __restrict__ gk) {
+    constexpr int Q = 2 + 2 * D;
+    const int64_t j = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+    if (j >= N) return;
+    double acc[Q];
+#pragma unroll
+    for (int q = 0; q < Q; ++q) acc[q] = 0.0;
+    for (int64_t c = 0; c < chunks; ++c) {
+        const double* src = part + c * Q * N;
+#pragma unroll
+        for (int q = 0; q < Q; ++q) acc[q] += src[q * N + j];
+    }
+    gz1[j] = acc[0];
+    gz2[j] = acc[1];
+    const double a1 = z1[j], a2 = z2[j];
+#pragma unroll
+    for (int d = 0; d < D; ++d) gk[d * N + j] = a2 * acc[2 + d] - a1 * acc[2 + D + d];
+}
+
+template <int D, typename GT>
+int adjoint_launch(int64_t N, int64_t P, const double* cov, const double* z1, const double* z2,
+                   const double* pos, const GT* g, double* gz1, double* gz2, double* gk, void* ws,
+                   hipStream_t s) {
+    const AdjPlan pl = adj_plan(N, P);
+    double* part = static_cast<double*>(ws);
+    hipLaunchKernelGGL((k_adjoint_stage1<D, GT>), dim3((unsigned)pl.mode_tiles, (unsigned)pl.chunks),
+                       dim3(kThreads), 0, s, N, P, pl.chunk_pts, cov, pos, g, part);
+    int st = launch_status();
+    if (st != GSX_OK) return st;
+    hipLaunchKernelGGL((k_adjoint_stage2<D>), dim3((unsigned)pl.mode_tiles), dim3(kThreads), 0, s, N,
+                       pl.chunks, z1, z2, part, gz1, gz2, gk);
+    return launch_status();
+}
+
+// Fills the gradients with zeros when there are no points.
+__global__ void k_zero(double* __restrict__ p, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0.0;
+}
+
+template <typename GT>
+int adjoint_entry(int64_t d, int64_t N, int64_t P, const double* cov, const double* z1,
+                  const double* z2, const double* pos, const GT* g, double* gz1, double* gz2,
+                  double* gk, void* ws, int64_t ws_bytes, void* stream) {
+    if (d < 1 || d > 3 || N < 0 || P < 0) return GSX_E_INVALID;
+    if (N == 0) return GSX_OK;
+    if (!cov || !z1 || !z2 || !gz1 || !gz2 || !gk) return GSX_E_INVALID;
+    hipStream_t s = as_stream(stream);
+    if (P == 0) {
+        const unsigned b = (unsigned)ceil_div(N, kThreads);
+        hipLaunchKernelGGL(k_zero, dim3(b), dim3(kThreads), 0, s, gz1, N);
+        hipLaunchKernelGGL(k_zero, dim3(b), dim3(kThreads), 0, s, gz2, N);
+        hipLaunchKernelGGL(k_zero, dim3((unsigned)ceil_div(d * N, kThreads)), dim3(kThreads), 0, s, gk, d * N);
+        return launch_status();
+    }
+    if (!pos || !g) return GSX_E_INVALID;
+    const int64_t need = gsx_summate_adjoint_workspace(d, N, P);
+    if (!ws || ws_bytes < need) return GSX_E_WORKSPACE;
+    switch (d) {
+        case 1: return adjoint_launch<1>(N, P, cov, z1, z2, pos, g, gz1, gz2, gk, ws, s);
+        case 2: return adjoint_launch<2>(N, P, cov, z1, z2, pos, g, gz1, gz2, gk, ws, s);
+        default: return adjoint_launch<3>(N, P, cov, z1, z2, pos, g, gz1, gz2, gk, ws, s);
+    }
+}
+
+}  // namespace gsx
+
+extern "C" {
+
+int64_t gsx_summate_adjoint_workspace(int64_t d, int64_t N, int64_t P) {
+    if (d < 1 || d > 3 || N < 0 || P < 0) return GSX_E_INVALID;
+    if (N == 0 || P == 0) return 0;
+    const gsx::AdjPlan pl = gsx::adj_plan(N, P);
+    return pl.chunks * (2 + 2 * d) * N * (int64_t)sizeof(double);
+}
+
+int gsx_summate_adjoint_f64(int64_t d, int64_t N, int64_t P, const double* cov, const double* z1,
+                            const double* z2, const double* pos, const double* g, double* gz1,
+                            double* gz2, double* gk, void* workspace, int64_t workspace_bytes,
+                            void* stream) {
+    return gsx::adjoint_entry<double>(d, N, P, cov, z1, z2, pos, g, gz1, gz2, gk, workspace,
+                                      workspace_bytes, stream);
+}
+
+int gsx_summate_adjoint_f32(int64_t d, int64_t N, int64_t P, const double* cov, const double* z1,
+                            const double* z2, const double* pos, const float* g, double* gz1,
+                            double* gz2, double* gk, void* workspace, int64_t workspace_bytes,
+                            void* stream) {
+    return gsx::adjoint_entry<float>(d, N, P, cov, z1, z2, pos, g, gz1, gz2, gk, workspace,
+                                     workspace_bytes, stream);
+}
+
+}  // extern "C"

@@ -1,0 +1,330 @@
+"""Randomisation-method generators whose summation runs on the GPU.
+
+:class:`RandMeth` and :class:`IncomprRandMeth` mirror the reference classes
+(``src/gstools/field/generator.py:165-446`` and ``:449-592``): same
+constructor keywords, same ``update`` / ``reset_seed`` / ``seed`` /
+``mode_no`` / ``sampling`` / ``model`` semantics, same private mode arrays
+``_cov_sample`` / ``_z_1`` / ``_z_2`` drawn from the same ``RNG`` streams,
+and the same scaling ``sqrt(var / mode_no) * summed + nugget``.  The sum
+itself is the gfx950 kernel (``gstools_amd.summate``).
+
+Plugging into the reference: when ``gstools`` is importable the classes
+derive from its ``Generator`` ABC, so ``gstools.SRF(model,
+generator=RandMeth, ...)`` accepts them unchanged (the seam at
+``src/gstools/field/srf.py:169-186``).  Models may be the reference's
+``CovModel`` objects or the ones in :mod:`gstools_amd.covmodel`.
+
+Extra keyword (not in the reference): ``precision`` -- ``"fp64"`` (default,
+matches the reference CPU sum to 1e-12) or ``"fp32"``.
+"""
+
+import warnings
+from abc import ABC, abstractmethod
+from copy import deepcopy as dcp
+
+import numpy as np
+import torch
+
+from .rng import RNG
+from .summate import (
+    GRID_PRECISIONS,
+    PRECISIONS,
+    summate_device,
+    summate_grid_device,
+    summate_incompr_device,
+)
+
+try:  # register with the reference's plugin seam when it is installed
+    from gstools.field.generator import Generator as _RefGenerator  # type: ignore
+except Exception:  # pragma: no cover - the reference is not importable here
+    _RefGenerator = None
+
+__all__ = ["Generator", "RandMeth", "IncomprRandMeth", "SAMPLING"]
+
+SAMPLING = ["auto", "inversion", "mcmc"]
+
+
+def _is_covmodel(model):
+    if model is None:
+        return False
+    try:
+        from gstools.covmodel.base import CovModel as _RefCov  # type: ignore
+
+        if isinstance(model, _RefCov):
+            return True
+    except Exception:
+        pass
+    from .covmodel import CovModel
+
+    return isinstance(model, CovModel)
+
+
+class _GeneratorBase(ABC):
+    """Abstract generator (``generator.py:78-162``)."""
+
+    @abstractmethod
+    def __init__(self, model, **kwargs):
+        pass
+
+    @abstractmethod
+    def update(self, model=None, seed=np.nan):
+        """Update the model and the seed."""
+
+    @abstractmethod
+    def get_nugget(self, shape):
+        """Normal nugget noise of the given shape."""
+
+    @abstractmethod
+    def __call__(self, pos, add_nugget=True):
+        """Generate the field."""
+
+    @property
+    @abstractmethod
+    def model(self):
+        """Covariance model."""
+
+    @property
+    def zero_var(self):
+        return np.isclose(self.model.var, 0)
+
+    @property
+    @abstractmethod
+    def value_type(self):
+        """Type of the field values (scalar, vector)."""
+
+    @property
+    def name(self):
+        return self.__class__.__name__
+
+
+Generator = _RefGenerator if _RefGenerator is not None else _GeneratorBase
+
+
+class RandMeth(Generator):
+    """Randomisation method for isotropic random fields (``generator.py:165-446``)."""
+
+    def __init__(self, model, *, mode_no=1000, seed=None, sampling="auto", precision="fp64", **kwargs):
+        if kwargs:
+            warnings.warn("gstools.RandMeth: **kwargs are ignored")
+        self._mode_no = int(mode_no)
+        self._model = None
+        self._seed = None
+        self._rng = None
+        self._z_1 = None
+        self._z_2 = None
+        self._cov_sample = None
+        self._value_type = "scalar"
+        self._sampling = None
+        self._dev_modes = None
+        self.sampling = sampling
+        self.precision = precision
+        self.update(model, seed)
+
+    # --------------------------------------------------------------- call --
+    def _device_modes(self):
+        dev = torch.device("cuda", torch.cuda.current_device())
+        key = (id(self._cov_sample), id(self._z_1), id(self._z_2), dev)
+        if self._dev_modes is None or self._dev_modes[0] != key:
+            tens = tuple(torch.from_numpy(np.ascontiguousarray(a, dtype=np.float64)).to(dev)
+                         for a in (self._cov_sample, self._z_1, self._z_2))
+            self._dev_modes = (key, tens)
+        return self._dev_modes[1]
+
+    def _scale(self):
+        return np.sqrt(self.model.var / self._mode_no)
+
+    def __call__(self, pos, add_nugget=True):
+        """Field at ``pos`` (d, n): ``sqrt(var / N) * summate(...) + nugget``.
+
+        Mirrors ``generator.py:243-270``.  numpy input gives a float64 numpy
+        result; a torch CUDA tensor gives a tensor on the device.
+        """
+        as_tensor = isinstance(pos, torch.Tensor)
+        if not as_tensor:
+            pos = np.asarray(pos, dtype=np.double)
+        if self.zero_var:
+            shp = tuple(pos.shape[1:])
+            return self.get_nugget(shp) if add_nugget else np.full(shp, 0.0)
+        cov, z1, z2 = self._device_modes()
+        summed = summate_device(cov, z1, z2, pos, precision=self.precision)
+        field = summed.to(torch.float64) * float(self._scale())
+        if not as_tensor:
+            field = field.cpu().numpy()
+        nugget = self.get_nugget(tuple(field.shape)) if add_nugget else 0.0
+        if as_tensor and not np.isscalar(nugget):
+            nugget = torch.as_tensor(nugget, device=field.device)
+        return field + nugget
+
+    def structured(self, axes, precision=None, add_nugget=True):
+        """Field on the 'ij' meshgrid of ``axes`` without materialising it.
+
+        ``precision`` may additionally be ``"bf16x3"`` or ``"fp16x3"`` (matrix
+        cores, d = 2 or 3).  The positions must already be isotropic (no
+        rotation or anisotropy left); returns a torch tensor shaped like the
+        grid.
+        """
+        precision = precision or self.precision
+        if precision not in GRID_PRECISIONS:
+            raise ValueError(f"precision must be one of {GRID_PRECISIONS}")
+        shape = tuple(int(np.size(a)) for a in axes)
+        if self.zero_var:
+            out = torch.zeros(shape, dtype=torch.float64, device="cuda")
+        else:
+            cov, z1, z2 = self._device_modes()
+            if precision in ("bf16x3", "fp16x3"):
+                out = summate_grid_device(cov, z1, z2, axes, precision, alpha=float(self._scale()))
+            else:
+                out = summate_grid_device(cov, z1, z2, axes, precision).to(torch.float64) * float(self._scale())
+            out = out.reshape(shape)
+        if add_nugget and self.model.nugget > 0:
+            out = out + torch.as_tensor(self.get_nugget(shape), device=out.device, dtype=out.dtype)
+        return out
+
+    def get_nugget(self, shape):
+        """Nugget noise (``generator.py:272-292``)."""
+        if self.model.nugget > 0:
+            nugget = np.sqrt(self.model.nugget) * self._rng.random.normal(size=shape)
+        else:
+            nugget = np.full(shape, 0.0) if self.zero_var else 0.0
+        return nugget
+
+    # -------------------------------------------------------------- modes --
+    def update(self, model=None, seed=np.nan):
+        """Update model and seed (``generator.py:294-344``)."""
+        if _is_covmodel(model):
+            if self.model != model:
+                self._model = dcp(model)
+                if seed is None or not np.isnan(seed):
+                    self.reset_seed(seed)
+                else:
+                    self.reset_seed(self._seed)
+            elif seed is None or not np.isnan(seed):
+                self.seed = seed
+        elif model is None and (seed is None or not np.isnan(seed)):
+            if _is_covmodel(self._model):
+                self.seed = seed
+            else:
+                raise ValueError("gstools.field.generator.RandMeth: no 'model' given")
+        elif model is None and np.isnan(seed):
+            if not (_is_covmodel(self._model) and self._z_1 is not None
+                    and self._z_2 is not None and self._cov_sample is not None):
+                raise ValueError("gstools.field.generator.RandMeth: neither 'model' nor 'seed' given!")
+        else:
+            raise ValueError("gstools.field.generator.RandMeth: 'model' is not an instance of 'gstools.CovModel'")
+
+    def reset_seed(self, seed=np.nan):
+        """Redraw z_1, z_2 and the wavenumbers (``generator.py:346-387``)."""
+        if seed is None or not np.isnan(seed):
+            self._seed = seed
+        self._rng = RNG(self._seed)
+        self._z_1 = self._rng.random.normal(size=self._mode_no)
+        self._z_2 = self._rng.random.normal(size=self._mode_no)
+        if self.zero_var:
+            self._cov_sample = np.full((self.model.dim, self._mode_no), 0.0)
+        else:
+            sph_crd = self._rng.sample_sphere(self.model.dim, self._mode_no)
+            if self.sampling == "inversion" or (self.sampling == "auto" and self.model.has_ppf):
+                pdf, cdf, ppf = self.model.dist_func
+                rad = self._rng.sample_dist(size=self._mode_no, pdf=pdf, cdf=cdf, ppf=ppf, a=0)
+            else:
+                rad = self._rng.sample_ln_pdf(
+                    ln_pdf=self.model.ln_spectral_rad_pdf,
+                    size=self._mode_no,
+                    sample_around=1.0 / self.model.len_rescaled,
+                )
+            self._cov_sample = rad * sph_crd
+        self._dev_modes = None
+
+    @property
+    def sampling(self):
+        return self._sampling
+
+    @sampling.setter
+    def sampling(self, sampling):
+        if sampling not in SAMPLING:
+            raise ValueError(f"RandMeth: sampling not in {SAMPLING}.")
+        self._sampling = sampling
+
+    @property
+    def precision(self):
+        return self._precision
+
+    @precision.setter
+    def precision(self, precision):
+        if precision not in PRECISIONS:
+            raise ValueError(f"RandMeth: precision not in {PRECISIONS}.")
+        self._precision = precision
+
+    @property
+    def seed(self):
+        return self._seed
+
+    @seed.setter
+    def seed(self, new_seed):
+        if new_seed is not self._seed:
+            self.reset_seed(new_seed)
+
+    @property
+    def model(self):
+        return self._model
+
+    @model.setter
+    def model(self, model):
+        self.update(model)
+
+    @property
+    def mode_no(self):
+        return self._mode_no
+
+    @mode_no.setter
+    def mode_no(self, mode_no):
+        if int(mode_no) != self._mode_no:
+            self._mode_no = int(mode_no)
+            self.reset_seed(self._seed)
+
+    @property
+    def value_type(self):
+        return self._value_type
+
+    def __repr__(self):
+        return f"{self.name}(model={self.model}, mode_no={self._mode_no}, seed={self.seed})"
+
+
+class IncomprRandMeth(RandMeth):
+    """Incompressible vector fields (``generator.py:449-592``)."""
+
+    def __init__(self, model, *, mean_velocity=1.0, mode_no=1000, seed=None, sampling="auto",
+                 precision="fp64", **kwargs):
+        if model.dim < 2 or model.dim > 3:
+            raise ValueError("Only 2D and 3D incompressible fields can be generated.")
+        super().__init__(model=model, mode_no=mode_no, seed=seed, sampling=sampling,
+                         precision=precision, **kwargs)
+        self.mean_u = mean_velocity
+        self._value_type = "vector"
+
+    def __call__(self, pos, add_nugget=True):
+        """``mean_u e1 + mean_u sqrt(var / N) summate_incompr + nugget`` (``:529-567``)."""
+        as_tensor = isinstance(pos, torch.Tensor)
+        if not as_tensor:
+            pos = np.asarray(pos, dtype=np.double)
+        nugget = self.get_nugget(tuple(pos.shape)) if add_nugget else 0.0
+        e1 = self._create_unit_vector(tuple(pos.shape))
+        if self.zero_var:
+            return self.mean_u * e1 + nugget
+        cov, z1, z2 = self._device_modes()
+        summed = summate_incompr_device(cov, z1, z2, pos, precision=self.precision).to(torch.float64)
+        scale = self.mean_u * np.sqrt(self.model.var / self._mode_no)
+        if as_tensor:
+            e1t = torch.as_tensor(e1, device=summed.device)
+            out = self.mean_u * e1t + scale * summed
+            if not np.isscalar(nugget):
+                nugget = torch.as_tensor(nugget, device=summed.device)
+            return out + nugget
+        return self.mean_u * e1 + scale * summed.cpu().numpy() + nugget
+
+    def _create_unit_vector(self, broadcast_shape, axis=0):
+        shape = np.ones(len(broadcast_shape), dtype=int)
+        shape[0] = self.model.dim
+        e1 = np.zeros(shape)
+        e1[axis] = 1.0
+        return e1

@@ -1,0 +1,70 @@
+"""Thin spatial-random-field caller around the GPU generators.
+
+Mirrors the parts of ``SRF.__call__`` (``src/gstools/field/srf.py:109-163``)
+and ``Field.pre_pos`` (``src/gstools/field/base.py:254-297``) that sit on the
+summation path: generator update, meshgrid of structured positions,
+isometrize, generator call, reshape, and a constant (scalar or per-component)
+mean as ``apply_mean_norm_trend`` adds it (``normalizer/tools.py:35-110``).
+Normalizers, trends, upscaling and field storage are not restated.
+
+Structured grids with no rotation and no anisotropy skip the meshgrid and use
+the grid kernels; pass ``precision="bf16x3"`` / ``"fp16x3"`` to run them on the
+matrix cores.
+"""
+
+import numpy as np
+import torch
+
+from .generator import IncomprRandMeth, RandMeth
+from .geometry import generate_grid
+
+__all__ = ["SRF", "GENERATOR"]
+
+GENERATOR = {"RandMeth": RandMeth, "IncomprRandMeth": IncomprRandMeth, "VectorField": IncomprRandMeth}
+
+
+class SRF:
+    """Spatial random field on the GPU generators."""
+
+    def __init__(self, model, mean=0.0, generator="RandMeth", **generator_kwargs):
+        self.model = model
+        self.mean = mean
+        gen = GENERATOR.get(generator, generator) if isinstance(generator, str) else generator
+        if not isinstance(gen, type):
+            raise ValueError(f"gstools.SRF: Unknown or wrong generator: {generator}")
+        self.generator = gen(model, **generator_kwargs)
+        self.value_type = self.generator.value_type
+        self.field = None
+
+    def __call__(self, pos, seed=np.nan, mesh_type="unstructured", precision=None):
+        self.generator.update(self.model, seed)
+        dim = self.model.dim
+        if mesh_type != "unstructured":
+            axes = [np.asarray(a, dtype=np.double).reshape(-1) for a in pos]
+            if len(axes) != dim:
+                raise ValueError("SRF: position tuple does not match the model dimension")
+            shape = tuple(a.size for a in axes)
+            iso_free = np.allclose(self.model.anis, 1.0) and np.allclose(self.model.angles, 0.0)
+            if iso_free and self.value_type == "scalar":
+                field = self.generator.structured(axes, precision=precision).cpu().numpy()
+                return self._post(np.asarray(field, dtype=np.double))
+            pos = generate_grid(axes)
+        else:
+            pos = np.asarray(pos, dtype=np.double).reshape(dim, -1)
+            shape = (pos.shape[1],)
+        iso = self.model.isometrize(pos)
+        field = self.generator(iso)
+        if self.value_type == "vector":
+            field = np.reshape(field, (dim,) + shape)
+        else:
+            field = np.reshape(field, shape)
+        return self._post(field)
+
+    def _post(self, field):
+        mean = np.asarray(self.mean, dtype=np.double)
+        if mean.size == 1:
+            field = field + mean.item()
+        else:
+            field = field + mean.reshape((-1,) + (1,) * (field.ndim - 1))
+        self.field = field
+        return field

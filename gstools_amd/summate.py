@@ -1,0 +1,216 @@
+"""Drop-in summation functions backed by the gfx950 kernels.
+
+``summate`` and ``summate_incompr`` keep the signature and semantics of the
+functions GSTools imports from its native backend
+(``src/gstools/field/generator.py:22-24``, dispatched at ``:42-64``):
+float64 numpy inputs ``cov_samples (d, N)``, ``z_1 (N,)``, ``z_2 (N,)``,
+``pos (d, P)``, a fresh float64 result, ``ValueError`` on bad shapes or
+dtypes, and a ``num_threads`` argument that the GPU path accepts and ignores.
+
+The ``*_device`` functions take and return torch tensors that stay on the
+GPU (numpy inputs are copied over); they add the FP32, structured-grid,
+matrix-core and adjoint paths.  Nothing here falls back to the CPU: without a
+HIP device or without ``libgsx.so`` every call raises ``RuntimeError``.
+"""
+
+import ctypes
+
+import numpy as np
+import torch
+
+from . import _native
+
+__all__ = [
+    "summate", "summate_incompr", "summate_device", "summate_incompr_device",
+    "summate_grid_device", "summate_adjoint_device", "PRECISIONS", "GRID_PRECISIONS",
+]
+
+PRECISIONS = ("fp64", "fp32")
+GRID_PRECISIONS = ("fp64", "fp32", "bf16x3", "fp16x3")
+
+
+def _device():
+    if not torch.cuda.is_available():
+        raise RuntimeError("gstools_amd: no HIP device is visible (the summation runs only on the GPU)")
+    return torch.device("cuda", torch.cuda.current_device())
+
+
+def _stream():
+    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def _ptr(t):
+    return ctypes.c_void_p(t.data_ptr()) if t is not None and t.numel() else ctypes.c_void_p(0)
+
+
+def _as_f64(x, name, dev):
+    if isinstance(x, torch.Tensor):
+        if x.dtype != torch.float64:
+            raise ValueError(f"{name}: expected float64, got {x.dtype}")
+        return x.to(dev).contiguous()
+    arr = np.asarray(x)
+    if arr.dtype != np.float64:
+        if not np.issubdtype(arr.dtype, np.number):
+            raise ValueError(f"{name}: expected a float64 array, got {arr.dtype}")
+        arr = arr.astype(np.float64)
+    return torch.from_numpy(np.ascontiguousarray(arr)).to(dev)
+
+
+def _modes(cov_samples, z_1, z_2, dev):
+    cov = _as_f64(cov_samples, "cov_samples", dev)
+    if cov.dim() == 1:
+        cov = cov.reshape(1, -1)
+    if cov.dim() != 2:
+        raise ValueError(f"cov_samples: expected shape (d, N), got {tuple(cov.shape)}")
+    z1 = _as_f64(z_1, "z_1", dev).reshape(-1)
+    z2 = _as_f64(z_2, "z_2", dev).reshape(-1)
+    d, n = cov.shape
+    if z1.numel() != n or z2.numel() != n:
+        raise ValueError(f"z_1/z_2 must have N={n} entries, got {z1.numel()} and {z2.numel()}")
+    if not 1 <= d <= 3:
+        raise ValueError(f"only 1, 2 or 3 dimensions are supported, got d={d}")
+    return cov, z1, z2
+
+
+def _points(pos, d, dev):
+    p = _as_f64(pos, "pos", dev)
+    if p.dim() == 1:
+        p = p.reshape(1, -1)
+    if p.dim() != 2 or p.shape[0] != d:
+        raise ValueError(f"pos: expected shape ({d}, P), got {tuple(p.shape)}")
+    return p
+
+
+def summate_device(cov_samples, z_1, z_2, pos, precision="fp64", out=None):
+    """Unscaled sum on the GPU -> torch tensor (P,) float64 or float32."""
+    dev = _device()
+    cov, z1, z2 = _modes(cov_samples, z_1, z_2, dev)
+    d, n = cov.shape
+    p = _points(pos, d, dev)
+    npts = p.shape[1]
+    if precision not in PRECISIONS:
+        raise ValueError(f"precision must be one of {PRECISIONS}")
+    dtype = torch.float64 if precision == "fp64" else torch.float32
+    if out is None:
+        out = torch.empty(npts, dtype=dtype, device=dev)
+    elif out.dtype != dtype or out.numel() != npts or not out.is_contiguous():
+        raise ValueError("out: wrong dtype, size or layout")
+    fn = _native.lib().gsx_summate_f64 if precision == "fp64" else _native.lib().gsx_summate_f32
+    _native.check(fn(d, n, npts, _ptr(cov), _ptr(z1), _ptr(z2), _ptr(p), _ptr(out), _stream()), "summate")
+    return out
+
+
+def summate_incompr_device(cov_samples, z_1, z_2, pos, precision="fp64", out=None):
+    """Unscaled incompressible sum on the GPU -> torch tensor (d, P)."""
+    dev = _device()
+    cov, z1, z2 = _modes(cov_samples, z_1, z_2, dev)
+    d, n = cov.shape
+    if d < 2:
+        raise ValueError("Only 2D and 3D incompressible fields can be generated.")
+    p = _points(pos, d, dev)
+    npts = p.shape[1]
+    if precision not in PRECISIONS:
+        raise ValueError(f"precision must be one of {PRECISIONS}")
+    dtype = torch.float64 if precision == "fp64" else torch.float32
+    if out is None:
+        out = torch.empty((d, npts), dtype=dtype, device=dev)
+    elif out.dtype != dtype or tuple(out.shape) != (d, npts) or not out.is_contiguous():
+        raise ValueError("out: wrong dtype, shape or layout")
+    fn = (_native.lib().gsx_summate_incompr_f64 if precision == "fp64"
+          else _native.lib().gsx_summate_incompr_f32)
+    _native.check(fn(d, n, npts, _ptr(cov), _ptr(z1), _ptr(z2), _ptr(p), _ptr(out), _stream()),
+                  "summate_incompr")
+    return out
+
+
+def summate_grid_device(cov_samples, z_1, z_2, axes, precision="fp64", alpha=1.0, out=None):
+    """Sum on the 'ij' meshgrid of ``axes`` without materialising it.
+
+    ``precision``: "fp64" / "fp32" (direct sums; "fp64" reproduces the phases
+    of the materialised grid bit for bit) or "bf16x3" / "fp16x3" (matrix-core
+    GEMM over per-axis phase factors, d = 2 or 3).  Returns a flat tensor of
+    prod(shape) values in C order, multiplied by ``alpha`` (matrix-core path
+    only; the direct paths return the unscaled sum and require alpha == 1).
+    """
+    dev = _device()
+    cov, z1, z2 = _modes(cov_samples, z_1, z_2, dev)
+    d, n = cov.shape
+    axes = [_as_f64(a, "axes", dev).reshape(-1) for a in axes]
+    if len(axes) != d:
+        raise ValueError(f"expected {d} axes, got {len(axes)}")
+    shape = [int(a.numel()) for a in axes]
+    npts = int(np.prod(shape))
+    flat = torch.cat(axes) if axes else torch.empty(0, dtype=torch.float64, device=dev)
+    shape_c = (ctypes.c_int64 * 3)(*(shape + [1] * (3 - d)))
+    if precision not in GRID_PRECISIONS:
+        raise ValueError(f"precision must be one of {GRID_PRECISIONS}")
+    dtype = torch.float64 if precision == "fp64" else torch.float32
+    if out is None:
+        out = torch.empty(npts, dtype=dtype, device=dev)
+    elif out.dtype != dtype or out.numel() != npts or not out.is_contiguous():
+        raise ValueError("out: wrong dtype, size or layout")
+    lib = _native.lib()
+    if precision in ("fp64", "fp32"):
+        if alpha != 1.0:
+            raise ValueError("alpha applies to the matrix-core path only")
+        fn = lib.gsx_summate_grid_f64 if precision == "fp64" else lib.gsx_summate_grid_f32
+        _native.check(fn(d, n, shape_c, _ptr(flat), _ptr(cov), _ptr(z1), _ptr(z2), _ptr(out), _stream()),
+                      "summate_grid")
+        return out
+    variant = _native.MFMA_BF16X3 if precision == "bf16x3" else _native.MFMA_FP16X3
+    ws_bytes = int(lib.gsx_summate_grid_mfma_workspace(d, n, shape_c))
+    if ws_bytes < 0:
+        _native.check(int(ws_bytes), "summate_grid_mfma")
+    ws = torch.empty(max(ws_bytes, 1), dtype=torch.uint8, device=dev)
+    _native.check(lib.gsx_summate_grid_mfma(d, n, shape_c, _ptr(flat), _ptr(cov), _ptr(z1), _ptr(z2),
+                                            variant, float(alpha), _ptr(out), _ptr(ws), ws_bytes,
+                                            _stream()), "summate_grid_mfma")
+    return out
+
+
+def summate_adjoint_device(cov_samples, z_1, z_2, pos, g, precision="fp64"):
+    """Unscaled adjoint: (gz1 (N,), gz2 (N,), gk (d, N)) float64 tensors."""
+    dev = _device()
+    cov, z1, z2 = _modes(cov_samples, z_1, z_2, dev)
+    d, n = cov.shape
+    p = _points(pos, d, dev)
+    npts = p.shape[1]
+    if precision not in PRECISIONS:
+        raise ValueError(f"precision must be one of {PRECISIONS}")
+    gdt = torch.float64 if precision == "fp64" else torch.float32
+    if isinstance(g, torch.Tensor):
+        gg = g.to(device=dev, dtype=gdt).contiguous().reshape(-1)
+    else:
+        gg = torch.from_numpy(np.ascontiguousarray(g, dtype=np.float64)).to(dev, gdt).reshape(-1)
+    if gg.numel() != npts:
+        raise ValueError(f"g: expected {npts} values, got {gg.numel()}")
+    gz1 = torch.empty(n, dtype=torch.float64, device=dev)
+    gz2 = torch.empty(n, dtype=torch.float64, device=dev)
+    gk = torch.empty((d, n), dtype=torch.float64, device=dev)
+    lib = _native.lib()
+    ws_bytes = int(lib.gsx_summate_adjoint_workspace(d, n, npts))
+    if ws_bytes < 0:
+        _native.check(int(ws_bytes), "summate_adjoint")
+    ws = torch.empty(max(ws_bytes, 1), dtype=torch.uint8, device=dev)
+    fn = lib.gsx_summate_adjoint_f64 if precision == "fp64" else lib.gsx_summate_adjoint_f32
+    _native.check(fn(d, n, npts, _ptr(cov), _ptr(z1), _ptr(z2), _ptr(p), _ptr(gg), _ptr(gz1), _ptr(gz2),
+                     _ptr(gk), _ptr(ws), ws_bytes, _stream()), "summate_adjoint")
+    return gz1, gz2, gk
+
+
+def _check_numpy(cov_samples, z_1, z_2, pos):
+    for name, arr in (("cov_samples", cov_samples), ("z_1", z_1), ("z_2", z_2), ("pos", pos)):
+        if isinstance(arr, np.ndarray) and arr.dtype != np.float64:
+            raise ValueError(f"Buffer dtype mismatch for {name}: expected 'double' but got '{arr.dtype}'")
+
+
+def summate(cov_samples, z_1, z_2, pos, num_threads=None):
+    """Drop-in for the backend ``summate``: FP64 sum on the GPU -> numpy (P,)."""
+    _check_numpy(cov_samples, z_1, z_2, pos)
+    return summate_device(cov_samples, z_1, z_2, pos, "fp64").cpu().numpy()
+
+
+def summate_incompr(cov_samples, z_1, z_2, pos, num_threads=None):
+    """Drop-in for the backend ``summate_incompr``: FP64 sum -> numpy (d, P)."""
+    _check_numpy(cov_samples, z_1, z_2, pos)
+    return summate_incompr_device(cov_samples, z_1, z_2, pos, "fp64").cpu().numpy()

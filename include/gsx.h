@@ -1,0 +1,160 @@
+/*
+ * gsx -- MI355X (gfx950) randomisation-method field summation, C ABI.
+ *
+ * Drop-in boundary for the native summation that GSTools' generators call:
+ *
+ *   reference                                   replaced by
+ *   ---------------------------------------------------------------------
+ *   summate(cov_samples, z_1, z_2, pos, nt)      gsx_summate_f64 / _f32
+ *     src/gstools/field/generator.py:22, :42-48, call :266-268
+ *   summate_incompr(cov_samples, z_1, z_2, pos)  gsx_summate_incompr_f64 / _f32
+ *     src/gstools/field/generator.py:24, :51-64, call :554-560
+ *   summate on a meshgrid from Field.pre_pos     gsx_summate_grid_f64 / _f32,
+ *     src/gstools/field/base.py:254-297 ->         gsx_summate_grid_mfma
+ *     tools/geometric.py:340-356
+ *   (no reference counterpart: gradient of        gsx_summate_adjoint_f64 / _f32
+ *    summate for calibration; formula is the
+ *    derivative of generator.py:193-199)
+ *   (no reference counterpart: many realizations  gsx_batch_*
+ *    in one launch; each item is one
+ *    RandMeth.__call__, generator.py:243-270)
+ *
+ * Conventions (all entry points):
+ *   - every array argument is a DEVICE pointer owned by the caller; inputs
+ *     are read-only, outputs are written in full;
+ *   - cov is (d, N) row-major, pos is (d, P) row-major, incompressible
+ *     outputs are (d, P) row-major -- the reference's numpy layouts;
+ *   - outputs are the UNSCALED sum (the reference's summate); callers apply
+ *     sqrt(var / N) as RandMeth.__call__ does (generator.py:270);
+ *   - work is enqueued on `stream` (a hipStream_t, 0 = null stream) and the
+ *     call returns without synchronising; no call allocates device memory;
+ *     calls on different streams are independent;
+ *   - the return value is GSX_OK (0) or a negative GSX_E_* status;
+ *     gsx_status_string() names it.
+ */
+#ifndef GSX_H
+#define GSX_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GSX_OK 0
+#define GSX_E_INVALID (-1)   /* bad dimension, size or null pointer       */
+#define GSX_E_LAUNCH (-2)    /* kernel launch or HIP runtime failure       */
+#define GSX_E_NO_DEVICE (-3) /* no usable HIP device                       */
+#define GSX_E_WORKSPACE (-4) /* workspace pointer null or too small        */
+#define GSX_E_UNSUPPORTED (-5) /* valid request this build does not cover  */
+
+/* MFMA split-precision variants for gsx_summate_grid_mfma. */
+#define GSX_MFMA_BF16X3 0 /* bf16 hi/lo split, hi*hi + hi*lo + lo*hi */
+#define GSX_MFMA_FP16X3 1 /* fp16 hi/lo split, hi*hi + hi*lo + lo*hi */
+
+const char* gsx_status_string(int status);
+int gsx_version(void);
+
+/* Plain FP64 summation, out (P,) float64 = sum_j z1 cos(k_j.x) + z2 sin(k_j.x).
+ * Phase computed as ((k0*x0) + k1*x1) + k2*x2 without fused multiply-add,
+ * like the reference loop.  Replaces summate (generator.py:42-48). d in 1..3. */
+int gsx_summate_f64(int64_t d, int64_t N, int64_t P, const double* cov,
+                    const double* z1, const double* z2, const double* pos,
+                    double* out, void* stream);
+
+/* FP32 summation of the same sum, out (P,) float32.  Phase in FP64 (range
+ * reduction is the precision-critical step), cosine and mode sum in FP32 with
+ * blocked FP32 -> FP64 accumulation.  Replaces summate where an FP32 field is
+ * acceptable (error <= 1e-5 relative to FP64). */
+int gsx_summate_f32(int64_t d, int64_t N, int64_t P, const double* cov,
+                    const double* z1, const double* z2, const double* pos,
+                    float* out, void* stream);
+
+/* Incompressible vector sum, out (d, P): projector e1 - k k0/|k|^2 per mode.
+ * Replaces summate_incompr (generator.py:51-64, formula :479-494).  d in 2..3. */
+int gsx_summate_incompr_f64(int64_t d, int64_t N, int64_t P, const double* cov,
+                            const double* z1, const double* z2,
+                            const double* pos, double* out, void* stream);
+int gsx_summate_incompr_f32(int64_t d, int64_t N, int64_t P, const double* cov,
+                            const double* z1, const double* z2,
+                            const double* pos, float* out, void* stream);
+
+/* Structured grid, direct sum: positions are the 'ij' meshgrid of the axes
+ * (generate_grid, tools/geometric.py:340-356) and are never materialised.
+ * shape[d] = axis lengths; axes = concatenated axis coordinates (sum(shape)
+ * doubles); out (prod(shape),) in C order.  The FP64 form gives the same
+ * phases, bit for bit, as gsx_summate_f64 on the materialised meshgrid. */
+int gsx_summate_grid_f64(int64_t d, int64_t N, const int64_t* shape,
+                         const double* axes, const double* cov,
+                         const double* z1, const double* z2, double* out,
+                         void* stream);
+int gsx_summate_grid_f32(int64_t d, int64_t N, const int64_t* shape,
+                         const double* axes, const double* cov,
+                         const double* z1, const double* z2, float* out,
+                         void* stream);
+
+/* Structured grid on the matrix cores (d = 2 or 3): per-axis phase factors
+ * e^{i k_j x} turn the sum into a real GEMM with K = 2N per z-plane, run with
+ * split bf16/fp16 products accumulated in FP32.  `shape` and `axes` as above;
+ * cov may already include a folded isometrize matrix (k' = M^T k).  out is
+ * float32, scaled by `alpha` (pass 1.0 for the unscaled sum).  Workspace
+ * bytes from gsx_summate_grid_mfma_workspace(). */
+int64_t gsx_summate_grid_mfma_workspace(int64_t d, int64_t N, const int64_t* shape);
+int gsx_summate_grid_mfma(int64_t d, int64_t N, const int64_t* shape,
+                          const double* axes, const double* cov,
+                          const double* z1, const double* z2, int variant,
+                          double alpha, float* out, void* workspace,
+                          int64_t workspace_bytes, void* stream);
+
+/* Adjoint of summate for an upstream gradient g (P,):
+ *   gz1[j]   = sum_i g_i cos(k_j.x_i)
+ *   gz2[j]   = sum_i g_i sin(k_j.x_i)
+ *   gk[d][j] = sum_i g_i x_id (z2_j cos(k_j.x_i) - z1_j sin(k_j.x_i))
+ * (unscaled, like the forward).  Deterministic two-stage reduction over
+ * points.  Workspace bytes from gsx_summate_adjoint_workspace(). */
+int64_t gsx_summate_adjoint_workspace(int64_t d, int64_t N, int64_t P);
+int gsx_summate_adjoint_f64(int64_t d, int64_t N, int64_t P, const double* cov,
+                            const double* z1, const double* z2,
+                            const double* pos, const double* g, double* gz1,
+                            double* gz2, double* gk, void* workspace,
+                            int64_t workspace_bytes, void* stream);
+int gsx_summate_adjoint_f32(int64_t d, int64_t N, int64_t P, const double* cov,
+                            const double* z1, const double* z2,
+                            const double* pos, const float* g, double* gz1,
+                            double* gz2, double* gk, void* workspace,
+                            int64_t workspace_bytes, void* stream);
+
+/* Batched FP32 summation of many realizations in one persistent launch.
+ * Each item is one RandMeth / IncomprRandMeth call (generator.py:243-270,
+ * :529-567) with its own modes and points; see gsx_batch_item. */
+typedef struct gsx_batch_item {
+    int32_t d;          /* 1..3                                            */
+    int32_t kind;       /* 0 = scalar summate, 1 = incompressible (d >= 2) */
+    int64_t N;          /* modes                                           */
+    int64_t P;          /* points                                          */
+    int64_t shape[3];   /* grid axis lengths when axes != NULL             */
+    const double* cov;  /* (d, N)                                          */
+    const double* z1;   /* (N,)                                            */
+    const double* z2;   /* (N,)                                            */
+    const double* pos;  /* (d, P), or NULL when axes is given              */
+    const double* axes; /* concatenated grid axes, or NULL                 */
+    float* out;         /* (P,) or (d, P), unscaled                        */
+} gsx_batch_item;
+
+/* Build a plan for `count` items; tile = points per work tile (0 = auto).
+ * The plan owns a device copy of the descriptors and a tile queue. */
+typedef struct gsx_batch_plan gsx_batch_plan;
+int gsx_batch_plan_create(const gsx_batch_item* items, int64_t count,
+                          int64_t tile, gsx_batch_plan** plan);
+/* One persistent launch over every tile of every item. */
+int gsx_batch_plan_launch(gsx_batch_plan* plan, void* stream);
+/* Capture the launch into a HIP graph once, then replay it. */
+int gsx_batch_plan_launch_graph(gsx_batch_plan* plan, void* stream);
+int64_t gsx_batch_plan_tiles(const gsx_batch_plan* plan);
+void gsx_batch_plan_destroy(gsx_batch_plan* plan);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GSX_H */

@@ -1,0 +1,107 @@
+"""CPU checks of the native boundary: the library loads, exports every symbol
+``include/gsx.h`` declares, rejects bad arguments before touching a device,
+and its device math (built for the host) matches mpmath."""
+
+import ctypes
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+from gstools_amd import _native
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "gsx.h")
+
+
+def declared_functions():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(gsx_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    lib = _native.lib()
+    names = declared_functions()
+    assert len(names) >= 15
+    for name in names:
+        assert hasattr(lib, name), name
+    assert set(names) == set(_native.EXPORTS), "ctypes table and header disagree"
+
+
+def test_status_strings_and_version():
+    lib = _native.lib()
+    assert lib.gsx_version() >= 100
+    assert lib.gsx_status_string(0) == b"ok"
+    assert b"invalid" in lib.gsx_status_string(-1)
+
+
+def test_invalid_arguments_are_rejected_on_the_host():
+    lib = _native.lib()
+    null = ctypes.c_void_p(0)
+    # d out of range, negative sizes: rejected before any HIP call
+    assert lib.gsx_summate_f64(4, 10, 10, null, null, null, null, null, null) == -1
+    assert lib.gsx_summate_f32(0, 10, 10, null, null, null, null, null, null) == -1
+    assert lib.gsx_summate_f64(2, -1, 10, null, null, null, null, null, null) == -1
+    assert lib.gsx_summate_incompr_f64(1, 10, 10, null, null, null, null, null, null) == -1
+    # nothing to do: P == 0 returns ok without launching
+    assert lib.gsx_summate_f64(2, 10, 0, null, null, null, null, null, null) == 0
+    # null pointers with work to do
+    assert lib.gsx_summate_f64(2, 10, 10, null, null, null, null, null, null) == -1
+    assert lib.gsx_summate_adjoint_workspace(4, 1, 1) == -1
+    assert lib.gsx_summate_adjoint_workspace(2, 0, 5) == 0
+    assert lib.gsx_summate_adjoint_workspace(2, 2048, 10**6) > 0
+    shape = (ctypes.c_int64 * 3)(10, 10, 1)
+    assert lib.gsx_summate_grid_f64(2, 5, shape, null, null, null, null, null, null) == -1
+
+
+def test_python_boundary_validates_like_the_reference():
+    from gstools_amd import summate
+
+    cov = np.zeros((2, 3))
+    z = np.zeros(3)
+    with pytest.raises(ValueError):
+        summate(cov.astype(np.float32), z, z, np.zeros((2, 4)))
+    with pytest.raises((ValueError, RuntimeError)):
+        summate(cov, z, z, np.zeros((3, 4)))
+
+
+@pytest.fixture(scope="module")
+def devmath(tmp_path_factory):
+    out = tmp_path_factory.mktemp("devmath") / "devmath.so"
+    src = os.path.join(ROOT, "tests", "native", "devmath_host.cpp")
+    subprocess.run(["g++", "-O2", "-ffp-contract=off", "-shared", "-fPIC", "-o", str(out), src], check=True)
+    lib = ctypes.CDLL(str(out))
+    dp = ctypes.POINTER(ctypes.c_double)
+    lib.devmath_cos_shifted.argtypes = [dp, dp, dp, ctypes.c_long]
+    lib.devmath_sincos.argtypes = [dp, dp, dp, ctypes.c_long]
+    return lib
+
+
+def _p(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def test_device_trig_matches_mpmath(devmath):
+    import mpmath as mp
+
+    rs = np.random.RandomState(11)
+    n = 3000
+    x = np.concatenate([rs.uniform(-10, 10, n // 3), rs.uniform(-3e5, 3e5, n // 3),
+                        rs.uniform(-2**30 + 1, 2**30 - 1, n - 2 * (n // 3))])
+    x[:40] = (np.arange(40) - 20) * (np.pi / 2)  # near the reduction boundaries
+    phi = rs.uniform(-np.pi, np.pi, n)
+    cs = np.empty(n)
+    s = np.empty(n)
+    c = np.empty(n)
+    devmath.devmath_cos_shifted(_p(x), _p(phi), _p(cs), n)
+    devmath.devmath_sincos(_p(x), _p(s), _p(c), n)
+    with mp.workdps(40):
+        ref_cs = np.array([float(mp.cos(mp.mpf(a) - mp.mpf(b))) for a, b in zip(x, phi)])
+        ref_s = np.array([float(mp.sin(mp.mpf(a))) for a in x])
+        ref_c = np.array([float(mp.cos(mp.mpf(a))) for a in x])
+    assert np.max(np.abs(cs - ref_cs)) < 1e-15
+    assert np.max(np.abs(s - ref_s)) < 3e-16
+    assert np.max(np.abs(c - ref_c)) < 3e-16

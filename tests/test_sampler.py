@@ -1,0 +1,208 @@
+"""Known-answer tests of the host mode sampler, mirrored from the reference.
+
+The reference pins RandMeth end to end (sampler + summation + scaling) with
+fixed-seed values at 7 decimal places; these tests rebuild the same fields
+from this package's sampler and the oracle's summation:
+
+* ``tests/test_randmeth.py:33-71``        (1D/2D inversion, 3D MCMC, reseeding)
+* ``tests/test_incomprrandmeth.py:34-59`` (2D inversion, 3D MCMC, vector mean)
+* ``tests/test_rng.py:20-89``             (sub-stream normals, sphere samples)
+* ``tests/test_srf.py:259-275``           (incompressible SRF, un/structured)
+"""
+
+import copy
+
+import numpy as np
+import pytest
+
+import oracle
+from gstools_amd import RNG, Exponential, Gaussian, IncomprRandMeth, Matern, RandMeth, Stable
+from gstools_amd.geometry import generate_grid, matrix_isometrize
+
+SEED = 19031977
+X_T = np.linspace(0.0, 10.0, 10)
+Y_T = np.linspace(-5.0, 5.0, 10)
+Z_T = np.linspace(-6.0, 8.0, 10)
+
+
+def field(rm, pos):
+    pos = np.asarray(pos, dtype=np.double)
+    return np.sqrt(rm.model.var / rm.mode_no) * oracle.summate_c(rm._cov_sample, rm._z_1, rm._z_2, pos)
+
+
+def vector_field(rm, pos):
+    pos = np.asarray(pos, dtype=np.double)
+    e1 = np.zeros((rm.model.dim, 1))
+    e1[0] = 1.0
+    s = oracle.summate_incompr_c(rm._cov_sample, rm._z_1, rm._z_2, pos)
+    return rm.mean_u * e1 + rm.mean_u * np.sqrt(rm.model.var / rm.mode_no) * s
+
+
+@pytest.fixture
+def models():
+    m1 = Gaussian(dim=1, var=1.5, len_scale=3.5)
+    m2 = copy.deepcopy(m1)
+    m2.dim = 2
+    m3 = copy.deepcopy(m1)
+    m3.dim = 3
+    return m1, m2, m3
+
+
+def test_randmeth_unstruct_1d(models):
+    f = field(RandMeth(models[0], mode_no=100, seed=SEED), (X_T,))
+    assert f[0] == pytest.approx(3.19799030, abs=5e-8)
+    assert f[1] == pytest.approx(2.44848295, abs=5e-8)
+
+
+def test_randmeth_unstruct_2d(models):
+    f = field(RandMeth(models[1], mode_no=100, seed=SEED), (X_T, Y_T))
+    assert f[0] == pytest.approx(1.67318010, abs=5e-8)
+    assert f[1] == pytest.approx(2.12310269, abs=5e-8)
+
+
+def test_randmeth_unstruct_3d_mcmc(models):
+    # 3D Gaussian has no ppf: this is the MCMC (stretch move) path
+    f = field(RandMeth(models[2], mode_no=100, seed=SEED), (X_T, Y_T, Z_T))
+    assert f[0] == pytest.approx(1.3240234883187239, abs=1e-12)
+    assert f[1] == pytest.approx(1.6367244277732766, abs=1e-12)
+
+
+def test_randmeth_reset(models):
+    m1, m2, m3 = models
+    rm_1d = RandMeth(m1, mode_no=100, seed=SEED)
+    rm_2d = RandMeth(m2, mode_no=100, seed=SEED)
+    rm_2d.seed = rm_2d.seed
+    f = field(rm_2d, (X_T, Y_T))
+    assert f[:2] == pytest.approx([1.67318010, 2.12310269], abs=5e-8)
+    rm_2d.seed = 74893621
+    f = field(rm_2d, (X_T, Y_T))
+    assert f[:2] == pytest.approx([-1.94278053, -1.12401651], abs=5e-8)
+    rm_1d.model = m3
+    f = field(rm_1d, (X_T, Y_T, Z_T))
+    assert f[:2] == pytest.approx([1.3240234883187239, 1.6367244277732766], abs=1e-12)
+    rm_2d.mode_no = 800
+    f = field(rm_2d, (X_T, Y_T))
+    assert f[:2] == pytest.approx([-3.20809251, -2.62032778], abs=5e-8)
+
+
+def test_incompr_2d_3d():
+    m2 = Gaussian(dim=2, var=1.5, len_scale=2.5)
+    m3 = copy.deepcopy(m2)
+    m3.dim = 3
+    f = vector_field(IncomprRandMeth(m2, mode_no=100, seed=SEED), (X_T, Y_T))
+    assert [f[0, 0], f[0, 1], f[1, 1]] == pytest.approx([0.50751115, 1.03291018, -0.22003005], abs=5e-8)
+    f = vector_field(IncomprRandMeth(m3, mode_no=100, seed=SEED), (X_T, Y_T, Z_T))
+    assert [f[0, 0], f[0, 1], f[1, 0]] == pytest.approx(
+        [0.7924546333550331, 1.660747056686244, -0.28049855754819514], abs=1e-12)
+    with pytest.raises(ValueError):
+        IncomprRandMeth(Gaussian(dim=1, var=1.5, len_scale=2.5))
+
+
+def test_incompr_vector_mean():
+    """``test_incomprrandmeth.py:50-59``: SRF(mean=(0.5, 0), VectorField), structured."""
+    model = Gaussian(dim=2, var=1.5, len_scale=2.5)
+    rm = IncomprRandMeth(model, seed=198412031)
+    x = np.linspace(0.0, 10.0, 9)
+    y = np.linspace(-5.0, 5.0, 16)
+    f = vector_field(rm, model.isometrize(generate_grid((x, y)))).reshape(2, 9, 16)
+    f = f + np.array([0.5, 0.0]).reshape(2, 1, 1)
+    assert np.mean(f[0]) == pytest.approx(1.3025621393180298, abs=1e-10)
+    assert np.mean(f[1]) == pytest.approx(-0.04729596839446052, abs=1e-10)
+
+
+def test_srf_incompr_known_values():
+    """``test_srf.py:259-275``: IncomprRandMeth through SRF, mean 0.3."""
+    rs = np.random.RandomState(123018)
+    xt, yt = rs.uniform(0.0, 10, 100), rs.uniform(0.0, 10, 100)
+    model = Gaussian(dim=2, var=0.5, len_scale=1.0)
+    rm = IncomprRandMeth(model, mode_no=100, mean_velocity=0.5)
+    rm.update(model, 476356)
+    f = vector_field(rm, model.isometrize(np.array([xt, yt]))) + 0.3
+    assert f[0, 0] == pytest.approx(1.23693272, abs=5e-8)
+    assert f[0, 1] == pytest.approx(0.89242284, abs=5e-8)
+    rm.update(model, 4734654)
+    xg, yg = np.linspace(0.0, 12.0, 48), np.linspace(0.0, 10.0, 46)
+    f = (vector_field(rm, model.isometrize(generate_grid((xg, yg)))) + 0.3).reshape(2, 48, 46)
+    assert f[0, 0, 0] == pytest.approx(1.07812013, abs=5e-8)
+    assert f[0, 1, 0] == pytest.approx(1.06180674, abs=5e-8)
+
+
+def test_rng_normal_consistency():
+    """``test_rng.py:20-37``."""
+    rng = RNG(SEED)
+    z1 = rng.random.normal(size=100)
+    z2 = rng.random.normal(size=100)
+    assert z1[:2] == pytest.approx([-1.93013270, 0.46330478], abs=5e-8)
+    assert z2[:2] == pytest.approx([-0.25536086, 0.98298696], abs=5e-8)
+    rng.seed = SEED
+    z1 = rng.random.normal(size=100)
+    assert z1[0] == pytest.approx(-1.93013270, abs=5e-8)
+
+
+@pytest.mark.parametrize("dim", [1, 2, 3])
+def test_sample_sphere(dim):
+    """``test_rng.py:39-67``."""
+    rng = RNG(SEED)
+    c = rng.sample_sphere(dim, 100)
+    assert c.shape == (dim, 100)
+    if dim > 1:
+        assert np.allclose(np.sum(c**2, axis=0), 1.0)
+    c = rng.sample_sphere(dim, 1000000)
+    assert abs(np.mean(c)) < 5e-3
+
+
+def test_sample_dist_shapes():
+    """``test_rng.py:69-89`` (inversion route only)."""
+    rng = RNG(SEED)
+    for dim in (1, 2):
+        pdf, cdf, ppf = Gaussian(dim=dim, var=3.5, len_scale=8.0).dist_func
+        assert rng.sample_dist(size=100, pdf=pdf, cdf=cdf, ppf=ppf, a=0).shape == (100,)
+
+
+def test_ppf_inverts_cdf():
+    u = np.linspace(0.01, 0.99, 50)
+    for model in (Gaussian(dim=1, len_scale=3.0), Gaussian(dim=2, len_scale=3.0),
+                  Exponential(dim=1, len_scale=3.0)):
+        assert np.allclose(model.spectral_rad_cdf(model.spectral_rad_ppf(u)), u, rtol=1e-10)
+    # The reference's 2D Exponential ppf (covmodel/models.py:257-263) inverts
+    # the cdf at 1 - u: same distribution, different draw per seed; kept as is.
+    m = Exponential(dim=2, len_scale=3.0)
+    assert np.allclose(m.spectral_rad_cdf(m.spectral_rad_ppf(u)), 1.0 - u, rtol=1e-10)
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_hankel_stable_special_cases(dim):
+    """Stable(alpha=1) is Exponential; Stable(alpha=2, L) is Gaussian(L sqrt(pi)/2).
+
+    Checked for k L in [0.05, 16], where the radial pdf carries its mass; the
+    Ogata quadrature at N=200, h=0.001 loses accuracy below k L ~ 0.02 (and
+    in 1D, which no configuration uses).
+    """
+    k = np.logspace(np.log10(0.05), np.log10(16.0), 25) / 5.0
+    s1 = Stable(dim=dim, len_scale=5.0, alpha=1.0).spectral_density(k)
+    e = Exponential(dim=dim, len_scale=5.0).spectral_density(k)
+    assert np.allclose(s1, e, rtol=1e-3)
+    s2 = Stable(dim=dim, len_scale=5.0, alpha=2.0).spectral_density(k)
+    g = Gaussian(dim=dim, len_scale=5.0 * np.sqrt(np.pi) / 2).spectral_density(k)
+    mask = g > 1e-6 * g.max()
+    assert np.allclose(s2[mask], g[mask], rtol=1e-3)
+
+
+def test_hankel_matches_matern_density():
+    m = Matern(dim=3, len_scale=8.0, nu=1.5)
+    k = np.logspace(-3, 0, 20)
+    from gstools_amd.hankel import SymmetricFourierTransform
+
+    sft = SymmetricFourierTransform(ndim=3)
+    assert np.allclose(sft.transform(m.correlation, k), m.spectral_density(k), rtol=5e-3)
+
+
+def test_isometrize_matches_matrix():
+    m = Exponential(dim=3, var=2.0, len_scale=[50, 20, 5], angles=[0.5, 0.2, 0.1])
+    assert m.len_scale == 50 and np.allclose(m.anis, [0.4, 0.1])
+    pos = np.random.RandomState(0).uniform(0, 1000, size=(3, 10))
+    iso = m.isometrize(pos)
+    mat = matrix_isometrize(3, m.angles, m.anis)
+    assert np.allclose(iso, mat @ pos)
+    # isometrize preserves the model's correlation distance along the main axis
+    assert np.allclose(np.linalg.norm(iso, axis=0) > 0, True)
