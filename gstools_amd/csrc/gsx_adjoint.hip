@@ -17,7 +17,7 @@
 // reproducible run to run (no atomics).
 //
 // FP64: exact reference phase and sincos_fast (two-term reduction) with a
-// library fallback for waves whose phases may exceed 2^30.  FP32: phase in
+// library fallback for waves whose phases may exceed 2^46.  FP32: phase in
 // revolutions in FP64, v_fract_f64, hardware v_sin_f32 / v_cos_f32, FP32
 // partial sums per LDS chunk folded into FP64 accumulators.
 #include "gsx_internal.h"
@@ -41,6 +41,15 @@ inline AdjPlan adj_plan(int64_t N, int64_t P) {
     pl.chunk_pts = ceil_div(ceil_div(P, chunks), kAdjPts) * kAdjPts;
     pl.chunks = ceil_div(P, pl.chunk_pts);
     return pl;
+}
+
+// sin / cos at any |x|: the fast reduction where it is exact, else the library.
+__device__ __noinline__ void sincos_any(double x, double* sn, double* cs) {
+    if (fabs(x) <= kFastPhaseMax) {
+        sincos_fast(x, sn, cs);
+    } else {
+        sincos(x, sn, cs);
+    }
 }
 
 template <int D, typename GT>
@@ -93,7 +102,7 @@ __global__ __launch_bounds__(kThreads) void k_adjoint_stage1(int64_t N, int64_t 
         __syncthreads();
         if constexpr (F64) {
             const double xmax = __longlong_as_double((long long)s_xmax);
-            if (__all(k1 * xmax <= 0x1.0p+29)) {
+            if (__all(k1 * xmax <= 0x1.0p+45)) {
                 for (int m = 0; m < cnt; ++m) {
                     double x[D];
 #pragma unroll
@@ -115,11 +124,7 @@ __global__ __launch_bounds__(kThreads) void k_adjoint_stage1(int64_t N, int64_t 
                     for (int d = 0; d < D; ++d) x[d] = s_x[m * D + d];
                     const double th = phase_exact<D>(k, x);
                     double sn, cs;
-                    if (fabs(th) < kFastPhaseMax) {
-                        sincos_fast(th, &sn, &cs);
-                    } else {
-                        sincos(th, &sn, &cs);
-                    }
+                    sincos_any(th, &sn, &cs);
                     const GT* gg = s_g + m * (D + 1);
                     acc[0] = fma_(gg[0], cs, acc[0]);
                     acc[1] = fma_(gg[0], sn, acc[1]);

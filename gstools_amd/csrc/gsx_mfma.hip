@@ -1,13 +1,420 @@
-// Structured-grid summation on the matrix cores (placeholder until the GEMM kernel lands).
+// Structured-grid randomisation-method sum on the gfx950 matrix cores.
+//
+// On a tensor grid (axes x_i0, y_i1, z_i2) the reference sum
+// (src/gstools/field/generator.py:193-199, positions from generate_grid,
+// tools/geometric.py:340-356) factorises by the angle-addition identity:
+//
+//   u[i0, i1, i2] = Re sum_j c_j e^{i k0_j x_i0} e^{i k1_j y_i1} e^{i k2_j z_i2},
+//   c_j = z1_j - i z2_j,
+//
+// i.e. for every column (i1, i2) a real GEMM with K = 2N:
+//
+//   U (n0 x n1 n2) = A (n0 x 2N) . B (2N x n1 n2)
+//   A[i0, 2j] = cos(k0_j x_i0),           A[i0, 2j+1] = -sin(k0_j x_i0)
+//   B[2j, c]  = Re(E1_j(i1) E2_j(i2)),    B[2j+1, c]  = Im(E1_j(i1) E2_j(i2))
+//   E1_j(i1)  = e^{i k1_j y_i1},          E2_j(i2)    = c_j e^{i k2_j z_i2}
+//
+// (2D grids use E1 = 1 on a one-point middle axis.)  Every operand is split
+// into a high and a low 16-bit part (bf16 or fp16) and the product is
+// A_hi B_hi + A_hi B_lo + A_lo B_hi on v_mfma_f32_16x16x32_{bf16,f16},
+// accumulated in FP32.
+//
+// Kernels:
+//  * k_mfma_tables: phase factors in FP64 (sincos of the FP64 phase), A split
+//    into hi/lo 16-bit rows [i0][2Np], E1/E2 as complex FP32 rows [i][Np].
+//  * k_mfma_grid: block = 4 waves, tile = 128 rows x 256 columns; the A tile
+//    of each K step (16 modes = 32 real K) is staged through double-buffered
+//    LDS and shared by the 4 waves; each wave owns 64 columns (4 fragments of
+//    16) and builds its B fragments in registers from E1/E2 (complex product
+//    + hi/lo split), so B never touches memory.  Block -> tile mapping is
+//    XCD-aware: the blocks resident on one XCD work on one row tile of A,
+//    so that XCD's L2 holds the A rows its blocks stream.
+//
+// MFMA operand layouts (v_mfma_f32_16x16x32, 64 lanes, g = lane / 16):
+//   A: lane holds row (lane % 16), K elements 8g .. 8g+7
+//   B: lane holds column (lane % 16), K elements 8g .. 8g+7
+//   C: lane holds column (lane % 16), rows 4g .. 4g+3
+// K element 2t (+1) of a lane's 8 is Re (Im) of mode 4g + t of the K step.
 #include "gsx_internal.h"
 
+namespace gsx {
+
+typedef __attribute__((ext_vector_type(8))) __bf16 bf16x8;
+typedef __attribute__((ext_vector_type(8))) _Float16 f16x8;
+typedef __attribute__((ext_vector_type(2))) __bf16 bf16x2;
+typedef __attribute__((ext_vector_type(2))) _Float16 f16x2;
+typedef __attribute__((ext_vector_type(4))) float f32x4;
+typedef __attribute__((ext_vector_type(4))) uint32_t u32x4;
+
+constexpr int kMBM = 128;            // rows per block
+constexpr int kMWaves = 4;
+constexpr int kMCF = 4;              // 16-column fragments per wave
+constexpr int kMRF = kMBM / 16;      // 16-row fragments per wave (all block rows)
+constexpr int kMKM = 16;             // modes per K step
+constexpr int kMLdsRow = 2 * kMKM + 8;  // halfwords per LDS row (16 B pad)
+
+struct MfmaGeom {
+    int64_t n0, n1, n2;   // grid (n1 = 1 for 2D grids)
+    int64_t n0p, Np;      // padded rows / modes
+    int64_t ncols;        // n1 * n2
+    int g2;               // i2 fragments per column group (1 or 16)
+    int64_t ti1, ti2;     // column tile extent along i1 / i2
+    int64_t t1, t2;       // column tiles along i1 / i2
+    int64_t row_tiles, col_tiles;
+    int64_t a_elems, e1_elems, e2_elems;
+};
+
+inline MfmaGeom mfma_geom(int64_t d, int64_t N, const int64_t* shape) {
+    MfmaGeom g;
+    g.n0 = shape[0];
+    g.n1 = d == 3 ? shape[1] : 1;
+    g.n2 = d == 3 ? shape[2] : shape[1];
+    g.n0p = ceil_div(g.n0, kMBM) * kMBM;
+    g.Np = ceil_div(N, kMKM) * kMKM;
+    g.ncols = g.n1 * g.n2;
+    g.g2 = (g.n1 >= 16 && g.n2 >= 16) ? 1 : 16;
+    g.ti1 = 16 / g.g2;
+    g.ti2 = 16 * (int64_t)g.g2;
+    g.t1 = ceil_div(g.n1, g.ti1);
+    g.t2 = ceil_div(g.n2, g.ti2);
+    g.row_tiles = g.n0p / kMBM;
+    g.col_tiles = g.t1 * g.t2;
+    g.a_elems = g.n0p * 2 * g.Np;  // 16-bit elements, hi and lo tables each
+    g.e1_elems = g.n1 * g.Np;      // complex float
+    g.e2_elems = g.n2 * g.Np;
+    return g;
+}
+
+inline int64_t mfma_ws_bytes(const MfmaGeom& g) {
+    return 2 * g.a_elems * 2 + (g.e1_elems + g.e2_elems) * 8;
+}
+
+// ---------------------------------------------------------------- splits ---
+template <bool FP16>
+__device__ __forceinline__ void split2(float a, float b, uint32_t& hi, uint32_t& lo) {
+    if constexpr (FP16) {
+        const f16x2 h = {(_Float16)a, (_Float16)b};
+        hi = __builtin_bit_cast(uint32_t, h);
+        const f16x2 l = {(_Float16)(a - (float)h[0]), (_Float16)(b - (float)h[1])};
+        lo = __builtin_bit_cast(uint32_t, l);
+    } else {
+        const bf16x2 h = {(__bf16)a, (__bf16)b};
+        hi = __builtin_bit_cast(uint32_t, h);
+        const float ha = __builtin_bit_cast(float, hi << 16);
+        const float hb = __builtin_bit_cast(float, hi & 0xffff0000u);
+        const bf16x2 l = {(__bf16)(a - ha), (__bf16)(b - hb)};
+        lo = __builtin_bit_cast(uint32_t, l);
+    }
+}
+
+// --------------------------------------------------------------- tables ----
+// One thread per table entry: A (n0p x Np), E1 (n1 x Np), E2 (n2 x Np).
+template <bool FP16>
+__global__ __launch_bounds__(256) void k_mfma_tables(MfmaGeom g, int64_t d, int64_t N,
+                                                     const double* __restrict__ ax0,
+                                                     const double* __restrict__ ax1,
+                                                     const double* __restrict__ ax2,
+                                                     const double* __restrict__ cov,
+                                                     const double* __restrict__ z1,
+                                                     const double* __restrict__ z2,
+                                                     uint32_t* __restrict__ a_hi,
+                                                     uint32_t* __restrict__ a_lo,
+                                                     float2* __restrict__ e1,
+                                                     float2* __restrict__ e2) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t na = g.n0p * g.Np, n1 = g.e1_elems, n2 = g.e2_elems;
+    if (t < na) {
+        const int64_t row = t / g.Np, j = t - row * g.Np;
+        float re = 0.0f, im = 0.0f;
+        if (row < g.n0 && j < N) {
+            double s, c;
+            const double th = cov[j] * ax0[row];
+            if (fabs(th) <= kFastPhaseMax) sincos_fast(th, &s, &c); else sincos(th, &s, &c);
+            re = (float)c;
+            im = (float)(-s);
+        }
+        uint32_t hi, lo;
+        split2<FP16>(re, im, hi, lo);
+        a_hi[t] = hi;  // element pair (2j, 2j+1) of row `row`
+        a_lo[t] = lo;
+    } else if (t < na + n1) {
+        const int64_t u = t - na;
+        const int64_t i1 = u / g.Np, j = u - i1 * g.Np;
+        float2 v = {0.0f, 0.0f};
+        if (j < N) {
+            if (d == 3) {
+                double s, c;
+                const double th = cov[N + j] * ax1[i1];
+                if (fabs(th) <= kFastPhaseMax) sincos_fast(th, &s, &c); else sincos(th, &s, &c);
+                v = {(float)c, (float)s};
+            } else {
+                v = {1.0f, 0.0f};
+            }
+        }
+        e1[u] = v;
+    } else if (t < na + n1 + n2) {
+        const int64_t u = t - na - n1;
+        const int64_t i2 = u / g.Np, j = u - i2 * g.Np;
+        float2 v = {0.0f, 0.0f};
+        if (j < N) {
+            double s, c;
+            const double kz = cov[(d - 1) * N + j];
+            const double th = kz * (d == 3 ? ax2[i2] : ax1[i2]);
+            if (fabs(th) <= kFastPhaseMax) sincos_fast(th, &s, &c); else sincos(th, &s, &c);
+            const double a = z1[j], b = z2[j];
+            // (a - i b)(c + i s)
+            v = {(float)(a * c + b * s), (float)(a * s - b * c)};
+        }
+        e2[u] = v;
+    }
+}
+
+// ---------------------------------------------------------------- GEMM -----
+template <bool FP16>
+__device__ __forceinline__ f32x4 mfma(u32x4 a, u32x4 b, f32x4 c) {
+    if constexpr (FP16) {
+        return __builtin_amdgcn_mfma_f32_16x16x32_f16(__builtin_bit_cast(f16x8, a),
+                                                      __builtin_bit_cast(f16x8, b), c, 0, 0, 0);
+    } else {
+        return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, a),
+                                                       __builtin_bit_cast(bf16x8, b), c, 0, 0, 0);
+    }
+}
+
+__device__ __forceinline__ float2 cmul(float2 a, float2 b) {
+    return {a.x * b.x - a.y * b.y, __builtin_fmaf(a.x, b.y, a.y * b.x)};
+}
+
+template <bool FP16, int G2>
+__global__ __launch_bounds__(kMWaves * 64, 2) void k_mfma_grid(MfmaGeom g, double alpha,
+                                                               const uint32_t* __restrict__ a_hi,
+                                                               const uint32_t* __restrict__ a_lo,
+                                                               const float2* __restrict__ e1,
+                                                               const float2* __restrict__ e2,
+                                                               float* __restrict__ out) {
+    __shared__ __align__(16) uint16_t lds[2][2][kMBM * kMLdsRow];  // [buffer][hi/lo][row * stride]
+
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int grp = lane >> 4;  // K group g
+    const int l16 = lane & 15;
+
+    // XCD-aware tile mapping: dispatch places block b on XCD b % 8.
+    const int64_t b = blockIdx.x;
+    int64_t rt, ct;
+    {
+        const int64_t xcd = b & 7, w = b >> 3;
+        const int64_t R = g.row_tiles;
+        if (R <= 8 && (8 % R) == 0 && (g.col_tiles % (8 / R)) == 0) {
+            const int64_t per = 8 / R;  // XCDs per row tile
+            rt = xcd / per;
+            ct = w * per + (xcd % per);
+        } else {
+            rt = b % R;
+            ct = b / R;
+        }
+    }
+    const int64_t tile1 = ct / g.t2, tile2 = ct - tile1 * g.t2;
+
+    // Column of each fragment; clamped copies index the tables.
+    int64_t col[kMCF];
+    bool cok[kMCF];
+    const float2* e1p[kMCF];
+    const float2* e2p[kMCF];
+#pragma unroll
+    for (int cf = 0; cf < kMCF; ++cf) {
+        const int f = wave * kMCF + cf;
+        const int64_t i1 = tile1 * (16 / G2) + f / G2;
+        const int64_t i2 = tile2 * (16 * G2) + 16 * (f % G2) + l16;
+        cok[cf] = i1 < g.n1 && i2 < g.n2;
+        col[cf] = i1 * g.n2 + i2;
+        const int64_t c1 = i1 < g.n1 ? i1 : g.n1 - 1;
+        const int64_t c2 = i2 < g.n2 ? i2 : g.n2 - 1;
+        e1p[cf] = e1 + c1 * g.Np + 4 * grp;
+        e2p[cf] = e2 + c2 * g.Np + 4 * grp;
+    }
+
+    f32x4 acc[kMRF][kMCF];
+#pragma unroll
+    for (int rf = 0; rf < kMRF; ++rf)
+#pragma unroll
+        for (int cf = 0; cf < kMCF; ++cf) acc[rf][cf] = f32x4{0.f, 0.f, 0.f, 0.f};
+
+    // A staging: 128 rows x 64 B per K step for hi and for lo = 512 chunks of
+    // 16 B each; thread t moves chunks t and t + 256 of both tables.
+    const int64_t row0 = rt * kMBM;
+    const int64_t a_stride = g.Np;  // uint32 (element pairs) per A row
+    u32x4 pre[2][2];
+    auto load_a = [&](int64_t step) {
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const int c = threadIdx.x + h * 256;
+            const int r = c >> 2, part = c & 3;
+            const int64_t off = (row0 + r) * a_stride + step * kMKM + part * 4;
+            pre[h][0] = *reinterpret_cast<const u32x4*>(a_hi + off);
+            pre[h][1] = *reinterpret_cast<const u32x4*>(a_lo + off);
+        }
+    };
+    auto store_a = [&](int buf) {
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const int c = threadIdx.x + h * 256;
+            const int r = c >> 2, part = c & 3;
+            *reinterpret_cast<u32x4*>(&lds[buf][0][r * kMLdsRow + part * 8]) = pre[h][0];
+            *reinterpret_cast<u32x4*>(&lds[buf][1][r * kMLdsRow + part * 8]) = pre[h][1];
+        }
+    };
+
+    const int64_t steps = g.Np / kMKM;
+    load_a(0);
+    store_a(0);
+    __syncthreads();
+
+    for (int64_t s = 0; s < steps; ++s) {
+        const int buf = (int)(s & 1);
+        if (s + 1 < steps) load_a(s + 1);
+
+        // B fragments for this K step, built in registers.
+        u32x4 bh[kMCF], bl[kMCF];
+        const int64_t moff = s * kMKM;
+        float2 zf[4];
+        if constexpr (G2 == 1) {
+            // all fragments of the wave share i2: one E2 load
+            const float4* p = reinterpret_cast<const float4*>(e2p[0] + moff);
+            const float4 q0 = p[0], q1 = p[1];
+            zf[0] = {q0.x, q0.y};
+            zf[1] = {q0.z, q0.w};
+            zf[2] = {q1.x, q1.y};
+            zf[3] = {q1.z, q1.w};
+        }
+#pragma unroll
+        for (int cf = 0; cf < kMCF; ++cf) {
+            float2 yf[4];
+            {
+                const float4* p = reinterpret_cast<const float4*>(e1p[cf] + moff);
+                const float4 q0 = p[0], q1 = p[1];
+                yf[0] = {q0.x, q0.y};
+                yf[1] = {q0.z, q0.w};
+                yf[2] = {q1.x, q1.y};
+                yf[3] = {q1.z, q1.w};
+            }
+            if constexpr (G2 != 1) {
+                const float4* p = reinterpret_cast<const float4*>(e2p[cf] + moff);
+                const float4 q0 = p[0], q1 = p[1];
+                zf[0] = {q0.x, q0.y};
+                zf[1] = {q0.z, q0.w};
+                zf[2] = {q1.x, q1.y};
+                zf[3] = {q1.z, q1.w};
+            }
+            uint32_t h4[4], l4[4];
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+                const float2 v = cmul(yf[t], zf[t]);
+                split2<FP16>(v.x, v.y, h4[t], l4[t]);
+            }
+            bh[cf] = u32x4{h4[0], h4[1], h4[2], h4[3]};
+            bl[cf] = u32x4{l4[0], l4[1], l4[2], l4[3]};
+        }
+
+#pragma unroll
+        for (int rf = 0; rf < kMRF; ++rf) {
+            const int r = rf * 16 + l16;
+            const u32x4 ah = *reinterpret_cast<const u32x4*>(&lds[buf][0][r * kMLdsRow + grp * 8]);
+            const u32x4 al = *reinterpret_cast<const u32x4*>(&lds[buf][1][r * kMLdsRow + grp * 8]);
+#pragma unroll
+            for (int cf = 0; cf < kMCF; ++cf) {
+                acc[rf][cf] = mfma<FP16>(ah, bh[cf], acc[rf][cf]);
+                acc[rf][cf] = mfma<FP16>(ah, bl[cf], acc[rf][cf]);
+                acc[rf][cf] = mfma<FP16>(al, bh[cf], acc[rf][cf]);
+            }
+        }
+        if (s + 1 < steps) store_a(buf ^ 1);
+        __syncthreads();
+    }
+
+    const float fa = (float)alpha;
+#pragma unroll
+    for (int rf = 0; rf < kMRF; ++rf) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int64_t row = row0 + rf * 16 + grp * 4 + i;
+            if (row < g.n0) {
+#pragma unroll
+                for (int cf = 0; cf < kMCF; ++cf)
+                    if (cok[cf]) out[row * g.ncols + col[cf]] = fa * acc[rf][cf][i];
+            }
+        }
+    }
+}
+
+__global__ void k_fill_zero_f32(float* __restrict__ p, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0.0f;
+}
+
+template <bool FP16>
+int mfma_launch(const MfmaGeom& g, int64_t d, int64_t N, const double* ax0, const double* ax1,
+                const double* ax2, const double* cov, const double* z1, const double* z2, double alpha,
+                float* out, void* ws, hipStream_t s) {
+    uint32_t* a_hi = static_cast<uint32_t*>(ws);
+    uint32_t* a_lo = a_hi + g.a_elems / 2;
+    float2* e1 = reinterpret_cast<float2*>(a_lo + g.a_elems / 2);
+    float2* e2 = e1 + g.e1_elems;
+    const int64_t entries = g.n0p * g.Np + g.e1_elems + g.e2_elems;
+    hipLaunchKernelGGL((k_mfma_tables<FP16>), dim3((unsigned)ceil_div(entries, 256)), dim3(256), 0, s, g, d,
+                       N, ax0, ax1, ax2, cov, z1, z2, a_hi, a_lo, e1, e2);
+    int st = launch_status();
+    if (st != GSX_OK) return st;
+    const int64_t blocks = g.row_tiles * g.col_tiles;
+    if (g.g2 == 1) {
+        hipLaunchKernelGGL((k_mfma_grid<FP16, 1>), dim3((unsigned)blocks), dim3(kMWaves * 64), 0, s, g, alpha,
+                           a_hi, a_lo, e1, e2, out);
+    } else {
+        hipLaunchKernelGGL((k_mfma_grid<FP16, 16>), dim3((unsigned)blocks), dim3(kMWaves * 64), 0, s, g, alpha,
+                           a_hi, a_lo, e1, e2, out);
+    }
+    return launch_status();
+}
+
+}  // namespace gsx
+
 extern "C" {
+
 int64_t gsx_summate_grid_mfma_workspace(int64_t d, int64_t N, const int64_t* shape) {
-    (void)d; (void)N; (void)shape;
-    return 0;
+    if ((d != 2 && d != 3) || N < 0 || !shape) return GSX_E_INVALID;
+    for (int i = 0; i < d; ++i)
+        if (shape[i] < 0) return GSX_E_INVALID;
+    const gsx::MfmaGeom g = gsx::mfma_geom(d, N, shape);
+    return gsx::mfma_ws_bytes(g);
 }
-int gsx_summate_grid_mfma(int64_t, int64_t, const int64_t*, const double*, const double*, const double*,
-                          const double*, int, double, float*, void*, int64_t, void*) {
-    return GSX_E_UNSUPPORTED;
+
+int gsx_summate_grid_mfma(int64_t d, int64_t N, const int64_t* shape, const double* axes, const double* cov,
+                          const double* z1, const double* z2, int variant, double alpha, float* out,
+                          void* workspace, int64_t workspace_bytes, void* stream) {
+    if (d == 1) return GSX_E_UNSUPPORTED;
+    if ((d != 2 && d != 3) || N < 0 || !shape) return GSX_E_INVALID;
+    if (variant != GSX_MFMA_BF16X3 && variant != GSX_MFMA_FP16X3) return GSX_E_INVALID;
+    int64_t P = 1;
+    for (int i = 0; i < d; ++i) {
+        if (shape[i] < 0) return GSX_E_INVALID;
+        P *= shape[i];
+    }
+    if (P == 0) return GSX_OK;
+    if (!axes || !out || (N > 0 && (!cov || !z1 || !z2))) return GSX_E_INVALID;
+    hipStream_t s = gsx::as_stream(stream);
+    if (N == 0) {  // empty sum
+        hipLaunchKernelGGL(gsx::k_fill_zero_f32, dim3((unsigned)gsx::ceil_div(P, 256)), dim3(256), 0, s, out, P);
+        return gsx::launch_status();
+    }
+    const gsx::MfmaGeom g = gsx::mfma_geom(d, N, shape);
+    if (g.row_tiles * g.col_tiles > 0x7fffffffLL) return GSX_E_INVALID;
+    if (!workspace || workspace_bytes < gsx::mfma_ws_bytes(g)) return GSX_E_WORKSPACE;
+    const double* ax0 = axes;
+    const double* ax1 = axes + shape[0];
+    const double* ax2 = d == 3 ? ax1 + shape[1] : nullptr;
+    if (variant == GSX_MFMA_FP16X3)
+        return gsx::mfma_launch<true>(g, d, N, ax0, ax1, ax2, cov, z1, z2, alpha, out, workspace, s);
+    return gsx::mfma_launch<false>(g, d, N, ax0, ax1, ax2, cov, z1, z2, alpha, out, workspace, s);
 }
-}
+
+}  // extern "C"

@@ -19,8 +19,8 @@
 // FP64 path: the phase is the reference's exact product-then-add sequence;
 // cos(t - phi) uses the two-term Cody-Waite reduction of gsx_math.h with phi
 // folded into the reduction (no rounding of t - phi at large |t|).  A wave
-// whose |x|_inf * max_j |k_j|_1 may exceed 2^29 takes a per-term guarded path
-// that falls back to the library sincos for |t| >= 2^30.
+// whose |x|_inf * max_j |k_j|_1 may exceed 2^45 takes a per-term guarded path
+// that falls back to the library sincos for |t| > 2^46.
 //
 // FP32 path: the phase is computed in revolutions in FP64 (t = k/2pi . x -
 // phi/2pi, three FMAs), reduced exactly with v_fract_f64, converted to FP32
@@ -34,7 +34,18 @@ namespace gsx {
 
 constexpr int kChunk = 256;  // modes per LDS chunk (== kThreads: one mode per thread to build)
 constexpr int kSub = 64;     // FP32 terms per FP32 partial sum
-constexpr double kSafeK1X = 0x1.0p+29;  // |k|_1 |x|_inf bound of the FP64 fast path
+constexpr double kSafeK1X = 0x1.0p+45;  // |k|_1 |x|_inf bound of the FP64 fast path
+
+// cos(theta - phi) at any |theta|: the fast reduction where it is exact, else the
+// library sincos.  Kept out of line so the fast loop's register budget does not
+// pay for the wide-range code.
+__device__ __noinline__ double cos_shifted_any(double th, double phi, double phipi) {
+    if (fabs(th) <= kFastPhaseMax) return cos_shifted(th, phi, phipi);
+    double st, ct, sp, cp;
+    sincos(th, &st, &ct);
+    sincos(phi, &sp, &cp);
+    return ct * cp + st * sp;
+}
 
 __device__ __forceinline__ void lds_max_u64(unsigned long long* p, double v) {
     // non-negative doubles order like their bit patterns
@@ -155,16 +166,7 @@ __global__ __launch_bounds__(kThreads) void k_forward_f64(Pts pts, int64_t N, in
                 const double phi = INCOMPR ? t[D] : t[D + 1];
                 const double phipi = INCOMPR ? t[D + 1] : t[D + 2];
                 for (int p = 0; p < PPT; ++p) {
-                    const double th = phase_exact<D>(k, x[p]);
-                    double cs;
-                    if (fabs(th) < kFastPhaseMax) {
-                        cs = cos_shifted(th, phi, phipi);
-                    } else {
-                        double st, ct, sp, cp;
-                        sincos(th, &st, &ct);
-                        sincos(phi, &sp, &cp);
-                        cs = ct * cp + st * sp;
-                    }
+                    const double cs = cos_shifted_any(phase_exact<D>(k, x[p]), phi, phipi);
                     if constexpr (INCOMPR) {
                         for (int c = 0; c < D; ++c) acc[p][c] = fma_(t[D + 2 + c], cs, acc[p][c]);
                     } else {
@@ -245,6 +247,7 @@ __global__ __launch_bounds__(kThreads) void k_forward_f32(Pts pts, int64_t N, in
             for (int p = 0; p < PPT; ++p)
 #pragma unroll
                 for (int c = 0; c < NOUT; ++c) part[p][c] = 0.0f;
+#pragma unroll 2
             for (int m = m0; m < m1; ++m) {
                 const double* t = tabd + m * kWd;
                 double kr[D];

@@ -55,7 +55,7 @@ def test_fp64_matches_oracle(d, n, p, kscale, xscale):
 def test_fp64_huge_phase_guarded_path():
     _say("fp64 huge phases")
     cov, z1, z2, pos = _case(2, 300, 2000, 7, 1.0, 1.0)
-    pos[:, :1000] *= 3e9  # |k|_1 |x| far above 2^29 for these points
+    pos[:, :1000] *= 3e14  # phases beyond 2^46: the guarded library-sincos path
     ref = oracle.summate_c(cov, z1, z2, pos)
     got = summate(cov, z1, z2, pos)
     assert oracle.rel_errors(got, ref)[0] <= FP64_TOL
