@@ -93,6 +93,94 @@ def cpu_baseline(rm, iso, target_s=10.0):
     }
 
 
+MFMA_PEAK_TFLOPS = 2500.0  # dense bf16 / fp16
+
+
+def bench_mfma_grid(torch, steps, warmup):
+    """512^3 Matern(nu=1.5, len 8) grid at spacing 0.5, N 4096: split bf16 / fp16 on MFMA."""
+    import oracle
+    from gstools_amd import Matern, RandMeth, summate_grid_device
+
+    log("bench: mfma grid (512^3, N 4096)")
+    model = Matern(dim=3, var=1.0, len_scale=8.0, nu=1.5)
+    rm = RandMeth(model, mode_no=4096, seed=20170519)
+    dev = torch.device("cuda", 0)
+    cov = torch.from_numpy(rm._cov_sample).to(dev)
+    z1 = torch.from_numpy(rm._z_1).to(dev)
+    z2 = torch.from_numpy(rm._z_2).to(dev)
+    ax = np.arange(512) * 0.5
+    axes = [torch.from_numpy(ax).to(dev)] * 3
+    n = 512**3
+    work = float(n) * 4096
+    out = torch.empty(n, dtype=torch.float32, device=dev)
+    scale = float(np.sqrt(model.var / 4096))
+    zsub = np.array([0, 101, 257, 511])
+    ref_axes = [axes[0], axes[1], torch.from_numpy(ax[zsub]).to(dev)]
+    ref = summate_grid_device(cov, z1, z2, ref_axes, "fp64").reshape(512, 512, len(zsub)).cpu().numpy() * scale
+    res = {"workload": "structured 512^3 Matern nu=1.5 len 8, spacing 0.5, mode_no 4096 (MCMC modes)",
+           "points": n, "modes": 4096}
+    for prec in ("bf16x3", "fp16x3"):
+        fn = lambda: summate_grid_device(cov, z1, z2, axes, prec, alpha=scale, out=out)
+        wall, ker = time_kernel(fn, steps, warmup, torch)
+        got = out.reshape(512, 512, 512)[:, :, zsub].cpu().numpy().astype(np.float64)
+        rel_max, rel_rms = oracle.rel_errors(got, ref)
+        tflops = 12.0 * work / ker / 1e12
+        res[prec] = {"value": work / wall, "ms_per_step": wall * 1e3, "rel_max_vs_fp64": rel_max,
+                     "rel_rms_vs_fp64": rel_rms, "mfma_tflops": tflops, "roofline_frac": tflops / MFMA_PEAK_TFLOPS}
+    # FP64 direct sum on a 64-plane slab (the work is exactly linear in the points)
+    slab = [axes[0][:64], axes[1], axes[2]]
+    f64 = lambda: summate_grid_device(cov, z1, z2, slab, "fp64")
+    wall64, ker64 = time_kernel(f64, 2, 1, torch)
+    res["fp64_direct"] = {"value": 64 * 512 * 512 * 4096 / wall64, "measured_on": "64 of 512 x-planes"}
+    res["speedup_bf16x3_vs_fp64"] = res["bf16x3"]["value"] / res["fp64_direct"]["value"]
+    res["speedup_fp16x3_vs_fp64"] = res["fp16x3"]["value"] / res["fp64_direct"]["value"]
+    return res
+
+
+def bench_adjoint(torch, steps, warmup):
+    """2D Stable(alpha 1.5, len 15), N 2048, 1e6 points in [0,500]^2, FP32: forward vs forward+adjoint."""
+    import oracle
+    from gstools_amd import Stable, RandMeth, summate_adjoint_device, summate_device
+
+    log("bench: forward + adjoint (2D Stable, 1e6 points, N 2048)")
+    model = Stable(dim=2, var=1.0, len_scale=15.0, alpha=1.5)
+    rm = RandMeth(model, mode_no=2048, seed=20170519)
+    dev = torch.device("cuda", 0)
+    cov = torch.from_numpy(rm._cov_sample).to(dev)
+    z1 = torch.from_numpy(rm._z_1).to(dev)
+    z2 = torch.from_numpy(rm._z_2).to(dev)
+    npts = 1_000_000
+    pos = torch.from_numpy(np.random.RandomState(19970222).uniform(0, 500, size=(2, npts))).to(dev)
+    g = torch.from_numpy(np.random.RandomState(3).normal(size=npts).astype(np.float32)).to(dev)
+    out = torch.empty(npts, dtype=torch.float32, device=dev)
+    work = float(npts) * 2048
+    fwd = lambda: summate_device(cov, z1, z2, pos, "fp32", out=out)
+    wall_f, ker_f = time_kernel(fwd, steps, warmup, torch)
+
+    def fwd_adj():
+        summate_device(cov, z1, z2, pos, "fp32", out=out)
+        summate_adjoint_device(cov, z1, z2, pos, g, "fp32")
+
+    wall_fa, _ = time_kernel(fwd_adj, steps, warmup, torch)
+    # gradient accuracy on 2000 observation points: FP32 adjoint vs FP64 adjoint vs the oracle
+    obs = np.random.RandomState(4).choice(npts, 2000, replace=False)
+    pobs = pos[:, obs].contiguous()
+    gobs = np.random.RandomState(5).normal(size=2000)
+    a32 = [t.cpu().numpy() for t in summate_adjoint_device(cov, z1, z2, pobs, torch.tensor(gobs, device=dev), "fp32")]
+    a64 = [t.cpu().numpy() for t in summate_adjoint_device(cov, z1, z2, pobs, torch.tensor(gobs, device=dev), "fp64")]
+    ora = oracle.summate_adjoint_c(rm._cov_sample, rm._z_1, rm._z_2, pobs.cpu().numpy(), gobs)
+    rel = lambda a, b: float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
+    return {
+        "workload": "unstructured 2D Stable alpha 1.5 len 15, mode_no 2048 (MCMC modes), 1e6 points in [0,500]^2, fp32",
+        "forward_ms": wall_f * 1e3,
+        "forward_plus_adjoint_ms": wall_fa * 1e3,
+        "ratio_fwd_adj_over_fwd": wall_fa / wall_f,
+        "forward_value": work / wall_f,
+        "grad_rel_err_fp32_vs_fp64": [rel(a32[i], a64[i]) for i in range(3)],
+        "grad_rel_err_fp64_vs_oracle": [rel(a64[i], ora[i]) for i in range(3)],
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -182,6 +270,15 @@ def main():
         },
         "fp32_vs_fp64": {"rel_max": rel_max, "rel_rms": rel_rms, "speedup": wall64 / wall32},
     }
+    if args.extras:
+        extras = {}
+        for name, fn in (("mfma_grid", bench_mfma_grid), ("adjoint", bench_adjoint)):
+            try:
+                extras[name] = fn(torch, max(3, args.steps // 4), max(1, args.warmup))
+            except Exception as exc:  # report, keep the headline line
+                extras[name] = {"error": repr(exc)}
+            log(f"bench: {name}: {json.dumps(extras[name])}")
+        line["extras"] = extras
     if not args.no_cpu:
         log("bench: cpu baseline")
         line["cpu_baseline"] = cpu_baseline(rm, iso)

@@ -206,3 +206,32 @@ def test_isometrize_matches_matrix():
     assert np.allclose(iso, mat @ pos)
     # isometrize preserves the model's correlation distance along the main axis
     assert np.allclose(np.linalg.norm(iso, axis=0) > 0, True)
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+@pytest.mark.parametrize("alpha", [0.7, 1.0, 1.5, 1.9])
+def test_stable_spectral_density_matches_mpmath(dim, alpha):
+    """Quadrature and series branches against mpmath.quadosc (golden fixture,
+    ``tests/golden/make_golden.py``); alpha = 1 against the closed form."""
+    import os
+
+    length = 15.0
+    m = Stable(dim=dim, len_scale=length, alpha=alpha)
+    if alpha == 1.0:  # closed form: the Exponential model
+        ys = np.array([0.5, 3.0, 20.0, 60.0, 500.0, 1e4])
+        ref = Exponential(dim=dim, len_scale=length).spectral_density(ys / length)
+        assert np.allclose(m.spectral_density(ys / length), ref, rtol=1e-6, atol=0)
+        return
+    gold = np.load(os.path.join(os.path.dirname(__file__), "golden", "stable_density.npz"))
+    sel = (gold["d"] == dim) & (gold["alpha"] == alpha)
+    ys, ref = gold["y"][sel], gold["p"][sel] * length**dim
+    got = m.spectral_density(ys / length)
+    assert np.allclose(got, ref, rtol=1e-5, atol=0), np.abs(got / ref - 1)
+
+
+def test_stable_mcmc_radii_stay_physical():
+    """With a normalisable pdf the MCMC radii concentrate around 1 / len_scale."""
+    rm = RandMeth(Stable(dim=2, var=1.0, len_scale=15.0, alpha=1.5), mode_no=2048, seed=20170519)
+    kn = np.linalg.norm(rm._cov_sample, axis=0)
+    assert 0.02 < np.median(kn) < 0.2
+    assert np.max(kn) < 1e3
