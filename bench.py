@@ -181,6 +181,142 @@ def bench_adjoint(torch, steps, warmup):
     }
 
 
+def _batch_specs():
+    """The 256 mixed realizations: (kind, model args, N, seed, size)."""
+    from gstools_amd import MasterRNG
+
+    rs = np.random.RandomState(20170519)
+    seeds = MasterRNG(20170519)
+    modes = [1000, 2048, 4096, 8192]
+    specs = []
+    for _ in range(128):
+        side = int(round(np.exp(rs.uniform(np.log(64), np.log(2048)))))
+        specs.append(("grid2d", int(rs.choice(modes)), seeds(), side))
+    for _ in range(96):
+        npts = int(round(np.exp(rs.uniform(np.log(1e4), np.log(1e6)))))
+        specs.append(("points3d", int(rs.choice(modes)), seeds(), npts))
+    for _ in range(32):
+        specs.append(("incompr3d", int(rs.choice(modes)), seeds(), 128))
+    return specs
+
+
+def _sample_modes(spec):
+    from gstools_amd import Exponential, Gaussian, RandMeth
+
+    kind, n, seed, _ = spec
+    if kind == "grid2d":
+        model = Gaussian(dim=2, var=1.0, len_scale=10.0)
+    elif kind == "points3d":
+        model = Exponential(dim=3, var=2.0, len_scale=[50, 20, 5], angles=[0.5, 0.2, 0.1])
+    else:
+        model = Gaussian(dim=3, var=1.0, len_scale=10.0)
+    rm = RandMeth(model, mode_no=n, seed=seed)
+    return rm._cov_sample, rm._z_1, rm._z_2
+
+
+def batch_modes():
+    """Sample all batch modes (host MCMC), in a fork pool before any GPU use."""
+    import multiprocessing as mp
+
+    specs = _batch_specs()
+    workers = max(1, min(16, len(os.sched_getaffinity(0))))
+    t0 = time.perf_counter()
+    if workers > 1:
+        with mp.get_context("fork").Pool(workers) as pool:
+            modes = pool.map(_sample_modes, specs, chunksize=4)
+    else:
+        modes = [_sample_modes(sp) for sp in specs]
+    log(f"bench: sampled modes of {len(specs)} realizations in {time.perf_counter() - t0:.1f} s ({workers} workers)")
+    return specs, modes
+
+
+def bench_batch(torch, steps, warmup, specs, modes):
+    """256 realizations: one persistent launch (HIP graph) vs per-realization launches."""
+    import oracle
+    from gstools_amd import Exponential, summate_device, summate_grid_device, summate_incompr_device
+    from gstools_amd.batch import BatchItem, BatchPlan, autotune
+
+    log("bench: batch of 256 realizations")
+    dev = torch.device("cuda", 0)
+    expo = Exponential(dim=3, var=2.0, len_scale=[50, 20, 5], angles=[0.5, 0.2, 0.1])
+    items, singles = [], []
+    work = 0.0
+    inc_pos = {}  # meshgrid of each incompressible grid size, built once (outside the timed loops)
+    for spec in specs:
+        if spec[0] == "incompr3d" and spec[3] not in inc_pos:
+            inc_pos[spec[3]] = _grid_pos(torch, [torch.arange(spec[3], dtype=torch.float64, device=dev)] * 3)
+    for r, (spec, (cov, z1, z2)) in enumerate(zip(specs, modes)):
+        kind, n, seed, size = spec
+        covt, z1t, z2t = (torch.from_numpy(a).to(dev) for a in (cov, z1, z2))
+        if kind == "grid2d":
+            axes = [torch.arange(size, dtype=torch.float64, device=dev)] * 2
+            it = BatchItem(covt, z1t, z2t, axes=axes)
+            singles.append(lambda c=covt, a=z1t, b=z2t, ax=axes, o=it.out: summate_grid_device(
+                c, a, b, ax, "fp32", out=o.view(-1)))
+        elif kind == "points3d":
+            pos = np.random.RandomState(seed).uniform(0.0, 1000.0, size=(3, size))
+            post = torch.from_numpy(np.ascontiguousarray(expo.isometrize(pos))).to(dev)
+            it = BatchItem(covt, z1t, z2t, pos=post)
+            singles.append(lambda c=covt, a=z1t, b=z2t, p=post, o=it.out: summate_device(
+                c, a, b, p, "fp32", out=o.view(-1)))
+        else:
+            axes = [torch.arange(size, dtype=torch.float64, device=dev)] * 3
+            it = BatchItem(covt, z1t, z2t, axes=axes, incompr=True)
+            singles.append(lambda c=covt, a=z1t, b=z2t, p=inc_pos[size], o=it.out: summate_incompr_device(
+                c, a, b, p, "fp32", out=o))
+        items.append(it)
+        work += float(it.P) * n
+    best, tune_ms = autotune(items, reps=1)
+    plan = BatchPlan(items, *best)
+    graph_wall, _ = time_kernel(lambda: plan.launch(graph=True), steps, warmup, torch)
+    eager_wall, _ = time_kernel(plan.launch, steps, warmup, torch)
+    batched = [it.out.clone() for it in items]
+    ntiles = plan.tiles
+
+    def per_item():
+        for f in singles:
+            f()
+    per_wall, _ = time_kernel(per_item, max(1, steps // 2), 1, torch)
+    same = all(torch.equal(it.out, b) for it, b in zip(items, batched))
+    # spot checks against the oracle on 2000 points of a few items
+    errs = []
+    for r in (0, 130, 250):
+        cov, z1, z2 = modes[r]
+        it = items[r]
+        idx = np.arange(min(2000, it.P))
+        if it.axes is not None:
+            ax = [a.cpu().numpy() for a in torch.split(it.axes, list(it.shape[:it.d]))]
+            full = np.asarray(np.meshgrid(*ax, indexing="ij")).reshape(it.d, -1)
+            p = full[:, idx]
+        else:
+            p = it.pos.cpu().numpy()[:, idx]
+        ref = oracle.summate_incompr_c(cov, z1, z2, p) if it.incompr else oracle.summate_c(cov, z1, z2, p)[None]
+        errs.append(oracle.rel_errors(batched[r][:, idx].cpu().numpy().astype(np.float64), ref)[0])
+    plan.close()
+    return {
+        "workload": "256 realizations: 128 2D Gaussian grids (side log-U[64,2048]), 96 3D Exponential point "
+                    "sets (log-U[1e4,1e6]), 32 incompressible 3D Gaussian 128^3 grids; mode_no in "
+                    "{1000,2048,4096,8192}; seeds from MasterRNG(20170519); fp32",
+        "mode_points": work,
+        "tiles": ntiles,
+        "autotuned_tile_points_x_mode_chunk": list(best),
+        "autotune_ms": tune_ms,
+        "graph_ms": graph_wall * 1e3,
+        "persistent_eager_ms": eager_wall * 1e3,
+        "per_realization_ms": per_wall * 1e3,
+        "value_graph": work / graph_wall,
+        "value_per_realization": work / per_wall,
+        "speedup_vs_per_realization": per_wall / graph_wall,
+        "batched_equals_single_launches": bool(same),
+        "rel_max_vs_oracle_spot": errs,
+    }
+
+
+def _grid_pos(torch, axes):
+    g = torch.meshgrid(*axes, indexing="ij")
+    return torch.stack([a.reshape(-1) for a in g]).contiguous()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -200,6 +336,7 @@ def main():
     from gstools_amd import summate_device
     import oracle
 
+    batch_inputs = batch_modes() if args.extras else None  # host-only, before the GPU is touched
     if not torch.cuda.is_available():
         print("bench.py: no HIP device visible", file=sys.stderr)
         sys.exit(1)
@@ -272,7 +409,9 @@ def main():
     }
     if args.extras:
         extras = {}
-        for name, fn in (("mfma_grid", bench_mfma_grid), ("adjoint", bench_adjoint)):
+        sections = (("mfma_grid", bench_mfma_grid), ("adjoint", bench_adjoint),
+                    ("batch", lambda t, k, w: bench_batch(t, k, w, *batch_inputs)))
+        for name, fn in sections:
             try:
                 extras[name] = fn(torch, max(3, args.steps // 4), max(1, args.warmup))
             except Exception as exc:  # report, keep the headline line

@@ -65,7 +65,7 @@ EXPORTS = {
     "gsx_summate_adjoint_workspace": (_i64, [_i64, _i64, _i64]),
     "gsx_summate_adjoint_f64": (_i32, _ADJ),
     "gsx_summate_adjoint_f32": (_i32, _ADJ),
-    "gsx_batch_plan_create": (_i32, [ctypes.POINTER(BatchItem), _i64, _i64, ctypes.POINTER(_vp)]),
+    "gsx_batch_plan_create": (_i32, [ctypes.POINTER(BatchItem), _i64, _i64, _i64, ctypes.POINTER(_vp)]),
     "gsx_batch_plan_launch": (_i32, [_vp, _vp]),
     "gsx_batch_plan_launch_graph": (_i32, [_vp, _vp]),
     "gsx_batch_plan_tiles": (_i64, [_vp]),

@@ -1,0 +1,154 @@
+"""Many realizations in one persistent launch (optionally a replayed HIP graph).
+
+Each item is one generator call (``RandMeth.__call__`` /
+``IncomprRandMeth.__call__``, ``src/gstools/field/generator.py:243-270``,
+``:529-567``): its own modes, points (array or grid axes) and FP32 output
+(unscaled sum).  :class:`BatchPlan` wraps ``gsx_batch_plan_*`` from the C ABI;
+:func:`autotune` times the kernel variants (points per tile, modes per LDS
+chunk) on a workload and caches the winner per device and workload shape.
+"""
+
+import ctypes
+import json
+import os
+import time
+
+import numpy as np
+import torch
+
+from . import _native
+
+__all__ = ["BatchItem", "BatchPlan", "autotune", "TILE_VARIANTS"]
+
+TILE_VARIANTS = [(512, 128), (512, 256), (1024, 128), (1024, 256), (2048, 128), (2048, 256)]
+
+
+class BatchItem:
+    """One realization: device tensors for modes, points or axes, and output."""
+
+    def __init__(self, cov, z1, z2, pos=None, axes=None, incompr=False, out=None):
+        dev = torch.device("cuda", torch.cuda.current_device())
+        as_dev = lambda a: (a if isinstance(a, torch.Tensor) else torch.as_tensor(np.asarray(a, dtype=np.float64))
+                            ).to(dev, torch.float64).contiguous()
+        self.cov = as_dev(cov)
+        if self.cov.dim() == 1:
+            self.cov = self.cov.reshape(1, -1)
+        self.d, self.N = int(self.cov.shape[0]), int(self.cov.shape[1])
+        self.z1, self.z2 = as_dev(z1).reshape(-1), as_dev(z2).reshape(-1)
+        if self.z1.numel() != self.N or self.z2.numel() != self.N:
+            raise ValueError("z_1 / z_2 must have N entries")
+        if (pos is None) == (axes is None):
+            raise ValueError("give exactly one of pos and axes")
+        self.pos = self.axes = None
+        if pos is not None:
+            self.pos = as_dev(pos).reshape(self.d, -1)
+            self.shape = (int(self.pos.shape[1]), 1, 1)
+            self.P = int(self.pos.shape[1])
+        else:
+            ax = [as_dev(a).reshape(-1) for a in axes]
+            if len(ax) != self.d:
+                raise ValueError("one axis per dimension")
+            self.axes = torch.cat(ax)
+            dims = [int(a.numel()) for a in ax]
+            self.shape = tuple(dims + [1] * (3 - self.d))
+            self.P = int(np.prod(dims))
+        self.incompr = bool(incompr)
+        if self.incompr and self.d < 2:
+            raise ValueError("Only 2D and 3D incompressible fields can be generated.")
+        nout = self.d if self.incompr else 1
+        self.out = out if out is not None else torch.empty((nout, self.P), dtype=torch.float32, device=dev)
+
+    def to_c(self):
+        it = _native.BatchItem()
+        it.d, it.kind, it.N, it.P = self.d, int(self.incompr), self.N, self.P
+        for i in range(3):
+            it.shape[i] = self.shape[i]
+        ptr = lambda t: t.data_ptr() if t is not None and t.numel() else None
+        it.cov, it.z1, it.z2 = ptr(self.cov), ptr(self.z1), ptr(self.z2)
+        it.pos, it.axes, it.out = ptr(self.pos), ptr(self.axes), ptr(self.out)
+        return it
+
+
+class BatchPlan:
+    """A persistent-launch plan over ``items`` (kept alive by the plan)."""
+
+    def __init__(self, items, tile_points=0, mode_chunk=0):
+        self.items = list(items)
+        arr = (_native.BatchItem * max(1, len(self.items)))(*[it.to_c() for it in self.items])
+        handle = ctypes.c_void_p()
+        _native.check(_native.lib().gsx_batch_plan_create(arr, len(self.items), int(tile_points), int(mode_chunk),
+                                                          ctypes.byref(handle)), "batch_plan_create")
+        self._h = handle
+        self.tile_points, self.mode_chunk = tile_points, mode_chunk
+
+    @property
+    def tiles(self):
+        return int(_native.lib().gsx_batch_plan_tiles(self._h))
+
+    def launch(self, graph=False):
+        s = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+        fn = _native.lib().gsx_batch_plan_launch_graph if graph else _native.lib().gsx_batch_plan_launch
+        _native.check(fn(self._h, s), "batch_plan_launch")
+
+    def close(self):
+        if getattr(self, "_h", None) and self._h.value:
+            _native.lib().gsx_batch_plan_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def _cache_path():
+    return os.environ.get("GSX_AUTOTUNE_CACHE",
+                          os.path.join(os.path.expanduser("~"), ".cache", "gstools_amd", "autotune.json"))
+
+
+def _workload_key(items):
+    work = sum(it.P * it.N * (it.d if it.incompr else 1) for it in items)
+    dims = sorted({(it.d, int(it.incompr), it.axes is not None) for it in items})
+    name = torch.cuda.get_device_name()
+    return f"{name}|batch|{dims}|work~2^{int(np.log2(max(work, 1)))}|items~2^{int(np.log2(max(len(items), 1)))}"
+
+
+def autotune(items, reps=3, graph=True, use_cache=True, verbose=False):
+    """Time every (tile_points, mode_chunk) variant; return (best, timings_ms).
+
+    Results are cached in ``$GSX_AUTOTUNE_CACHE`` (default
+    ``~/.cache/gstools_amd/autotune.json``) keyed by device and workload shape.
+    """
+    key = _workload_key(items)
+    path = _cache_path()
+    cache = {}
+    if use_cache and os.path.exists(path):
+        try:
+            cache = json.load(open(path))
+        except (OSError, ValueError):
+            cache = {}
+        if key in cache:
+            return tuple(cache[key]["best"]), cache[key]["ms"]
+    timings = {}
+    for tp, mc in TILE_VARIANTS:
+        plan = BatchPlan(items, tp, mc)
+        plan.launch(graph)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            plan.launch(graph)
+        torch.cuda.synchronize()
+        timings[f"{tp}x{mc}"] = (time.perf_counter() - t0) / reps * 1e3
+        plan.close()
+        if verbose:
+            print(f"autotune {tp}x{mc}: {timings[f'{tp}x{mc}']:.3f} ms", flush=True)
+    best = min(TILE_VARIANTS, key=lambda v: timings[f"{v[0]}x{v[1]}"])
+    if use_cache:
+        cache[key] = {"best": list(best), "ms": timings}
+        try:
+            os.makedirs(os.path.dirname(path), exist_ok=True)
+            json.dump(cache, open(path, "w"), indent=1)
+        except OSError:
+            pass
+    return best, timings

@@ -13,9 +13,9 @@ numpy ``RandomState`` in this order per step:
 1. one ``random_sample()`` choosing the move (the move list has one entry);
 2. ``shuffle`` of the half labels ``arange(nwalkers) % 2``;
 3. for each half in turn: ``rand(Ns)`` for the stretch factors,
-   ``randint(Nc, size=Ns)`` for the partner walkers, then one ``rand()``
-   per proposed walker for the Metropolis test; the half is updated before
-   the other half proposes.
+   ``randint(Nc, size=Ns)`` for the partner walkers, then one uniform per
+   proposed walker, in walker order, for the Metropolis test; the half is
+   updated before the other half proposes.
 
 Only this order is pinned, by the 3D known values of the reference tests
 (``tests/test_randmeth.py:43-46``, ``tests/test_incomprrandmeth.py:40-44``).
@@ -44,8 +44,8 @@ class StretchEnsemble:
             raise ValueError("At least one parameter value was infinite")
         if np.any(np.isnan(coords)):
             raise ValueError("At least one parameter value was NaN")
-        res = self.ln_pdf(coords)
-        log_prob = np.array([float(np.asarray(v).reshape(-1)[0]) for v in res])
+        res = np.asarray(self.ln_pdf(coords), dtype=np.double)
+        log_prob = res.reshape(len(coords), -1)[:, 0].copy()
         if np.any(np.isnan(log_prob)):
             raise ValueError("Probability function returned NaN")
         return log_prob
@@ -69,10 +69,11 @@ class StretchEnsemble:
             rint = rs.randint(nc, size=(ns,))
             q = c[rint] - (c[rint] - s) * zz[:, None]
             new_lp = self._log_prob(q)
-            for j, f, nlp in zip(all_inds[sel], factors, new_lp):
-                lnpdiff = f + nlp - log_prob[j]
-                if lnpdiff > np.log(rs.rand()):
-                    accepted[j] = True
+            # one uniform per proposed walker, in walker order (rand(ns) draws
+            # the same sequence as ns scalar rand() calls)
+            lnpdiff = factors + new_lp - log_prob[sel]
+            with np.errstate(invalid="ignore"):
+                accepted[sel] = lnpdiff > np.log(rs.rand(ns))
             m1 = sel & accepted
             m2 = accepted[sel]
             coords[m1] = q[m2]

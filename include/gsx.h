@@ -27,8 +27,11 @@
  *   - outputs are the UNSCALED sum (the reference's summate); callers apply
  *     sqrt(var / N) as RandMeth.__call__ does (generator.py:270);
  *   - work is enqueued on `stream` (a hipStream_t, 0 = null stream) and the
- *     call returns without synchronising; no call allocates device memory;
- *     calls on different streams are independent;
+ *     call returns without synchronising; no call allocates device memory
+ *     except gsx_batch_plan_create (the plan's descriptors) -- scratch space
+ *     comes from a caller-provided workspace; calls on different streams are
+ *     independent (calls sharing one workspace must share one stream);
+ *   - `shape` arrays are HOST memory;
  *   - the return value is GSX_OK (0) or a negative GSX_E_* status;
  *     gsx_status_string() names it.
  */
@@ -141,11 +144,15 @@ typedef struct gsx_batch_item {
     float* out;         /* (P,) or (d, P), unscaled                        */
 } gsx_batch_item;
 
-/* Build a plan for `count` items; tile = points per work tile (0 = auto).
- * The plan owns a device copy of the descriptors and a tile queue. */
+/* Build a plan for `count` items (host array of descriptors whose pointers
+ * are device pointers).  tile_points = points per work tile (512, 1024 or
+ * 2048; 0 = 1024); mode_chunk = modes staged per LDS chunk (128 or 256;
+ * 0 = 256).  The plan owns a device copy of the descriptors and of the tile
+ * queue, ordered longest-processing-time first (descending N). */
 typedef struct gsx_batch_plan gsx_batch_plan;
 int gsx_batch_plan_create(const gsx_batch_item* items, int64_t count,
-                          int64_t tile, gsx_batch_plan** plan);
+                          int64_t tile_points, int64_t mode_chunk,
+                          gsx_batch_plan** plan);
 /* One persistent launch over every tile of every item. */
 int gsx_batch_plan_launch(gsx_batch_plan* plan, void* stream);
 /* Capture the launch into a HIP graph once, then replay it. */

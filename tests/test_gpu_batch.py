@@ -1,0 +1,113 @@
+"""GPU tests of the batched persistent launch and its HIP-graph replay.
+
+The batch tiles run the same FP32 instruction sequence, on the same 64-mode
+sub-blocks, as the single-realization FP32 kernels, so every item must equal
+its own single launch bit for bit; each is also checked against the oracle
+(<= 1e-5 relative).  Graph replays are interleaved with eager work and every
+replay is compared with the eager result.
+"""
+
+import sys
+
+import numpy as np
+import pytest
+import torch
+
+import oracle
+from gstools_amd import summate_device, summate_grid_device, summate_incompr_device
+from gstools_amd.batch import TILE_VARIANTS, BatchItem, BatchPlan, autotune
+from gstools_amd.geometry import generate_grid
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def _mixed_items(seed=0):
+    rs = np.random.RandomState(seed)
+    specs = []
+    # (d, N, kind, points or axes)
+    specs.append((2, 1000, False, None, [np.arange(64.0), np.arange(70.0)]))
+    specs.append((3, 300, False, rs.uniform(0, 1000, size=(3, 5003)) / 7.0, None))
+    specs.append((3, 257, True, None, [np.arange(9.0), np.arange(10.0), np.arange(11.0)]))
+    specs.append((2, 700, True, rs.uniform(0, 50, size=(2, 3000)), None))
+    specs.append((1, 129, False, rs.uniform(-5, 5, size=(1, 999)), None))
+    specs.append((2, 64, False, rs.uniform(0, 10, size=(2, 1)), None))
+    items, refs = [], []
+    for d, n, incompr, pos, axes in specs:
+        cov = rs.normal(size=(d, n)) * 0.3
+        z1, z2 = rs.normal(size=n), rs.normal(size=n)
+        items.append(BatchItem(cov, z1, z2, pos=pos, axes=axes, incompr=incompr))
+        p = pos if pos is not None else generate_grid(axes)
+        refs.append((cov, z1, z2, p, axes, incompr))
+    return items, refs
+
+
+def _single(cov, z1, z2, p, axes, incompr):
+    if incompr:
+        return summate_incompr_device(cov, z1, z2, p, "fp32").cpu().numpy()
+    if axes is not None:
+        return summate_grid_device(cov, z1, z2, axes, "fp32").cpu().numpy().reshape(1, -1)
+    return summate_device(cov, z1, z2, p, "fp32").cpu().numpy().reshape(1, -1)
+
+
+@pytest.mark.parametrize("variant", TILE_VARIANTS)
+def test_batch_matches_single_launches(variant):
+    _say(f"batch variant {variant}")
+    items, refs = _mixed_items()
+    plan = BatchPlan(items, *variant)
+    assert plan.tiles >= len(items)
+    plan.launch()
+    torch.cuda.synchronize()
+    for it, (cov, z1, z2, p, axes, incompr) in zip(items, refs):
+        got = it.out.cpu().numpy()
+        single = _single(cov, z1, z2, p, axes, incompr)
+        assert np.array_equal(got, single)
+        ref = oracle.summate_incompr_c(cov, z1, z2, p) if incompr else oracle.summate_c(cov, z1, z2, p)[None]
+        assert oracle.rel_errors(got.astype(np.float64), ref)[0] <= 1e-5
+    plan.close()
+
+
+def test_batch_graph_replays():
+    _say("batch graph replays")
+    items, refs = _mixed_items(seed=1)
+    plan = BatchPlan(items, 1024, 256)
+    plan.launch()
+    torch.cuda.synchronize()
+    eager = [it.out.clone() for it in items]
+    for rep in range(4):
+        for it in items:
+            it.out.fill_(float("nan"))
+        # unrelated eager work between replays
+        _ = summate_device(refs[1][0], refs[1][1], refs[1][2], refs[1][3], "fp32")
+        torch.cuda.synchronize()
+        plan.launch(graph=True)
+        torch.cuda.synchronize()
+        for it, e in zip(items, eager):
+            assert torch.equal(it.out, e), f"replay {rep} differs"
+    plan.close()
+
+
+def test_batch_empty_and_invalid():
+    plan = BatchPlan([], 1024, 256)
+    assert plan.tiles == 0
+    plan.launch()
+    plan.launch(graph=True)
+    torch.cuda.synchronize()
+    plan.close()
+    items, _ = _mixed_items()
+    with pytest.raises(ValueError):
+        BatchPlan(items, 777, 256)
+
+
+def test_autotune_returns_a_variant(tmp_path, monkeypatch):
+    _say("autotune")
+    monkeypatch.setenv("GSX_AUTOTUNE_CACHE", str(tmp_path / "tune.json"))
+    items, _ = _mixed_items(seed=2)
+    best, ms = autotune(items, reps=1)
+    assert best in TILE_VARIANTS and len(ms) == len(TILE_VARIANTS)
+    best2, _ = autotune(items, reps=1)  # from the cache
+    assert best2 == best
