@@ -317,6 +317,26 @@ def _grid_pos(torch, axes):
     return torch.stack([a.reshape(-1) for a in g]).contiguous()
 
 
+def bench_copy(torch):
+    """A plain 1 GiB device copy: the known-size reference for HBM byte counters."""
+    n = 1 << 27  # 2^27 doubles = 1 GiB
+    a = torch.empty(n, dtype=torch.float64, device="cuda")
+    a.fill_(1.0)
+    b = torch.empty_like(a)
+    wall, ker = time_kernel(lambda: b.copy_(a), 5, 2, torch)
+    return {"bytes_read": n * 8, "bytes_written": n * 8, "ms": ker * 1e3, "GBps": 2 * n * 8 / ker / 1e9}
+
+
+def load_traffic():
+    """HBM bytes per launch of the headline kernel, measured with rocprofv3 --pmc."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "traffic.json")
+    try:
+        with open(path) as fh:
+            return json.load(fh).get("k_forward_f32_exponential3d")
+    except (OSError, ValueError):
+        return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -324,6 +344,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--extras", type=int, default=1, help="also run the other configurations")
+    ap.add_argument("--only", choices=["fp32", "fp64", "mfma", "adjoint", "batch", "copy"],
+                    help="run one section only (for profiling) and print its JSON")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (asked for --gpus {args.gpus}); "
@@ -336,11 +358,22 @@ def main():
     from gstools_amd import summate_device
     import oracle
 
-    batch_inputs = batch_modes() if args.extras else None  # host-only, before the GPU is touched
+    want_batch = args.only == "batch" or (args.only is None and args.extras)
+    batch_inputs = batch_modes() if want_batch else None  # host-only, before the GPU is touched
     if not torch.cuda.is_available():
         print("bench.py: no HIP device visible", file=sys.stderr)
         sys.exit(1)
     dev = torch.device("cuda", 0)
+    k_ext = max(3, args.steps // 4)
+    w_ext = max(1, args.warmup)
+    if args.only in ("mfma", "adjoint", "batch", "copy"):
+        fn = {"mfma": lambda: bench_mfma_grid(torch, k_ext, w_ext),
+              "adjoint": lambda: bench_adjoint(torch, k_ext, w_ext),
+              "batch": lambda: bench_batch(torch, k_ext, w_ext, *batch_inputs),
+              "copy": lambda: bench_copy(torch)}[args.only]
+        print(json.dumps({args.only: fn()}), flush=True)
+        return
+
     log("bench: sampling modes and points")
     rm, iso = headline_inputs()
     cov = torch.from_numpy(rm._cov_sample).to(dev)
@@ -350,12 +383,20 @@ def main():
     out32 = torch.empty(P_HEADLINE, dtype=torch.float32, device=dev)
     out64 = torch.empty(P_HEADLINE, dtype=torch.float64, device=dev)
     work = float(P_HEADLINE) * N_HEADLINE
+    f64 = lambda: summate_device(cov, z1, z2, pos, "fp64", out=out64)
+    f32 = lambda: summate_device(cov, z1, z2, pos, "fp32", out=out32)
+    if args.only == "fp64":
+        wall64, ker64 = time_kernel(f64, args.steps, args.warmup, torch)
+        print(json.dumps({"fp64": {"value": work / wall64, "kernel_ms": ker64 * 1e3}}), flush=True)
+        return
+    if args.only == "fp32":
+        wall32, ker32 = time_kernel(f32, args.steps, args.warmup, torch)
+        print(json.dumps({"fp32": {"value": work / wall32, "kernel_ms": ker32 * 1e3}}), flush=True)
+        return
 
     log("bench: fp64 plain")
-    f64 = lambda: summate_device(cov, z1, z2, pos, "fp64", out=out64)
     wall64, ker64 = time_kernel(f64, max(3, args.steps // 4), args.warmup, torch)
     log("bench: fp32")
-    f32 = lambda: summate_device(cov, z1, z2, pos, "fp32", out=out32)
     wall32, ker32 = time_kernel(f32, args.steps, args.warmup, torch)
 
     ref64 = out64.cpu().numpy()
@@ -394,7 +435,7 @@ def main():
             "peak": FP32_VALU_PEAK_TFLOPS,
             "unit": "TFLOP/s",
             "frac": achieved / FP32_VALU_PEAK_TFLOPS,
-            "traffic": None,
+            "traffic": load_traffic(),
             "flops_per_mode_point": flops,
             "kernel_ms": ker32 * 1e3,
         },
@@ -413,7 +454,7 @@ def main():
                     ("batch", lambda t, k, w: bench_batch(t, k, w, *batch_inputs)))
         for name, fn in sections:
             try:
-                extras[name] = fn(torch, max(3, args.steps // 4), max(1, args.warmup))
+                extras[name] = fn(torch, k_ext, w_ext)
             except Exception as exc:  # report, keep the headline line
                 extras[name] = {"error": repr(exc)}
             log(f"bench: {name}: {json.dumps(extras[name])}")
