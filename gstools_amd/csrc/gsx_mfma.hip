@@ -92,18 +92,18 @@ inline int64_t mfma_ws_bytes(const MfmaGeom& g) {
 // ---------------------------------------------------------------- splits ---
 template <bool FP16>
 __device__ __forceinline__ void split2(float a, float b, uint32_t& hi, uint32_t& lo) {
+    // hi = nearest 16-bit values of (a, b), lo = nearest 16-bit values of the
+    // exact remainders.  The float value of hi is taken from the packed word
+    // (v_perm_b32) so the compiler does not convert a and b a second time.
     if constexpr (FP16) {
-        const f16x2 h = {(_Float16)a, (_Float16)b};
-        hi = __builtin_bit_cast(uint32_t, h);
-        const f16x2 l = {(_Float16)(a - (float)h[0]), (_Float16)(b - (float)h[1])};
-        lo = __builtin_bit_cast(uint32_t, l);
+        hi = __builtin_bit_cast(uint32_t, f16x2{(_Float16)a, (_Float16)b});
+        const f16x2 h = __builtin_bit_cast(f16x2, __builtin_amdgcn_perm(0u, hi, 0x03020100u));
+        lo = __builtin_bit_cast(uint32_t, f16x2{(_Float16)(a - (float)h[0]), (_Float16)(b - (float)h[1])});
     } else {
-        const bf16x2 h = {(__bf16)a, (__bf16)b};
-        hi = __builtin_bit_cast(uint32_t, h);
-        const float ha = __builtin_bit_cast(float, hi << 16);
+        hi = __builtin_bit_cast(uint32_t, bf16x2{(__bf16)a, (__bf16)b});
+        const float ha = __builtin_bit_cast(float, __builtin_amdgcn_perm(0u, hi, 0x01000c0cu));
         const float hb = __builtin_bit_cast(float, hi & 0xffff0000u);
-        const bf16x2 l = {(__bf16)(a - ha), (__bf16)(b - hb)};
-        lo = __builtin_bit_cast(uint32_t, l);
+        lo = __builtin_bit_cast(uint32_t, bf16x2{(__bf16)(a - ha), (__bf16)(b - hb)});
     }
 }
 
@@ -347,6 +347,164 @@ __global__ __launch_bounds__(kMWaves * 64, 2) void k_mfma_grid(MfmaGeom g, doubl
     }
 }
 
+// 3D grids with n1, n2 >= 16: block tile = 128 rows x (16 i1 x 16 i2) columns.
+// Each K step stages, through double-buffered LDS: the A tile (hi and lo,
+// 128 rows x 16 element pairs) and the E1 / E2 slices of the block's 16 i1
+// and 16 i2 values (16 modes each), all loaded one step ahead into
+// registers by the whole block.  Layouts are XOR-swizzled so that the 16
+// lanes reading 16 rows hit 16 distinct 16-byte bank groups:
+//   A row r (64 B = 4 chunks):   chunk c stored at c ^ ((r >> 2) & 3)
+//   E row r (128 B = 8 chunks):  chunk c stored at c ^ ((r >> 1) & 7)
+struct alignas(16) Stage3 {
+    uint32_t ah[kMBM * 16];
+    uint32_t al[kMBM * 16];
+    float e1[16 * 32];
+    float e2[16 * 32];
+};
+
+template <bool FP16>
+__global__ __launch_bounds__(kMWaves * 64, 2) void k_mfma_grid3(MfmaGeom g, double alpha,
+                                                                const uint32_t* __restrict__ a_hi,
+                                                                const uint32_t* __restrict__ a_lo,
+                                                                const float2* __restrict__ e1,
+                                                                const float2* __restrict__ e2,
+                                                                float* __restrict__ out) {
+    __shared__ Stage3 st[2];
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int wave = tid >> 6;
+    const int grp = lane >> 4;
+    const int l16 = lane & 15;
+
+    const int64_t b = blockIdx.x;
+    int64_t rt, ct;
+    {
+        const int64_t xcd = b & 7, w = b >> 3;
+        const int64_t R = g.row_tiles;
+        if (R <= 8 && (8 % R) == 0 && (g.col_tiles % (8 / R)) == 0) {
+            const int64_t per = 8 / R;
+            rt = xcd / per;
+            ct = w * per + (xcd % per);
+        } else {
+            rt = b % R;
+            ct = b / R;
+        }
+    }
+    const int64_t tile1 = ct / g.t2, tile2 = ct - tile1 * g.t2;
+    const int64_t i1b = tile1 * 16, i2b = tile2 * 16, row0 = rt * kMBM;
+
+    // cooperative loads: A chunks tid and tid + 256 (row c >> 2, chunk c & 3);
+    // E chunk: threads 0..127 -> E1, 128..255 -> E2 (row (tid & 127) >> 3, chunk tid & 7)
+    const int erow = (tid & 127) >> 3, ech = tid & 7;
+    const bool is_e1 = tid < 128;
+    const int64_t eidx = is_e1 ? (i1b + erow < g.n1 ? i1b + erow : g.n1 - 1)
+                               : (i2b + erow < g.n2 ? i2b + erow : g.n2 - 1);
+    const float2* ebase = (is_e1 ? e1 : e2) + eidx * g.Np + ech * 2;
+    u32x4 pa[2][2];
+    float4 pe;
+    auto load = [&](int64_t step) {
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const int c = tid + h * 256;
+            const int64_t off = (row0 + (c >> 2)) * g.Np + step * kMKM + (c & 3) * 4;
+            pa[h][0] = *reinterpret_cast<const u32x4*>(a_hi + off);
+            pa[h][1] = *reinterpret_cast<const u32x4*>(a_lo + off);
+        }
+        pe = *reinterpret_cast<const float4*>(ebase + step * kMKM);
+    };
+    auto store = [&](int buf) {
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const int c = tid + h * 256;
+            const int r = c >> 2;
+            const int pos = r * 16 + (((c & 3) ^ ((r >> 2) & 3)) * 4);
+            *reinterpret_cast<u32x4*>(&st[buf].ah[pos]) = pa[h][0];
+            *reinterpret_cast<u32x4*>(&st[buf].al[pos]) = pa[h][1];
+        }
+        float* es = is_e1 ? st[buf].e1 : st[buf].e2;
+        *reinterpret_cast<float4*>(es + erow * 32 + ((ech ^ ((erow >> 1) & 7)) * 4)) = pe;
+    };
+
+    f32x4 acc[kMRF][kMCF];
+#pragma unroll
+    for (int rf = 0; rf < kMRF; ++rf)
+#pragma unroll
+        for (int cf = 0; cf < kMCF; ++cf) acc[rf][cf] = f32x4{0.f, 0.f, 0.f, 0.f};
+
+    const int64_t steps = g.Np / kMKM;
+    load(0);
+    store(0);
+    __syncthreads();
+    const int zsw = (l16 >> 1) & 7;
+    for (int64_t s = 0; s < steps; ++s) {
+        const int buf = (int)(s & 1);
+        if (s + 1 < steps) load(s + 1);
+
+        // B fragments: E2 of this lane's i2 (row l16) times E1 of each fragment's i1
+        float2 zf[4];
+        {
+            const float* zr = st[buf].e2 + l16 * 32;
+            const float4 q0 = *reinterpret_cast<const float4*>(zr + (((2 * grp) ^ zsw) * 4));
+            const float4 q1 = *reinterpret_cast<const float4*>(zr + (((2 * grp + 1) ^ zsw) * 4));
+            zf[0] = {q0.x, q0.y};
+            zf[1] = {q0.z, q0.w};
+            zf[2] = {q1.x, q1.y};
+            zf[3] = {q1.z, q1.w};
+        }
+        u32x4 bh[kMCF], bl[kMCF];
+#pragma unroll
+        for (int cf = 0; cf < kMCF; ++cf) {
+            const int er = wave * kMCF + cf;
+            const int ysw = (er >> 1) & 7;
+            const float* yr = st[buf].e1 + er * 32;
+            const float4 q0 = *reinterpret_cast<const float4*>(yr + (((2 * grp) ^ ysw) * 4));
+            const float4 q1 = *reinterpret_cast<const float4*>(yr + (((2 * grp + 1) ^ ysw) * 4));
+            const float2 yf[4] = {{q0.x, q0.y}, {q0.z, q0.w}, {q1.x, q1.y}, {q1.z, q1.w}};
+            uint32_t h4[4], l4[4];
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+                const float2 v = cmul(yf[t], zf[t]);
+                split2<FP16>(v.x, v.y, h4[t], l4[t]);
+            }
+            bh[cf] = u32x4{h4[0], h4[1], h4[2], h4[3]};
+            bl[cf] = u32x4{l4[0], l4[1], l4[2], l4[3]};
+        }
+
+        const int asw = (l16 >> 2) & 3;
+#pragma unroll
+        for (int rf = 0; rf < kMRF; ++rf) {
+            const int pos = (rf * 16 + l16) * 16 + ((grp ^ asw) * 4);
+            const u32x4 ah = *reinterpret_cast<const u32x4*>(&st[buf].ah[pos]);
+            const u32x4 al = *reinterpret_cast<const u32x4*>(&st[buf].al[pos]);
+#pragma unroll
+            for (int cf = 0; cf < kMCF; ++cf) {
+                acc[rf][cf] = mfma<FP16>(ah, bh[cf], acc[rf][cf]);
+                acc[rf][cf] = mfma<FP16>(ah, bl[cf], acc[rf][cf]);
+                acc[rf][cf] = mfma<FP16>(al, bh[cf], acc[rf][cf]);
+            }
+        }
+        if (s + 1 < steps) store(buf ^ 1);
+        __syncthreads();
+    }
+
+    const float fa = (float)alpha;
+    const int64_t i2 = i2b + l16;
+#pragma unroll
+    for (int cf = 0; cf < kMCF; ++cf) {
+        const int64_t i1 = i1b + wave * kMCF + cf;
+        if (i1 >= g.n1 || i2 >= g.n2) continue;
+        const int64_t col = i1 * g.n2 + i2;
+#pragma unroll
+        for (int rf = 0; rf < kMRF; ++rf) {
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const int64_t row = row0 + rf * 16 + grp * 4 + i;
+                if (row < g.n0) out[row * g.ncols + col] = fa * acc[rf][cf][i];
+            }
+        }
+    }
+}
+
 __global__ void k_fill_zero_f32(float* __restrict__ p, int64_t n) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = 0.0f;
@@ -367,7 +525,7 @@ int mfma_launch(const MfmaGeom& g, int64_t d, int64_t N, const double* ax0, cons
     if (st != GSX_OK) return st;
     const int64_t blocks = g.row_tiles * g.col_tiles;
     if (g.g2 == 1) {
-        hipLaunchKernelGGL((k_mfma_grid<FP16, 1>), dim3((unsigned)blocks), dim3(kMWaves * 64), 0, s, g, alpha,
+        hipLaunchKernelGGL((k_mfma_grid3<FP16>), dim3((unsigned)blocks), dim3(kMWaves * 64), 0, s, g, alpha,
                            a_hi, a_lo, e1, e2, out);
     } else {
         hipLaunchKernelGGL((k_mfma_grid<FP16, 16>), dim3((unsigned)blocks), dim3(kMWaves * 64), 0, s, g, alpha,
