@@ -1,17 +1,20 @@
 """gstools_amd -- GSTools' randomisation-method field summation on AMD MI355X (gfx950).
 
-A drop-in for ``RandMeth.__call__`` and the ``summate`` / ``summate_incompr``
-kernels it calls (``src/gstools/field/generator.py:42-64``, ``:243-270``,
-``:529-567`` of GSTools), with the sum computed by hand-written HIP kernels
+A drop-in for ``RandMeth.__call__`` / ``Fourier.__call__`` and the
+``summate`` / ``summate_incompr`` / ``summate_fourier`` kernels they call
+(``src/gstools/field/generator.py:42-75``, ``:243-270``, ``:529-567``,
+``:659-691`` of GSTools), with the sum computed by hand-written HIP kernels
 in ``libgsx.so`` (C ABI: ``include/gsx.h``).
 
 Public surface:
 
-* ``summate``, ``summate_incompr`` -- the backend functions (numpy in/out);
+* ``summate``, ``summate_incompr``, ``summate_fourier`` -- the backend
+  functions (numpy in/out);
 * ``summate_device``, ``summate_incompr_device``, ``summate_grid_device``,
-  ``summate_adjoint_device`` -- torch-tensor forms with FP32, structured-grid,
-  matrix-core and adjoint paths;
-* ``RandMeth``, ``IncomprRandMeth`` -- ``Generator`` plugins;
+  ``summate_adjoint_device``, ``summate_fourier_device``,
+  ``summate_fourier_grid_device`` -- torch-tensor forms with FP32,
+  structured-grid, matrix-core and adjoint paths;
+* ``RandMeth``, ``IncomprRandMeth``, ``Fourier`` -- ``Generator`` plugins;
 * ``SRF`` -- a thin caller (positions -> isometrize -> generator -> mean);
 * ``Gaussian``, ``Exponential``, ``Matern``, ``Stable`` -- the spectral
   members of the covariance models the configurations use;
@@ -19,13 +22,16 @@ Public surface:
 """
 
 from .covmodel import CovModel, Exponential, Gaussian, Matern, Stable  # noqa: F401
-from .generator import Generator, IncomprRandMeth, RandMeth  # noqa: F401
+from .generator import Fourier, Generator, IncomprRandMeth, RandMeth  # noqa: F401
 from .rng import RNG, MasterRNG  # noqa: F401
 from .srf import SRF  # noqa: F401
 from .summate import (  # noqa: F401
     summate,
     summate_adjoint_device,
     summate_device,
+    summate_fourier,
+    summate_fourier_device,
+    summate_fourier_grid_device,
     summate_grid_device,
     summate_incompr,
     summate_incompr_device,

@@ -48,6 +48,8 @@ class BatchItem(ctypes.Structure):
 
 _FWD = [_i64, _i64, _i64, _vp, _vp, _vp, _vp, _vp, _vp]
 _GRID = [_i64, _i64, _i64p, _vp, _vp, _vp, _vp, _vp, _vp]
+_FOUR = [_i64, _i64, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp]
+_FOUR_GRID = [_i64, _i64, _i64p, _vp, _vp, _vp, _vp, _vp, _vp, _vp]
 _ADJ = [_i64, _i64, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i64, _vp]
 
 # name -> (restype, argtypes); every function declared in include/gsx.h
@@ -62,6 +64,12 @@ EXPORTS = {
     "gsx_summate_grid_f32": (_i32, _GRID),
     "gsx_summate_grid_mfma_workspace": (_i64, [_i64, _i64, _i64p]),
     "gsx_summate_grid_mfma": (_i32, [_i64, _i64, _i64p, _vp, _vp, _vp, _vp, _i32, _f64, _vp, _vp, _i64, _vp]),
+    "gsx_summate_fourier_f64": (_i32, _FOUR),
+    "gsx_summate_fourier_f32": (_i32, _FOUR),
+    "gsx_summate_fourier_grid_f64": (_i32, _FOUR_GRID),
+    "gsx_summate_fourier_grid_f32": (_i32, _FOUR_GRID),
+    "gsx_summate_fourier_grid_mfma": (_i32, [_i64, _i64, _i64p, _vp, _vp, _vp, _vp, _vp, _i32, _f64, _vp, _vp, _i64,
+                                             _vp]),
     "gsx_summate_adjoint_workspace": (_i64, [_i64, _i64, _i64]),
     "gsx_summate_adjoint_f64": (_i32, _ADJ),
     "gsx_summate_adjoint_f32": (_i32, _ADJ),

@@ -183,6 +183,10 @@ class CovModel:
     def needs_fourier_transform(self):
         return type(self).spectral_density is CovModel.spectral_density
 
+    def spectrum(self, k):
+        """Spectrum ``var * spectral_density(k)`` (``covmodel/base.py:511-532``)."""
+        return self.spectral_density(k) * self.var
+
     def spectral_density(self, k):
         """Spectral density by the radial Fourier transform of the correlation."""
         k = np.asarray(np.abs(k), dtype=np.double)

@@ -117,6 +117,7 @@ __global__ __launch_bounds__(256) void k_mfma_tables(MfmaGeom g, int64_t d, int6
                                                      const double* __restrict__ cov,
                                                      const double* __restrict__ z1,
                                                      const double* __restrict__ z2,
+                                                     const double* __restrict__ fac,
                                                      uint32_t* __restrict__ a_hi,
                                                      uint32_t* __restrict__ a_lo,
                                                      float2* __restrict__ e1,
@@ -161,7 +162,8 @@ __global__ __launch_bounds__(256) void k_mfma_tables(MfmaGeom g, int64_t d, int6
             const double kz = cov[(d - 1) * N + j];
             const double th = kz * (d == 3 ? ax2[i2] : ax1[i2]);
             if (fabs(th) <= kFastPhaseMax) sincos_fast(th, &s, &c); else sincos(th, &s, &c);
-            const double a = z1[j], b = z2[j];
+            const double f = fac ? fac[j] : 1.0;
+            const double a = f * z1[j], b = f * z2[j];
             // (a - i b)(c + i s)
             v = {(float)(a * c + b * s), (float)(a * s - b * c)};
         }
@@ -512,15 +514,15 @@ __global__ void k_fill_zero_f32(float* __restrict__ p, int64_t n) {
 
 template <bool FP16>
 int mfma_launch(const MfmaGeom& g, int64_t d, int64_t N, const double* ax0, const double* ax1,
-                const double* ax2, const double* cov, const double* z1, const double* z2, double alpha,
-                float* out, void* ws, hipStream_t s) {
+                const double* ax2, const double* cov, const double* z1, const double* z2, const double* fac,
+                double alpha, float* out, void* ws, hipStream_t s) {
     uint32_t* a_hi = static_cast<uint32_t*>(ws);
     uint32_t* a_lo = a_hi + g.a_elems / 2;
     float2* e1 = reinterpret_cast<float2*>(a_lo + g.a_elems / 2);
     float2* e2 = e1 + g.e1_elems;
     const int64_t entries = g.n0p * g.Np + g.e1_elems + g.e2_elems;
     hipLaunchKernelGGL((k_mfma_tables<FP16>), dim3((unsigned)ceil_div(entries, 256)), dim3(256), 0, s, g, d,
-                       N, ax0, ax1, ax2, cov, z1, z2, a_hi, a_lo, e1, e2);
+                       N, ax0, ax1, ax2, cov, z1, z2, fac, a_hi, a_lo, e1, e2);
     int st = launch_status();
     if (st != GSX_OK) return st;
     const int64_t blocks = g.row_tiles * g.col_tiles;
@@ -532,6 +534,35 @@ int mfma_launch(const MfmaGeom& g, int64_t d, int64_t N, const double* ax0, cons
                            a_hi, a_lo, e1, e2, out);
     }
     return launch_status();
+}
+
+int mfma_entry(int64_t d, int64_t N, const int64_t* shape, const double* axes, const double* cov,
+               const double* z1, const double* z2, const double* fac, int variant, double alpha, float* out,
+               void* workspace, int64_t workspace_bytes, void* stream) {
+    if (d == 1) return GSX_E_UNSUPPORTED;
+    if ((d != 2 && d != 3) || N < 0 || !shape) return GSX_E_INVALID;
+    if (variant != GSX_MFMA_BF16X3 && variant != GSX_MFMA_FP16X3) return GSX_E_INVALID;
+    int64_t P = 1;
+    for (int i = 0; i < d; ++i) {
+        if (shape[i] < 0) return GSX_E_INVALID;
+        P *= shape[i];
+    }
+    if (P == 0) return GSX_OK;
+    if (!axes || !out || (N > 0 && (!cov || !z1 || !z2))) return GSX_E_INVALID;
+    hipStream_t s = as_stream(stream);
+    if (N == 0) {  // empty sum
+        hipLaunchKernelGGL(k_fill_zero_f32, dim3((unsigned)ceil_div(P, 256)), dim3(256), 0, s, out, P);
+        return launch_status();
+    }
+    const MfmaGeom g = mfma_geom(d, N, shape);
+    if (g.row_tiles * g.col_tiles > 0x7fffffffLL) return GSX_E_INVALID;
+    if (!workspace || workspace_bytes < mfma_ws_bytes(g)) return GSX_E_WORKSPACE;
+    const double* ax0 = axes;
+    const double* ax1 = axes + shape[0];
+    const double* ax2 = d == 3 ? ax1 + shape[1] : nullptr;
+    if (variant == GSX_MFMA_FP16X3)
+        return mfma_launch<true>(g, d, N, ax0, ax1, ax2, cov, z1, z2, fac, alpha, out, workspace, s);
+    return mfma_launch<false>(g, d, N, ax0, ax1, ax2, cov, z1, z2, fac, alpha, out, workspace, s);
 }
 
 }  // namespace gsx
@@ -549,30 +580,18 @@ int64_t gsx_summate_grid_mfma_workspace(int64_t d, int64_t N, const int64_t* sha
 int gsx_summate_grid_mfma(int64_t d, int64_t N, const int64_t* shape, const double* axes, const double* cov,
                           const double* z1, const double* z2, int variant, double alpha, float* out,
                           void* workspace, int64_t workspace_bytes, void* stream) {
+    return gsx::mfma_entry(d, N, shape, axes, cov, z1, z2, nullptr, variant, alpha, out, workspace,
+                           workspace_bytes, stream);
+}
+
+int gsx_summate_fourier_grid_mfma(int64_t d, int64_t N, const int64_t* shape, const double* axes,
+                                  const double* spectrum_factor, const double* modes, const double* z1,
+                                  const double* z2, int variant, double alpha, float* out, void* workspace,
+                                  int64_t workspace_bytes, void* stream) {
     if (d == 1) return GSX_E_UNSUPPORTED;
-    if ((d != 2 && d != 3) || N < 0 || !shape) return GSX_E_INVALID;
-    if (variant != GSX_MFMA_BF16X3 && variant != GSX_MFMA_FP16X3) return GSX_E_INVALID;
-    int64_t P = 1;
-    for (int i = 0; i < d; ++i) {
-        if (shape[i] < 0) return GSX_E_INVALID;
-        P *= shape[i];
-    }
-    if (P == 0) return GSX_OK;
-    if (!axes || !out || (N > 0 && (!cov || !z1 || !z2))) return GSX_E_INVALID;
-    hipStream_t s = gsx::as_stream(stream);
-    if (N == 0) {  // empty sum
-        hipLaunchKernelGGL(gsx::k_fill_zero_f32, dim3((unsigned)gsx::ceil_div(P, 256)), dim3(256), 0, s, out, P);
-        return gsx::launch_status();
-    }
-    const gsx::MfmaGeom g = gsx::mfma_geom(d, N, shape);
-    if (g.row_tiles * g.col_tiles > 0x7fffffffLL) return GSX_E_INVALID;
-    if (!workspace || workspace_bytes < gsx::mfma_ws_bytes(g)) return GSX_E_WORKSPACE;
-    const double* ax0 = axes;
-    const double* ax1 = axes + shape[0];
-    const double* ax2 = d == 3 ? ax1 + shape[1] : nullptr;
-    if (variant == GSX_MFMA_FP16X3)
-        return gsx::mfma_launch<true>(g, d, N, ax0, ax1, ax2, cov, z1, z2, alpha, out, workspace, s);
-    return gsx::mfma_launch<false>(g, d, N, ax0, ax1, ax2, cov, z1, z2, alpha, out, workspace, s);
+    if (N > 0 && !spectrum_factor) return GSX_E_INVALID;
+    return gsx::mfma_entry(d, N, shape, axes, modes, z1, z2, spectrum_factor, variant, alpha, out, workspace,
+                           workspace_bytes, stream);
 }
 
 }  // extern "C"

@@ -16,6 +16,10 @@
 //    z1 cos(t) + z2 sin(t) = a cos(t - phi) needs ONE cosine per term;
 //  * every lane reads the same table entry (LDS broadcast, no bank conflict).
 //
+// The Fourier method (generator.py:659-691) is the same sum with a per-mode
+// spectrum factor f_j multiplying each term; it is folded into the amplitude
+// when the table is built (fac == nullptr for the randomisation method).
+//
 // FP64 path: the phase is the reference's exact product-then-add sequence;
 // cos(t - phi) uses the two-term Cody-Waite reduction of gsx_math.h with phi
 // folded into the reduction (no rounding of t - phi at large |t|).  A wave
@@ -65,6 +69,7 @@ __global__ __launch_bounds__(kThreads) void k_forward_f64(Pts pts, int64_t N, in
                                                           const double* __restrict__ cov,
                                                           const double* __restrict__ z1,
                                                           const double* __restrict__ z2,
+                                                          const double* __restrict__ fac,
                                                           double* __restrict__ out) {
     using T = TabF64<D, INCOMPR>;
     constexpr int NOUT = INCOMPR ? D : 1;
@@ -110,7 +115,7 @@ __global__ __launch_bounds__(kThreads) void k_forward_f64(Pts pts, int64_t N, in
             }
             const double u1 = z1[j], u2 = z2[j];
             const double phi = atan2(u2, u1);
-            const double a = hypot(u1, u2);
+            const double a = fac ? fac[j] * hypot(u1, u2) : hypot(u1, u2);
             if constexpr (INCOMPR) {
                 t[D] = phi;
                 t[D + 1] = phi * kInvPi;
@@ -193,6 +198,7 @@ __global__ __launch_bounds__(kThreads) void k_forward_f32(Pts pts, int64_t N, in
                                                           const double* __restrict__ cov,
                                                           const double* __restrict__ z1,
                                                           const double* __restrict__ z2,
+                                                          const double* __restrict__ fac,
                                                           float* __restrict__ out) {
     constexpr int NOUT = INCOMPR ? D : 1;
     constexpr int kWd = D + 1;  // k/2pi [D], -phi/2pi
@@ -230,7 +236,7 @@ __global__ __launch_bounds__(kThreads) void k_forward_f32(Pts pts, int64_t N, in
             }
             const double u1 = z1[j], u2 = z2[j];
             tabd[threadIdx.x * kWd + D] = -atan2(u2, u1) * kInv2Pi;
-            const double a = hypot(u1, u2);
+            const double a = fac ? fac[j] * hypot(u1, u2) : hypot(u1, u2);
             if constexpr (INCOMPR) {
 #pragma unroll
                 for (int c = 0; c < D; ++c)
@@ -291,37 +297,38 @@ constexpr int kPptF32 = 4;
 
 template <int D, bool INCOMPR, class Pts, class OutT>
 int launch_forward(Pts pts, int64_t N, int64_t P, const double* cov, const double* z1,
-                   const double* z2, OutT* out, hipStream_t s) {
+                   const double* z2, const double* fac, OutT* out, hipStream_t s) {
     if constexpr (sizeof(OutT) == 8) {
         const int64_t blocks = ceil_div(P, (int64_t)kThreads * kPptF64);
         hipLaunchKernelGGL((k_forward_f64<D, kPptF64, INCOMPR, Pts>), dim3((unsigned)blocks),
-                           dim3(kThreads), 0, s, pts, N, P, cov, z1, z2, out);
+                           dim3(kThreads), 0, s, pts, N, P, cov, z1, z2, fac, out);
     } else {
         const int64_t blocks = ceil_div(P, (int64_t)kThreads * kPptF32);
         hipLaunchKernelGGL((k_forward_f32<D, kPptF32, INCOMPR, Pts>), dim3((unsigned)blocks),
-                           dim3(kThreads), 0, s, pts, N, P, cov, z1, z2, out);
+                           dim3(kThreads), 0, s, pts, N, P, cov, z1, z2, fac, out);
     }
     return launch_status();
 }
 
 template <bool INCOMPR, class OutT>
 int forward_unstructured(int64_t d, int64_t N, int64_t P, const double* cov, const double* z1,
-                         const double* z2, const double* pos, OutT* out, void* stream) {
+                         const double* z2, const double* fac, const double* pos, OutT* out,
+                         void* stream) {
     if (d < (INCOMPR ? 2 : 1) || d > 3 || N < 0 || P < 0) return GSX_E_INVALID;
     if (P == 0) return GSX_OK;
     if (!pos || !out || (N > 0 && (!cov || !z1 || !z2))) return GSX_E_INVALID;
     if (P > (int64_t)0x7fffffff * kThreads) return GSX_E_INVALID;
     hipStream_t s = as_stream(stream);
     switch (d) {
-        case 1: return launch_forward<1, INCOMPR>(UnstructuredPoints<1>{pos, P}, N, P, cov, z1, z2, out, s);
-        case 2: return launch_forward<2, INCOMPR>(UnstructuredPoints<2>{pos, P}, N, P, cov, z1, z2, out, s);
-        default: return launch_forward<3, INCOMPR>(UnstructuredPoints<3>{pos, P}, N, P, cov, z1, z2, out, s);
+        case 1: return launch_forward<1, INCOMPR>(UnstructuredPoints<1>{pos, P}, N, P, cov, z1, z2, fac, out, s);
+        case 2: return launch_forward<2, INCOMPR>(UnstructuredPoints<2>{pos, P}, N, P, cov, z1, z2, fac, out, s);
+        default: return launch_forward<3, INCOMPR>(UnstructuredPoints<3>{pos, P}, N, P, cov, z1, z2, fac, out, s);
     }
 }
 
 template <class OutT>
 int forward_grid(int64_t d, int64_t N, const int64_t* shape, const double* axes, const double* cov,
-                 const double* z1, const double* z2, OutT* out, void* stream) {
+                 const double* z1, const double* z2, const double* fac, OutT* out, void* stream) {
     if (d < 1 || d > 3 || N < 0 || !shape) return GSX_E_INVALID;
     int64_t P = 1;
     for (int i = 0; i < d; ++i) {
@@ -338,9 +345,9 @@ int forward_grid(int64_t d, int64_t N, const int64_t* shape, const double* axes,
     const uint32_t n1 = d > 1 ? (uint32_t)shape[1] : 1u;
     const uint32_t n2 = d > 2 ? (uint32_t)shape[2] : 1u;
     switch (d) {
-        case 1: return launch_forward<1, false>(GridPoints<1>{a0, a1, a2, n1, n2}, N, P, cov, z1, z2, out, s);
-        case 2: return launch_forward<2, false>(GridPoints<2>{a0, a1, a2, n1, n2}, N, P, cov, z1, z2, out, s);
-        default: return launch_forward<3, false>(GridPoints<3>{a0, a1, a2, n1, n2}, N, P, cov, z1, z2, out, s);
+        case 1: return launch_forward<1, false>(GridPoints<1>{a0, a1, a2, n1, n2}, N, P, cov, z1, z2, fac, out, s);
+        case 2: return launch_forward<2, false>(GridPoints<2>{a0, a1, a2, n1, n2}, N, P, cov, z1, z2, fac, out, s);
+        default: return launch_forward<3, false>(GridPoints<3>{a0, a1, a2, n1, n2}, N, P, cov, z1, z2, fac, out, s);
     }
 }
 
@@ -350,34 +357,63 @@ extern "C" {
 
 int gsx_summate_f64(int64_t d, int64_t N, int64_t P, const double* cov, const double* z1,
                     const double* z2, const double* pos, double* out, void* stream) {
-    return gsx::forward_unstructured<false>(d, N, P, cov, z1, z2, pos, out, stream);
+    return gsx::forward_unstructured<false>(d, N, P, cov, z1, z2, nullptr, pos, out, stream);
 }
 
 int gsx_summate_f32(int64_t d, int64_t N, int64_t P, const double* cov, const double* z1,
                     const double* z2, const double* pos, float* out, void* stream) {
-    return gsx::forward_unstructured<false>(d, N, P, cov, z1, z2, pos, out, stream);
+    return gsx::forward_unstructured<false>(d, N, P, cov, z1, z2, nullptr, pos, out, stream);
 }
 
 int gsx_summate_incompr_f64(int64_t d, int64_t N, int64_t P, const double* cov, const double* z1,
                             const double* z2, const double* pos, double* out, void* stream) {
-    return gsx::forward_unstructured<true>(d, N, P, cov, z1, z2, pos, out, stream);
+    return gsx::forward_unstructured<true>(d, N, P, cov, z1, z2, nullptr, pos, out, stream);
 }
 
 int gsx_summate_incompr_f32(int64_t d, int64_t N, int64_t P, const double* cov, const double* z1,
                             const double* z2, const double* pos, float* out, void* stream) {
-    return gsx::forward_unstructured<true>(d, N, P, cov, z1, z2, pos, out, stream);
+    return gsx::forward_unstructured<true>(d, N, P, cov, z1, z2, nullptr, pos, out, stream);
 }
 
 int gsx_summate_grid_f64(int64_t d, int64_t N, const int64_t* shape, const double* axes,
                          const double* cov, const double* z1, const double* z2, double* out,
                          void* stream) {
-    return gsx::forward_grid(d, N, shape, axes, cov, z1, z2, out, stream);
+    return gsx::forward_grid(d, N, shape, axes, cov, z1, z2, nullptr, out, stream);
 }
 
 int gsx_summate_grid_f32(int64_t d, int64_t N, const int64_t* shape, const double* axes,
                          const double* cov, const double* z1, const double* z2, float* out,
                          void* stream) {
-    return gsx::forward_grid(d, N, shape, axes, cov, z1, z2, out, stream);
+    return gsx::forward_grid(d, N, shape, axes, cov, z1, z2, nullptr, out, stream);
+}
+
+// Fourier method: the randomisation sum with per-mode spectrum factors.
+int gsx_summate_fourier_f64(int64_t d, int64_t N, int64_t P, const double* spectrum_factor,
+                            const double* modes, const double* z1, const double* z2, const double* pos,
+                            double* out, void* stream) {
+    if (N > 0 && !spectrum_factor) return GSX_E_INVALID;
+    return gsx::forward_unstructured<false>(d, N, P, modes, z1, z2, spectrum_factor, pos, out, stream);
+}
+
+int gsx_summate_fourier_f32(int64_t d, int64_t N, int64_t P, const double* spectrum_factor,
+                            const double* modes, const double* z1, const double* z2, const double* pos,
+                            float* out, void* stream) {
+    if (N > 0 && !spectrum_factor) return GSX_E_INVALID;
+    return gsx::forward_unstructured<false>(d, N, P, modes, z1, z2, spectrum_factor, pos, out, stream);
+}
+
+int gsx_summate_fourier_grid_f64(int64_t d, int64_t N, const int64_t* shape, const double* axes,
+                                 const double* spectrum_factor, const double* modes, const double* z1,
+                                 const double* z2, double* out, void* stream) {
+    if (N > 0 && !spectrum_factor) return GSX_E_INVALID;
+    return gsx::forward_grid(d, N, shape, axes, modes, z1, z2, spectrum_factor, out, stream);
+}
+
+int gsx_summate_fourier_grid_f32(int64_t d, int64_t N, const int64_t* shape, const double* axes,
+                                 const double* spectrum_factor, const double* modes, const double* z1,
+                                 const double* z2, float* out, void* stream) {
+    if (N > 0 && !spectrum_factor) return GSX_E_INVALID;
+    return gsx::forward_grid(d, N, shape, axes, modes, z1, z2, spectrum_factor, out, stream);
 }
 
 }  // extern "C"

@@ -1,12 +1,14 @@
 """Randomisation-method generators whose summation runs on the GPU.
 
-:class:`RandMeth` and :class:`IncomprRandMeth` mirror the reference classes
-(``src/gstools/field/generator.py:165-446`` and ``:449-592``): same
+:class:`RandMeth`, :class:`IncomprRandMeth` and :class:`Fourier` mirror the
+reference classes (``src/gstools/field/generator.py:165-446``, ``:449-592``
+and ``:595-919``): same
 constructor keywords, same ``update`` / ``reset_seed`` / ``seed`` /
 ``mode_no`` / ``sampling`` / ``model`` semantics, same private mode arrays
 ``_cov_sample`` / ``_z_1`` / ``_z_2`` drawn from the same ``RNG`` streams,
-and the same scaling ``sqrt(var / mode_no) * summed + nugget``.  The sum
-itself is the gfx950 kernel (``gstools_amd.summate``).
+and the same scaling ``sqrt(var / mode_no) * summed + nugget`` (Fourier: the
+spectrum factors, no global scale).  The sum itself is the gfx950 kernel
+(``gstools_amd.summate``).
 
 Plugging into the reference: when ``gstools`` is importable the classes
 derive from its ``Generator`` ABC, so ``gstools.SRF(model,
@@ -39,7 +41,7 @@ try:  # register with the reference's plugin seam when it is installed
 except Exception:  # pragma: no cover - the reference is not importable here
     _RefGenerator = None
 
-__all__ = ["Generator", "RandMeth", "IncomprRandMeth", "SAMPLING"]
+__all__ = ["Generator", "RandMeth", "IncomprRandMeth", "Fourier", "SAMPLING"]
 
 SAMPLING = ["auto", "inversion", "mcmc"]
 
@@ -328,3 +330,224 @@ class IncomprRandMeth(RandMeth):
         e1 = np.zeros(shape)
         e1[axis] = 1.0
         return e1
+
+
+class Fourier(Generator):
+    """Fourier method for periodic fields (``generator.py:595-919``).
+
+    ``u(x) = sum_i sqrt(S(|k_i|) prod(dk)) (z1_i cos(k_i.x) + z2_i sin(k_i.x))``
+    over the regular wavenumber grid ``k`` of ``mode_no`` points per axis with
+    spacing ``dk = 2 pi / period * [1, anis]``; ``S`` is the model's
+    spectrum.  Only even ``mode_no`` are accepted, as in the reference.
+    """
+
+    def __init__(self, model, period, mode_no=32, seed=None, precision="fp64", **kwargs):
+        if kwargs:
+            warnings.warn("gstools.Fourier: **kwargs are ignored")
+        self._modes = None
+        self._period = None
+        self._mode_no = None
+        self._delta_k = None
+        self._model = None
+        self._seed = None
+        self._rng = None
+        self._z_1 = None
+        self._z_2 = None
+        self._spectrum_factor = None
+        self._value_type = "scalar"
+        self._dev_modes = None
+        self.precision = precision
+        self.update(model, seed, period, mode_no)
+
+    # --------------------------------------------------------------- call --
+    def _device_modes(self):
+        dev = torch.device("cuda", torch.cuda.current_device())
+        key = (id(self._spectrum_factor), id(self._modes), id(self._z_1), id(self._z_2), dev)
+        if self._dev_modes is None or self._dev_modes[0] != key:
+            tens = tuple(torch.from_numpy(np.ascontiguousarray(a, dtype=np.float64)).to(dev)
+                         for a in (self._spectrum_factor, self._modes, self._z_1, self._z_2))
+            self._dev_modes = (key, tens)
+        return self._dev_modes[1]
+
+    def __call__(self, pos, add_nugget=True):
+        """Field at ``pos`` (d, n): ``summate_fourier(...) + nugget`` (``generator.py:659-691``).
+
+        numpy input gives a float64 numpy result; a torch CUDA tensor gives a
+        tensor on the device.
+        """
+        as_tensor = isinstance(pos, torch.Tensor)
+        if not as_tensor:
+            pos = np.asarray(pos, dtype=np.double)
+        if self.zero_var:
+            shp = tuple(pos.shape[1:])
+            return self.get_nugget(shp) if add_nugget else np.full(shp, 0.0)
+        fac, modes, z1, z2 = self._device_modes()
+        field = summate_device(modes, z1, z2, pos, precision=self.precision, spectrum_factor=fac)
+        field = field.to(torch.float64)
+        if not as_tensor:
+            field = field.cpu().numpy()
+        nugget = self.get_nugget(tuple(field.shape)) if add_nugget else 0.0
+        if as_tensor and not np.isscalar(nugget):
+            nugget = torch.as_tensor(nugget, device=field.device)
+        return field + nugget
+
+    def structured(self, axes, precision=None, add_nugget=True):
+        """Field on the 'ij' meshgrid of ``axes`` without materialising it.
+
+        As :meth:`RandMeth.structured`: ``"bf16x3"`` / ``"fp16x3"`` run on the
+        matrix cores (d = 2 or 3); returns a torch tensor shaped like the grid.
+        """
+        precision = precision or self.precision
+        if precision not in GRID_PRECISIONS:
+            raise ValueError(f"precision must be one of {GRID_PRECISIONS}")
+        shape = tuple(int(np.size(a)) for a in axes)
+        if self.zero_var:
+            out = torch.zeros(shape, dtype=torch.float64, device="cuda")
+        else:
+            fac, modes, z1, z2 = self._device_modes()
+            out = summate_grid_device(modes, z1, z2, axes, precision, spectrum_factor=fac)
+            if precision in ("fp64", "fp32"):
+                out = out.to(torch.float64)
+            out = out.reshape(shape)
+        if add_nugget and self.model.nugget > 0:
+            out = out + torch.as_tensor(self.get_nugget(shape), device=out.device, dtype=out.dtype)
+        return out
+
+    def get_nugget(self, shape):
+        """Nugget noise (``generator.py:693-713``)."""
+        if self.model.nugget > 0:
+            nugget = np.sqrt(self.model.nugget) * self._rng.random.normal(size=shape)
+        else:
+            nugget = np.full(shape, 0.0) if self.zero_var else 0.0
+        return nugget
+
+    # -------------------------------------------------------------- modes --
+    def update(self, model=None, seed=np.nan, period=None, mode_no=None):
+        """Update model, seed, period and mode numbers (``generator.py:715-789``)."""
+        tmp_model = model if model is not None else self._model
+        dim = tmp_model.dim
+        if period is not None:
+            self._period = self._fill_to_dim(period, dim)
+            anis = np.insert(np.atleast_1d(tmp_model.anis).copy(), 0, 1.0)
+            self._delta_k = 2.0 * np.pi / self._period * anis
+            if mode_no is None:
+                self._set_modes(self._mode_no, dim)
+        if mode_no is not None:
+            mode_no = self._fill_to_dim(mode_no, dim)
+            if (np.asarray([m % 2 for m in mode_no]) != 0).any():
+                raise ValueError("Fourier: Odd mode_no not supported.")
+            self._set_modes(mode_no, dim)
+
+        if _is_covmodel(model):
+            if self.model != model:
+                self._model = dcp(model)
+                if seed is None or not np.isnan(seed):
+                    self.reset_seed(seed)
+                else:
+                    self.reset_seed(self._seed)
+            elif seed is None or not np.isnan(seed):
+                self.seed = seed
+        elif model is None and (seed is None or not np.isnan(seed)):
+            if _is_covmodel(self._model):
+                self.seed = seed
+            else:
+                raise ValueError("gstools.field.generator.Fourier: no 'model' given")
+        elif mode_no is not None or period is not None:
+            if seed is None or not np.isnan(seed):
+                self.reset_seed(seed)
+            else:
+                self.reset_seed(self._seed)
+        elif model is None and np.isnan(seed):
+            if (_is_covmodel(self._model) and self._z_1 is not None and self._z_2 is not None
+                    and self._spectrum_factor is not None):
+                raise ValueError("gstools.field.generator.Fourier: neither 'model' nor 'seed' given!")
+        else:
+            raise ValueError("gstools.field.generator.Fourier: 'model' is not an instance of 'gstools.CovModel'")
+
+    def reset_seed(self, seed=np.nan):
+        """Redraw z_1, z_2 and recompute the spectrum factors (``generator.py:791-820``)."""
+        if seed is None or not np.isnan(seed):
+            self._seed = seed
+        self._rng = RNG(self._seed)
+        n = int(np.prod(self._mode_no))
+        self._z_1 = self._rng.random.normal(size=n)
+        self._z_2 = self._rng.random.normal(size=n)
+        if self.zero_var:
+            self._spectrum_factor = np.full(n, 0.0)
+        else:
+            k_norm = np.linalg.norm(self._modes, axis=0)
+            self._spectrum_factor = np.sqrt(self._model.spectrum(k_norm) * np.prod(self._delta_k))
+        self._dev_modes = None
+
+    def _fill_to_dim(self, values, dim, dtype=float, default_value=None):
+        """Pad ``values`` with its last entry up to ``dim`` entries (``generator.py:822-836``)."""
+        r = np.atleast_1d(values)
+        if values is None:
+            if default_value is None:
+                raise ValueError("Fourier: Value has to be provided")
+            r = default_value
+        r = np.atleast_1d(np.array(r, dtype=dtype))[:dim]
+        if len(r) < dim:
+            r = np.pad(r, (0, dim - len(r)), "edge")
+        return r
+
+    def _set_modes(self, mode_no, dim):
+        """Regular wavenumber grid (``generator.py:838-862``): ``arange(-m/2 dk, m/2 dk, dk)`` per axis."""
+        axes = [np.arange(-mode_no[d] / 2.0 * self._delta_k[d], mode_no[d] / 2.0 * self._delta_k[d],
+                          self._delta_k[d]) for d in range(dim)]
+        self._modes = np.asarray(np.meshgrid(*axes, indexing="ij"), dtype=np.double).reshape((dim, -1))
+        self._mode_no = [len(m) for m in axes]
+
+    @property
+    def precision(self):
+        return self._precision
+
+    @precision.setter
+    def precision(self, precision):
+        if precision not in PRECISIONS:
+            raise ValueError(f"Fourier: precision not in {PRECISIONS}.")
+        self._precision = precision
+
+    @property
+    def seed(self):
+        return self._seed
+
+    @seed.setter
+    def seed(self, new_seed):
+        if new_seed is not self._seed:
+            self.reset_seed(new_seed)
+
+    @property
+    def model(self):
+        return self._model
+
+    @model.setter
+    def model(self, model):
+        self.update(model)
+
+    @property
+    def modes(self):
+        return self._modes
+
+    @property
+    def mode_no(self):
+        return self._mode_no
+
+    @mode_no.setter
+    def mode_no(self, mode_no):
+        self.update(mode_no=mode_no)
+
+    @property
+    def period(self):
+        return self._period
+
+    @period.setter
+    def period(self, period):
+        self.update(period=period)
+
+    @property
+    def value_type(self):
+        return self._value_type
+
+    def __repr__(self):
+        return f"{self.name}(model={self.model}, seed={self.seed})"

@@ -15,12 +15,17 @@ matrix cores.
 import numpy as np
 import torch
 
-from .generator import IncomprRandMeth, RandMeth
+from .generator import Fourier, IncomprRandMeth, RandMeth
 from .geometry import generate_grid
 
 __all__ = ["SRF", "GENERATOR"]
 
-GENERATOR = {"RandMeth": RandMeth, "IncomprRandMeth": IncomprRandMeth, "VectorField": IncomprRandMeth}
+GENERATOR = {
+    "RandMeth": RandMeth,
+    "IncomprRandMeth": IncomprRandMeth,
+    "VectorField": IncomprRandMeth,
+    "Fourier": Fourier,
+}
 
 
 class SRF:

@@ -1,11 +1,13 @@
 """Drop-in summation functions backed by the gfx950 kernels.
 
-``summate`` and ``summate_incompr`` keep the signature and semantics of the
-functions GSTools imports from its native backend
-(``src/gstools/field/generator.py:22-24``, dispatched at ``:42-64``):
+``summate``, ``summate_incompr`` and ``summate_fourier`` keep the signature
+and semantics of the functions GSTools imports from its native backend
+(``src/gstools/field/generator.py:22-24``, dispatched at ``:42-75``):
 float64 numpy inputs ``cov_samples (d, N)``, ``z_1 (N,)``, ``z_2 (N,)``,
 ``pos (d, P)``, a fresh float64 result, ``ValueError`` on bad shapes or
 dtypes, and a ``num_threads`` argument that the GPU path accepts and ignores.
+``summate_fourier`` takes the per-mode ``spectrum_factor (N,)`` first and the
+regular wavenumber grid ``modes (d, N)`` in place of ``cov_samples``.
 
 The ``*_device`` functions take and return torch tensors that stay on the
 GPU (numpy inputs are copied over); they add the FP32, structured-grid,
@@ -22,7 +24,8 @@ from . import _native
 
 __all__ = [
     "summate", "summate_incompr", "summate_device", "summate_incompr_device",
-    "summate_grid_device", "summate_adjoint_device", "PRECISIONS", "GRID_PRECISIONS",
+    "summate_grid_device", "summate_adjoint_device", "summate_fourier", "summate_fourier_device",
+    "summate_fourier_grid_device", "PRECISIONS", "GRID_PRECISIONS",
 ]
 
 PRECISIONS = ("fp64", "fp32")
@@ -72,6 +75,15 @@ def _modes(cov_samples, z_1, z_2, dev):
     return cov, z1, z2
 
 
+def _factor(spectrum_factor, n, dev):
+    if spectrum_factor is None:
+        return None
+    f = _as_f64(spectrum_factor, "spectrum_factor", dev).reshape(-1)
+    if f.numel() != n:
+        raise ValueError(f"spectrum_factor must have N={n} entries, got {f.numel()}")
+    return f
+
+
 def _points(pos, d, dev):
     p = _as_f64(pos, "pos", dev)
     if p.dim() == 1:
@@ -81,11 +93,16 @@ def _points(pos, d, dev):
     return p
 
 
-def summate_device(cov_samples, z_1, z_2, pos, precision="fp64", out=None):
-    """Unscaled sum on the GPU -> torch tensor (P,) float64 or float32."""
+def summate_device(cov_samples, z_1, z_2, pos, precision="fp64", out=None, spectrum_factor=None):
+    """Unscaled sum on the GPU -> torch tensor (P,) float64 or float32.
+
+    With ``spectrum_factor`` (N,) every term is multiplied by its factor (the
+    Fourier method's sum).
+    """
     dev = _device()
     cov, z1, z2 = _modes(cov_samples, z_1, z_2, dev)
     d, n = cov.shape
+    fac = _factor(spectrum_factor, n, dev)
     p = _points(pos, d, dev)
     npts = p.shape[1]
     if precision not in PRECISIONS:
@@ -95,7 +112,13 @@ def summate_device(cov_samples, z_1, z_2, pos, precision="fp64", out=None):
         out = torch.empty(npts, dtype=dtype, device=dev)
     elif out.dtype != dtype or out.numel() != npts or not out.is_contiguous():
         raise ValueError("out: wrong dtype, size or layout")
-    fn = _native.lib().gsx_summate_f64 if precision == "fp64" else _native.lib().gsx_summate_f32
+    lib = _native.lib()
+    if fac is not None:
+        fn = lib.gsx_summate_fourier_f64 if precision == "fp64" else lib.gsx_summate_fourier_f32
+        _native.check(fn(d, n, npts, _ptr(fac), _ptr(cov), _ptr(z1), _ptr(z2), _ptr(p), _ptr(out), _stream()),
+                      "summate_fourier")
+        return out
+    fn = lib.gsx_summate_f64 if precision == "fp64" else lib.gsx_summate_f32
     _native.check(fn(d, n, npts, _ptr(cov), _ptr(z1), _ptr(z2), _ptr(p), _ptr(out), _stream()), "summate")
     return out
 
@@ -123,7 +146,8 @@ def summate_incompr_device(cov_samples, z_1, z_2, pos, precision="fp64", out=Non
     return out
 
 
-def summate_grid_device(cov_samples, z_1, z_2, axes, precision="fp64", alpha=1.0, out=None):
+def summate_grid_device(cov_samples, z_1, z_2, axes, precision="fp64", alpha=1.0, out=None,
+                        spectrum_factor=None):
     """Sum on the 'ij' meshgrid of ``axes`` without materialising it.
 
     ``precision``: "fp64" / "fp32" (direct sums; "fp64" reproduces the phases
@@ -131,10 +155,12 @@ def summate_grid_device(cov_samples, z_1, z_2, axes, precision="fp64", alpha=1.0
     GEMM over per-axis phase factors, d = 2 or 3).  Returns a flat tensor of
     prod(shape) values in C order, multiplied by ``alpha`` (matrix-core path
     only; the direct paths return the unscaled sum and require alpha == 1).
+    ``spectrum_factor`` (N,) weights each term (the Fourier method's sum).
     """
     dev = _device()
     cov, z1, z2 = _modes(cov_samples, z_1, z_2, dev)
     d, n = cov.shape
+    fac = _factor(spectrum_factor, n, dev)
     axes = [_as_f64(a, "axes", dev).reshape(-1) for a in axes]
     if len(axes) != d:
         raise ValueError(f"expected {d} axes, got {len(axes)}")
@@ -153,6 +179,11 @@ def summate_grid_device(cov_samples, z_1, z_2, axes, precision="fp64", alpha=1.0
     if precision in ("fp64", "fp32"):
         if alpha != 1.0:
             raise ValueError("alpha applies to the matrix-core path only")
+        if fac is not None:
+            fn = lib.gsx_summate_fourier_grid_f64 if precision == "fp64" else lib.gsx_summate_fourier_grid_f32
+            _native.check(fn(d, n, shape_c, _ptr(flat), _ptr(fac), _ptr(cov), _ptr(z1), _ptr(z2), _ptr(out),
+                             _stream()), "summate_fourier_grid")
+            return out
         fn = lib.gsx_summate_grid_f64 if precision == "fp64" else lib.gsx_summate_grid_f32
         _native.check(fn(d, n, shape_c, _ptr(flat), _ptr(cov), _ptr(z1), _ptr(z2), _ptr(out), _stream()),
                       "summate_grid")
@@ -162,10 +193,34 @@ def summate_grid_device(cov_samples, z_1, z_2, axes, precision="fp64", alpha=1.0
     if ws_bytes < 0:
         _native.check(int(ws_bytes), "summate_grid_mfma")
     ws = torch.empty(max(ws_bytes, 1), dtype=torch.uint8, device=dev)
+    if fac is not None:
+        _native.check(lib.gsx_summate_fourier_grid_mfma(d, n, shape_c, _ptr(flat), _ptr(fac), _ptr(cov), _ptr(z1),
+                                                        _ptr(z2), variant, float(alpha), _ptr(out), _ptr(ws),
+                                                        ws_bytes, _stream()), "summate_fourier_grid_mfma")
+        return out
     _native.check(lib.gsx_summate_grid_mfma(d, n, shape_c, _ptr(flat), _ptr(cov), _ptr(z1), _ptr(z2),
                                             variant, float(alpha), _ptr(out), _ptr(ws), ws_bytes,
                                             _stream()), "summate_grid_mfma")
     return out
+
+
+def summate_fourier_device(spectrum_factor, modes, z_1, z_2, pos, precision="fp64", out=None):
+    """Fourier-method sum on the GPU -> torch tensor (P,).
+
+    ``sum_j f_j (z1_j cos(k_j.x) + z2_j sin(k_j.x))`` with ``f`` =
+    ``spectrum_factor`` (``generator.py:681-688``); unscaled, like the
+    reference's ``summate_fourier``.
+    """
+    if spectrum_factor is None:
+        raise ValueError("spectrum_factor is required")
+    return summate_device(modes, z_1, z_2, pos, precision, out, spectrum_factor=spectrum_factor)
+
+
+def summate_fourier_grid_device(spectrum_factor, modes, z_1, z_2, axes, precision="fp64", alpha=1.0, out=None):
+    """Fourier-method sum on the 'ij' meshgrid of ``axes`` (flat, C order)."""
+    if spectrum_factor is None:
+        raise ValueError("spectrum_factor is required")
+    return summate_grid_device(modes, z_1, z_2, axes, precision, alpha, out, spectrum_factor=spectrum_factor)
 
 
 def summate_adjoint_device(cov_samples, z_1, z_2, pos, g, precision="fp64"):
@@ -208,6 +263,15 @@ def summate(cov_samples, z_1, z_2, pos, num_threads=None):
     """Drop-in for the backend ``summate``: FP64 sum on the GPU -> numpy (P,)."""
     _check_numpy(cov_samples, z_1, z_2, pos)
     return summate_device(cov_samples, z_1, z_2, pos, "fp64").cpu().numpy()
+
+
+def summate_fourier(spectrum_factor, modes, z_1, z_2, pos, num_threads=None):
+    """Drop-in for the backend ``summate_fourier``: FP64 sum on the GPU -> numpy (P,)."""
+    _check_numpy(modes, z_1, z_2, pos)
+    if isinstance(spectrum_factor, np.ndarray) and spectrum_factor.dtype != np.float64:
+        raise ValueError(f"Buffer dtype mismatch for spectrum_factor: expected 'double' but got "
+                         f"'{spectrum_factor.dtype}'")
+    return summate_fourier_device(spectrum_factor, modes, z_1, z_2, pos, "fp64").cpu().numpy()
 
 
 def summate_incompr(cov_samples, z_1, z_2, pos, num_threads=None):

@@ -12,6 +12,10 @@
  *   summate on a meshgrid from Field.pre_pos     gsx_summate_grid_f64 / _f32,
  *     src/gstools/field/base.py:254-297 ->         gsx_summate_grid_mfma
  *     tools/geometric.py:340-356
+ *   summate_fourier(spectrum_factor, modes,       gsx_summate_fourier_f64 / _f32,
+ *     z_1, z_2, pos)                              gsx_summate_fourier_grid_f64 / _f32,
+ *     src/gstools/field/generator.py:23, :67-75,  gsx_summate_fourier_grid_mfma
+ *     call :681-688
  *   (no reference counterpart: gradient of        gsx_summate_adjoint_f64 / _f32
  *    summate for calibration; formula is the
  *    derivative of generator.py:193-199)
@@ -108,6 +112,40 @@ int gsx_summate_grid_mfma(int64_t d, int64_t N, const int64_t* shape,
                           const double* z1, const double* z2, int variant,
                           double alpha, float* out, void* workspace,
                           int64_t workspace_bytes, void* stream);
+
+/* Fourier method: out = sum_j f_j (z1_j cos(k_j.x) + z2_j sin(k_j.x)) with the
+ * per-mode spectrum factor f = spectrum_factor (N,) (Fourier.reset_seed,
+ * generator.py:812-818) and modes (d, N) the regular wavenumber grid.  Same
+ * phase, precision and layout conventions as gsx_summate_f64 / _f32; the
+ * grid forms take `shape` / `axes` like gsx_summate_grid_f64 and
+ * gsx_summate_grid_mfma.  The output is unscaled, as Fourier.__call__ adds
+ * no sqrt(var / N) (generator.py:681-691).  Replaces summate_fourier
+ * (generator.py:67-75). */
+int gsx_summate_fourier_f64(int64_t d, int64_t N, int64_t P,
+                            const double* spectrum_factor, const double* modes,
+                            const double* z1, const double* z2,
+                            const double* pos, double* out, void* stream);
+int gsx_summate_fourier_f32(int64_t d, int64_t N, int64_t P,
+                            const double* spectrum_factor, const double* modes,
+                            const double* z1, const double* z2,
+                            const double* pos, float* out, void* stream);
+int gsx_summate_fourier_grid_f64(int64_t d, int64_t N, const int64_t* shape,
+                                 const double* axes,
+                                 const double* spectrum_factor,
+                                 const double* modes, const double* z1,
+                                 const double* z2, double* out, void* stream);
+int gsx_summate_fourier_grid_f32(int64_t d, int64_t N, const int64_t* shape,
+                                 const double* axes,
+                                 const double* spectrum_factor,
+                                 const double* modes, const double* z1,
+                                 const double* z2, float* out, void* stream);
+int gsx_summate_fourier_grid_mfma(int64_t d, int64_t N, const int64_t* shape,
+                                  const double* axes,
+                                  const double* spectrum_factor,
+                                  const double* modes, const double* z1,
+                                  const double* z2, int variant, double alpha,
+                                  float* out, void* workspace,
+                                  int64_t workspace_bytes, void* stream);
 
 /* Adjoint of summate for an upstream gradient g (P,):
  *   gz1[j]   = sum_i g_i cos(k_j.x_i)

@@ -25,6 +25,9 @@ Pinning (each function is checked by the tests in ``tests/test_oracle.py``):
   by t equals rotating (z1, z2) by k.t; the tensor-grid GEMM identity; the
   reference's own known-answer values (``tests/test_randmeth.py:33-71``,
   ``tests/test_pgs.py:71-75`` through the host sampler).
+* ``summate_fourier``: equals ``summate`` with amplitudes scaled by the
+  factor; the reference's Fourier known values and periodicity
+  (``tests/test_fouriergen.py:47-81``) through the host mode grid.
 * ``summate_incompr``: mpmath brute force; analytic divergence-free check
   (central differences); ``tests/test_incomprrandmeth.py:34-44`` and
   ``tests/test_srf.py:259-275`` known values.
@@ -43,6 +46,7 @@ from .oracle import (  # noqa: F401
     summate_adjoint_c,
     summate_adjoint_np,
     summate_c,
+    summate_fourier_c,
     summate_incompr_c,
     summate_incompr_mp,
     summate_incompr_np,

@@ -41,6 +41,8 @@ def load_lib():
         lib.oracle_summate_adjoint.argtypes = [
             i64, i64, i64, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, ctypes.c_int]
         lib.oracle_summate_adjoint.restype = None
+        lib.oracle_summate_fourier.argtypes = [i64, i64, i64, _dp, _dp, _dp, _dp, _dp, _dp, ctypes.c_int]
+        lib.oracle_summate_fourier.restype = None
         lib.oracle_max_threads.restype = ctypes.c_int
         _LIB = lib
     return _LIB
@@ -77,6 +79,20 @@ def summate_c(cov, z1, z2, pos, nthreads=0):
     p = pos.shape[1]
     out = np.empty(p, dtype=np.float64)
     load_lib().oracle_summate(d, n, p, _ptr(cov), _ptr(z1), _ptr(z2), _ptr(pos), _ptr(out), int(nthreads))
+    return out
+
+
+def summate_fourier_c(spectrum_factor, modes, z1, z2, pos, nthreads=0):
+    """C/OpenMP restatement of ``summate_fourier`` -> (P,) float64."""
+    cov, z1, z2, pos = _check(modes, z1, z2, pos)
+    f = _c(spectrum_factor)
+    d, n = cov.shape
+    if f.shape != (n,):
+        raise ValueError("oracle: spectrum factor shape mismatch")
+    p = pos.shape[1]
+    out = np.empty(p, dtype=np.float64)
+    load_lib().oracle_summate_fourier(d, n, p, _ptr(f), _ptr(cov), _ptr(z1), _ptr(z2), _ptr(pos), _ptr(out),
+                                      int(nthreads))
     return out
 
 

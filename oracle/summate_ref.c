@@ -10,6 +10,9 @@
  *   summate         generator.py:193-199
  *       out[i] = sum_j z1[j] cos(theta_ij) + z2[j] sin(theta_ij)
  *       theta_ij = sum_d cov[d, j] * pos[d, i]          (d in order)
+ *   summate_fourier generator.py:619-624 (Fourier.__call__, :659-691)
+ *       out[i] = sum_j f[j] (z1[j] cos(theta_ij) + z2[j] sin(theta_ij)),
+ *       f = spectrum factor sqrt(S(|k_j|) prod(dk)) computed by the caller
  *   summate_incompr generator.py:479-494
  *       out[c, i] = sum_j p_c(k_j) (z1[j] cos + z2[j] sin),
  *       p_c(k) = delta_{c0} - k_c k_0 / |k|^2
@@ -57,6 +60,24 @@ void oracle_summate(int64_t d, int64_t N, int64_t P, const double *cov,
             double phase = 0.0;
             for (int64_t k = 0; k < d; ++k) phase += cov[k * N + j] * pos[k * P + i];
             acc += z1[j] * cos(phase) + z2[j] * sin(phase);
+        }
+        out[i] = acc;
+    }
+}
+
+/* Fourier method: the same sum with a per-mode spectrum factor f (N,). */
+void oracle_summate_fourier(int64_t d, int64_t N, int64_t P, const double *f,
+                            const double *cov, const double *z1,
+                            const double *z2, const double *pos, double *out,
+                            int nthreads) {
+    set_threads(nthreads);
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < P; ++i) {
+        double acc = 0.0;
+        for (int64_t j = 0; j < N; ++j) {
+            double phase = 0.0;
+            for (int64_t k = 0; k < d; ++k) phase += cov[k * N + j] * pos[k * P + i];
+            acc += f[j] * (z1[j] * cos(phase) + z2[j] * sin(phase));
         }
         out[i] = acc;
     }

@@ -1,0 +1,140 @@
+"""GPU parity of the Fourier method against the oracle.
+
+Bars as for the randomisation method: FP64 <= 1e-12 relative (rel_max)
+against ``oracle.summate_fourier_c``; FP32 <= 1e-5; split-precision matrix
+cores <= 1e-5 rms (bf16x3 <= 5e-5 max).  The reference's known values and
+periodicity checks (``tests/test_fouriergen.py:47-81``) run through SRF on the
+GPU, on the meshgrid route and on the grid kernels.
+"""
+
+import ctypes
+import sys
+
+import numpy as np
+import pytest
+import torch
+
+import oracle
+from gstools_amd import (SRF, Fourier, Gaussian, _native, summate_fourier, summate_fourier_device,
+                         summate_fourier_grid_device)
+from gstools_amd.geometry import generate_grid
+
+pytestmark = pytest.mark.gpu
+
+SEED = 19900408
+L = [80, 30, 91]
+X = np.linspace(0, L[0], 11)
+Y = np.linspace(0, L[1], 31)
+Z = np.linspace(0, L[2], 13)
+KNOWN = {
+    1: (lambda: Gaussian(dim=1, var=0.5, len_scale=10.0), [12], [80], (X,), 0.6236929351309081),
+    2: (lambda: Gaussian(dim=2, var=2.0, len_scale=30.0), [12, 6], L[:2], (X, Y), -0.1431996611581266),
+    3: (lambda: Gaussian(dim=3, var=2.1, len_scale=21.0), [12, 6, 14], L, (X, Y, Z), -1.0433325279452803),
+}
+
+
+def _say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def _gen(d, mode_no, period, len_scale, seed):
+    return Fourier(Gaussian(dim=d, var=1.3, len_scale=len_scale), period=period, mode_no=mode_no, seed=seed)
+
+
+@pytest.mark.parametrize("dim", [1, 2, 3])
+def test_known_values_and_periodicity(dim):
+    _say(f"fourier known values dim={dim}")
+    model, mode_no, period, axes, want = KNOWN[dim]
+    srf = SRF(model(), generator="Fourier", mode_no=mode_no, period=period, seed=SEED)
+    mesh = srf(generate_grid(axes)).reshape([a.size for a in axes])  # meshgrid route
+    grid = srf(axes, mesh_type="structured")                         # grid kernel
+    assert grid.shape == mesh.shape
+    assert mesh.flat[0] == pytest.approx(want, abs=1e-12)
+    assert np.array_equal(grid, mesh)
+    if dim == 1:
+        assert grid[0] == pytest.approx(grid[-1], abs=1e-7)
+    elif dim == 2:
+        assert grid[0, len(Y) // 2] == pytest.approx(grid[-1, len(Y) // 2], abs=1e-7)
+        assert grid[len(X) // 2, 0] == pytest.approx(grid[len(X) // 2, -1], abs=1e-7)
+    else:
+        assert grid[0, len(Y) // 2, 0] == pytest.approx(grid[-1, len(Y) // 2, 0], abs=1e-7)
+        assert grid[0, 0, 0] == pytest.approx(grid[0, -1, 0], abs=1e-7)
+        assert grid[len(X) // 2, len(Y) // 2, 0] == pytest.approx(grid[len(X) // 2, len(Y) // 2, -1], abs=1e-7)
+
+
+@pytest.mark.parametrize("d,mode_no,p", [
+    (1, [64], 1001),
+    (2, [32, 48], 5003),
+    (3, [16, 12, 20], 3001),   # 3840 modes: 15 LDS chunks
+    (3, [2, 2, 2], 1),
+])
+def test_unstructured_fp64_fp32_match_oracle(d, mode_no, p):
+    _say(f"fourier unstructured d={d} modes={mode_no} P={p}")
+    gen = _gen(d, mode_no, [100.0, 70.0, 50.0][:d], 6.0, 11 + d)
+    rs = np.random.RandomState(d)
+    pos = rs.uniform(-300.0, 300.0, size=(d, p))
+    args = (gen._spectrum_factor, gen._modes, gen._z_1, gen._z_2)
+    ref = oracle.summate_fourier_c(*args, pos)
+    got = summate_fourier(*args, pos)
+    assert got.dtype == np.float64 and got.shape == (p,)
+    assert oracle.rel_errors(got, ref)[0] <= 1e-12
+    assert np.array_equal(gen(pos, add_nugget=False), got)
+    f32 = summate_fourier_device(*args, pos, precision="fp32").cpu().numpy()
+    rel_max, rel_rms = oracle.rel_errors(f32, ref)
+    assert rel_max <= 1e-5 and rel_rms <= 1e-5, (rel_max, rel_rms)
+
+
+@pytest.mark.parametrize("shape,mode_no", [
+    ((37, 40, 23), [8, 6, 10]),
+    ((64, 300), [24, 20]),
+    ((130, 17, 33), [4, 4, 4]),
+])
+@pytest.mark.parametrize("precision", ["fp64", "fp32", "bf16x3", "fp16x3"])
+def test_grid_paths_match_oracle(shape, mode_no, precision):
+    _say(f"fourier grid {precision} shape={shape} modes={mode_no}")
+    d = len(shape)
+    period = [float(s) * 0.7 for s in shape]
+    gen = _gen(d, mode_no, period, 3.0, 5 + d)
+    axes = [np.linspace(0.0, 0.7 * s, s) for s in shape]
+    args = (gen._spectrum_factor, gen._modes, gen._z_1, gen._z_2)
+    ref = oracle.summate_fourier_c(*args, generate_grid(axes))
+    got = summate_fourier_grid_device(*args, axes, precision).cpu().numpy().astype(np.float64)
+    rel_max, rel_rms = oracle.rel_errors(got, ref)
+    _say(f"  rel_max={rel_max:.3e} rel_rms={rel_rms:.3e}")
+    if precision == "fp64":
+        assert rel_max <= 1e-12
+    elif precision in ("fp32", "fp16x3"):
+        assert rel_max <= 1e-5 and rel_rms <= 1e-5
+    else:
+        assert rel_rms <= 1e-5 and rel_max <= 5e-5
+    st = gen.structured(axes, precision=precision, add_nugget=False).cpu().numpy().astype(np.float64)
+    assert st.shape == shape
+    assert np.array_equal(st.ravel(), got)
+
+
+def test_empty_and_errors():
+    _say("fourier empty and error statuses")
+    gen = _gen(2, [4, 4], [10.0, 10.0], 2.0, 1)
+    args = (gen._spectrum_factor, gen._modes, gen._z_1, gen._z_2)
+    assert summate_fourier(*args, np.zeros((2, 0))).shape == (0,)
+    empty = summate_fourier(np.zeros(0), np.zeros((2, 0)), np.zeros(0), np.zeros(0), np.ones((2, 5)))
+    assert np.array_equal(empty, np.zeros(5))
+    with pytest.raises(ValueError):
+        summate_fourier(args[0][:-1], *args[1:], np.zeros((2, 3)))
+    with pytest.raises(ValueError):
+        summate_fourier(args[0].astype(np.float32), *args[1:], np.zeros((2, 3)))
+    lib = _native.lib()
+    dev = torch.device("cuda")
+    k = torch.zeros((2, 16), dtype=torch.float64, device=dev)
+    z = torch.zeros(16, dtype=torch.float64, device=dev)
+    pos = torch.zeros((2, 10), dtype=torch.float64, device=dev)
+    out = torch.empty(10, dtype=torch.float64, device=dev)
+    p = lambda t: ctypes.c_void_p(t.data_ptr())
+    assert lib.gsx_summate_fourier_f64(2, 16, 10, None, p(k), p(z), p(z), p(pos), p(out), None) == -1
+    shape = (ctypes.c_int64 * 3)(4, 4, 1)
+    ax = torch.zeros(8, dtype=torch.float64, device=dev)
+    assert lib.gsx_summate_fourier_grid_f64(2, 16, shape, p(ax), None, p(k), p(z), p(z), p(out), None) == -1
+    assert lib.gsx_summate_fourier_grid_mfma(1, 16, shape, p(ax), p(z), p(k), p(z), p(z), 0, 1.0, p(out),
+                                             p(out), 0, None) == -5
+    torch.cuda.synchronize()
