@@ -16,8 +16,8 @@ baseline (the oracle's C/OpenMP restatement of the reference loop) timed on
 a bounded sample of the same workload.
 
 Extra sections (``--extras``, on by default): structured 512^3 Matern grid
-on the matrix cores, forward + adjoint on the 2D Stable configuration and
-the 256-realization batch; each prints its own JSON object on stderr and
+on the matrix cores, the periodic Fourier method on a 4096^2 grid, forward +
+adjoint on the 2D Stable configuration and the 256-realization batch; each prints its own JSON object on stderr and
 under ``gpurun_out/`` when that folder exists.
 """
 
@@ -134,6 +134,55 @@ def bench_mfma_grid(torch, steps, warmup):
     res["fp64_direct"] = {"value": 64 * 512 * 512 * 4096 / wall64, "measured_on": "64 of 512 x-planes"}
     res["speedup_bf16x3_vs_fp64"] = res["bf16x3"]["value"] / res["fp64_direct"]["value"]
     res["speedup_fp16x3_vs_fp64"] = res["fp16x3"]["value"] / res["fp64_direct"]["value"]
+    return res
+
+
+def bench_fourier(torch, steps, warmup):
+    """Periodic 2D Gaussian (var 1, len 200) on a 4096^2 grid, period 4096, mode_no 256^2 (Fourier method)."""
+    import oracle
+    from gstools_amd import Fourier, Gaussian, summate_fourier_grid_device
+
+    log("bench: fourier grid (4096^2, 256^2 modes)")
+    side, m = 4096, 256
+    gen = Fourier(Gaussian(dim=2, var=1.0, len_scale=200.0), period=float(side), mode_no=m, seed=1681903)
+    dev = torch.device("cuda", 0)
+    fac, modes, z1, z2 = (torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+                          for a in (gen._spectrum_factor, gen._modes, gen._z_1, gen._z_2))
+    ax = np.linspace(0.0, float(side), side, endpoint=False)
+    axes = [torch.from_numpy(ax).to(dev)] * 2
+    n, nm = side * side, m * m
+    work = float(n) * nm
+    out = torch.empty(n, dtype=torch.float32, device=dev)
+    rows = np.array([0, 1, 1000, 2047, 4095])
+    ref = summate_fourier_grid_device(fac, modes, z1, z2, [torch.from_numpy(ax[rows]).to(dev), axes[1]], "fp64")
+    ref = ref.reshape(len(rows), side).cpu().numpy()
+    # the oracle on a few points of those rows
+    cols = np.array([0, 17, 2048, 4095])
+    pts = np.stack(np.meshgrid(ax[rows], ax[cols], indexing="ij")).reshape(2, -1)
+    ora = oracle.summate_fourier_c(gen._spectrum_factor, gen._modes, gen._z_1, gen._z_2, pts)
+    res = {"workload": "periodic 2D Gaussian var 1 len 200, grid 4096^2 spacing 1, period 4096, "
+                       "mode_no [256, 256] (Fourier method, seed 1681903)",
+           "points": n, "modes": nm,
+           "fp64_parity_vs_oracle_rel_max": oracle.rel_errors(ref[:, cols].ravel(), ora)[0]}
+    for prec in ("bf16x3", "fp16x3"):
+        fn = lambda: summate_fourier_grid_device(fac, modes, z1, z2, axes, prec, out=out)
+        wall, ker = time_kernel(fn, steps, warmup, torch)
+        got = out.reshape(side, side)[rows].cpu().numpy().astype(np.float64)
+        rel_max, rel_rms = oracle.rel_errors(got, ref)
+        tflops = 12.0 * work / ker / 1e12
+        res[prec] = {"value": work / wall, "ms_per_step": wall * 1e3, "rel_max_vs_fp64": rel_max,
+                     "rel_rms_vs_fp64": rel_rms, "mfma_tflops": tflops, "roofline_frac": tflops / MFMA_PEAK_TFLOPS}
+    f32 = lambda: summate_fourier_grid_device(fac, modes, z1, z2, axes, "fp32", out=out)
+    wall32, ker32 = time_kernel(f32, max(2, steps // 2), 1, torch)
+    got = out.reshape(side, side)[rows].cpu().numpy().astype(np.float64)
+    res["fp32_direct"] = {"value": work / wall32, "ms_per_step": wall32 * 1e3,
+                          "rel_max_vs_fp64": oracle.rel_errors(got, ref)[0],
+                          "roofline_frac": (2 * 2 + 6) * work / ker32 / 1e12 / FP32_VALU_PEAK_TFLOPS}
+    slab = [axes[0][:128], axes[1]]
+    f64 = lambda: summate_fourier_grid_device(fac, modes, z1, z2, slab, "fp64")
+    wall64, _ = time_kernel(f64, 2, 1, torch)
+    res["fp64_direct"] = {"value": 128 * side * float(nm) / wall64, "measured_on": "128 of 4096 x-rows"}
+    res["speedup_bf16x3_vs_fp64"] = res["bf16x3"]["value"] / res["fp64_direct"]["value"]
     return res
 
 
@@ -344,7 +393,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--extras", type=int, default=1, help="also run the other configurations")
-    ap.add_argument("--only", choices=["fp32", "fp64", "mfma", "adjoint", "batch", "copy"],
+    ap.add_argument("--only", choices=["fp32", "fp64", "mfma", "fourier", "adjoint", "batch", "copy"],
                     help="run one section only (for profiling) and print its JSON")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -366,8 +415,9 @@ def main():
     dev = torch.device("cuda", 0)
     k_ext = max(3, args.steps // 4)
     w_ext = max(1, args.warmup)
-    if args.only in ("mfma", "adjoint", "batch", "copy"):
+    if args.only in ("mfma", "fourier", "adjoint", "batch", "copy"):
         fn = {"mfma": lambda: bench_mfma_grid(torch, k_ext, w_ext),
+              "fourier": lambda: bench_fourier(torch, k_ext, w_ext),
               "adjoint": lambda: bench_adjoint(torch, k_ext, w_ext),
               "batch": lambda: bench_batch(torch, k_ext, w_ext, *batch_inputs),
               "copy": lambda: bench_copy(torch)}[args.only]
@@ -450,7 +500,7 @@ def main():
     }
     if args.extras:
         extras = {}
-        sections = (("mfma_grid", bench_mfma_grid), ("adjoint", bench_adjoint),
+        sections = (("mfma_grid", bench_mfma_grid), ("fourier", bench_fourier), ("adjoint", bench_adjoint),
                     ("batch", lambda t, k, w: bench_batch(t, k, w, *batch_inputs)))
         for name, fn in sections:
             try:
