@@ -250,36 +250,60 @@ def _batch_specs():
 
 
 def _sample_modes(spec):
-    from gstools_amd import Exponential, Gaussian, RandMeth
+    """One realization's modes through ``RandMeth`` (the one-at-a-time sampler)."""
+    from gstools_amd import RandMeth
 
     kind, n, seed, _ = spec
-    if kind == "grid2d":
-        model = Gaussian(dim=2, var=1.0, len_scale=10.0)
-    elif kind == "points3d":
-        model = Exponential(dim=3, var=2.0, len_scale=[50, 20, 5], angles=[0.5, 0.2, 0.1])
-    else:
-        model = Gaussian(dim=3, var=1.0, len_scale=10.0)
-    rm = RandMeth(model, mode_no=n, seed=seed)
+    rm = RandMeth(_model_of(kind), mode_no=n, seed=seed)
     return rm._cov_sample, rm._z_1, rm._z_2
 
 
+def _model_of(kind):
+    from gstools_amd import Exponential, Gaussian
+
+    if kind == "grid2d":
+        return Gaussian(dim=2, var=1.0, len_scale=10.0)
+    if kind == "points3d":
+        return Exponential(dim=3, var=2.0, len_scale=[50, 20, 5], angles=[0.5, 0.2, 0.1])
+    return Gaussian(dim=3, var=1.0, len_scale=10.0)
+
+
 def batch_modes():
-    """Sample all batch modes (host MCMC), in a fork pool before any GPU use."""
-    import multiprocessing as mp
+    """Sample all batch modes on the host before any GPU use.
+
+    Each kind shares one model, so its realizations are sampled together by
+    the lockstep sampler (identical arrays to one ``RandMeth`` per
+    realization).  The one-at-a-time sampler is timed on a few MCMC
+    realizations and scaled by their share of the MCMC modes.
+    """
+    from gstools_amd.sampling import sample_modes_batch
 
     specs = _batch_specs()
-    workers = max(1, min(16, len(os.sched_getaffinity(0))))
     t0 = time.perf_counter()
-    if workers > 1:
-        with mp.get_context("fork").Pool(workers) as pool:
-            modes = pool.map(_sample_modes, specs, chunksize=4)
-    else:
-        modes = [_sample_modes(sp) for sp in specs]
-    log(f"bench: sampled modes of {len(specs)} realizations in {time.perf_counter() - t0:.1f} s ({workers} workers)")
-    return specs, modes
+    modes = [None] * len(specs)
+    for kind in ("grid2d", "points3d", "incompr3d"):
+        idx = [i for i, sp in enumerate(specs) if sp[0] == kind]
+        got = sample_modes_batch(_model_of(kind), [specs[i][1] for i in idx], [specs[i][2] for i in idx])
+        for i, g in zip(idx, got):
+            modes[i] = g
+    t_lock = time.perf_counter() - t0
+    mcmc = [sp for sp in specs if sp[0] != "grid2d"]
+    sub = mcmc[::16]
+    t0 = time.perf_counter()
+    for sp in sub:
+        _sample_modes(sp)
+    t_sub = time.perf_counter() - t0
+    share = sum(sp[1] for sp in sub) / sum(sp[1] for sp in mcmc)
+    info = {"realizations": len(specs), "mcmc_realizations": len(mcmc),
+            "lockstep_s": t_lock, "one_at_a_time_s_est": t_sub / share,
+            "one_at_a_time_measured_on": f"{len(sub)} of {len(mcmc)} MCMC realizations, scaled by mode count",
+            "speedup": (t_sub / share) / t_lock}
+    log(f"bench: sampled modes of {len(specs)} realizations in {t_lock:.2f} s (lockstep); "
+        f"one at a time ~{t_sub / share:.1f} s")
+    return specs, modes, info
 
 
-def bench_batch(torch, steps, warmup, specs, modes):
+def bench_batch(torch, steps, warmup, specs, modes, sampling_info=None):
     """256 realizations: one persistent launch (HIP graph) vs per-realization launches."""
     import oracle
     from gstools_amd import Exponential, summate_device, summate_grid_device, summate_incompr_device
@@ -358,6 +382,7 @@ def bench_batch(torch, steps, warmup, specs, modes):
         "speedup_vs_per_realization": per_wall / graph_wall,
         "batched_equals_single_launches": bool(same),
         "rel_max_vs_oracle_spot": errs,
+        "mode_sampling": sampling_info,
     }
 
 

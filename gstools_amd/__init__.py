@@ -18,12 +18,15 @@ Public surface:
 * ``SRF`` -- a thin caller (positions -> isometrize -> generator -> mean);
 * ``Gaussian``, ``Exponential``, ``Matern``, ``Stable`` -- the spectral
   members of the covariance models the configurations use;
-* ``RandMethSum`` (autograd), ``batch`` (many realizations per launch).
+* ``RandMethSum`` (autograd), ``batch`` (many realizations per launch),
+  ``sample_modes_batch`` (the modes of many realizations, sampled in
+  lockstep, identical to one ``RandMeth`` each).
 """
 
 from .covmodel import CovModel, Exponential, Gaussian, Matern, Stable  # noqa: F401
 from .generator import Fourier, Generator, IncomprRandMeth, RandMeth  # noqa: F401
 from .rng import RNG, MasterRNG  # noqa: F401
+from .sampling import sample_ln_pdf_batch, sample_modes_batch  # noqa: F401
 from .srf import SRF  # noqa: F401
 from .summate import (  # noqa: F401
     summate,

@@ -1,7 +1,8 @@
 """Build recipe for ``libgsx.so`` (the gfx950 kernels + C ABI).
 
-Compiles every ``csrc/*.hip`` with hipcc for gfx950 into ``_build/`` and
-links ``gstools_amd/libgsx.so`` in-tree.  Run ``python -m gstools_amd.build``.
+Compiles every ``csrc/*.hip`` with hipcc for gfx950 and every host-only
+``csrc/*.cpp`` with the host C++ compiler into ``_build/``, and links
+``gstools_amd/libgsx.so`` in-tree.  Run ``python -m gstools_amd.build``.
 """
 
 import concurrent.futures as cf
@@ -27,16 +28,26 @@ FLAGS = [
 ]
 
 
+CXX = os.environ.get("CXX", "g++")
+# host code: no contraction of multiply-adds (the sampler's arithmetic is
+# restated from numpy's, which rounds every product)
+HOST_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-Wall", "-ffp-contract=off", "-pthread"]
+
+
 def sources():
-    return sorted(os.path.join(CSRC, f) for f in os.listdir(CSRC) if f.endswith(".hip"))
+    return sorted(os.path.join(CSRC, f) for f in os.listdir(CSRC) if f.endswith((".hip", ".cpp")))
 
 
 def _compile(src):
-    obj = os.path.join(BUILD, os.path.basename(src)[:-4] + ".o")
-    cmd = [HIPCC, *FLAGS, "-c", src, "-o", obj]
+    stem, ext = os.path.splitext(os.path.basename(src))
+    obj = os.path.join(BUILD, stem + ("_host" if ext == ".cpp" else "") + ".o")
+    if ext == ".cpp":
+        cmd = [CXX, *HOST_FLAGS, "-c", src, "-o", obj]
+    else:
+        cmd = [HIPCC, *FLAGS, "-c", src, "-o", obj]
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
-        raise RuntimeError(f"hipcc failed for {src}:\n{res.stderr}")
+        raise RuntimeError(f"compile failed for {src}:\n{res.stderr}")
     return obj
 
 
@@ -48,7 +59,7 @@ def build(verbose=False):
     with cf.ThreadPoolExecutor(max_workers=max(1, workers)) as ex:
         objs = list(ex.map(_compile, srcs))
     tmp = LIB + ".tmp"
-    cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", *objs, "-o", tmp]
+    cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-pthread", *objs, "-o", tmp]
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
         raise RuntimeError(f"link failed:\n{res.stderr}")

@@ -19,12 +19,16 @@
  *   (no reference counterpart: gradient of        gsx_summate_adjoint_f64 / _f32
  *    summate for calibration; formula is the
  *    derivative of generator.py:193-199)
+ *   emcee stretch-move draws of                   gsx_stretch_draws (host only)
+ *     RNG.sample_ln_pdf for many chains,
+ *     src/gstools/random/rng.py:38-104
  *   (no reference counterpart: many realizations  gsx_batch_*
  *    in one launch; each item is one
  *    RandMeth.__call__, generator.py:243-270)
  *
  * Conventions (all entry points):
- *   - every array argument is a DEVICE pointer owned by the caller; inputs
+ *   - every array argument is a DEVICE pointer owned by the caller (except
+ *     gsx_stretch_draws, which works on host memory only); inputs
  *     are read-only, outputs are written in full;
  *   - cov is (d, N) row-major, pos is (d, P) row-major, incompressible
  *     outputs are (d, P) row-major -- the reference's numpy layouts;
@@ -164,6 +168,46 @@ int gsx_summate_adjoint_f32(int64_t d, int64_t N, int64_t P, const double* cov,
                             const double* pos, const float* g, double* gz1,
                             double* gz2, double* gk, void* workspace,
                             int64_t workspace_bytes, void* stream);
+
+/* Host-only (no device memory, no stream): every random draw of the stretch-
+ * move ensemble sampler that RNG.sample_ln_pdf runs through emcee
+ * (src/gstools/random/rng.py:38-104), for R independent chains.  Chain r's
+ * generator state is states[r * 625 .. r * 625 + 624]: the 624 MT19937 key
+ * words and the position, as numpy's RandomState.get_state() reports them;
+ * it is advanced in place.  For steps[r] <= max_steps steps, each of the
+ * (R, max_steps, nwalkers) outputs receives, per step: labels = the shuffled
+ * arange(nwalkers) % 2; uz, partner, uacc = the stretch variates, partner
+ * indices and acceptance uniforms of the walkers labelled 0 (first
+ * ceil(nwalkers / 2) slots) and then labelled 1.  nthreads > 1 splits the
+ * chains over host threads.  Entries past steps[r] are left untouched. */
+int gsx_stretch_draws(int64_t R, uint32_t* states, const int64_t* steps,
+                      int64_t max_steps, int32_t nwalkers, int8_t* labels,
+                      double* uz, int64_t* partner, double* uacc,
+                      int32_t nthreads);
+
+/* Host-only: the two halves of one stretch-move half-move for every chain r
+ * with steps[r] > step, one-dimensional walkers (emcee's arithmetic, every
+ * operation rounded, no fused multiply-add).  Draws are row block_row of
+ * (R, block, nwalkers) arrays made by gsx_stretch_draws.  Propose writes the
+ * proposals of the walkers labelled `split` of the n_active running chains,
+ * compactly, to q (n_active, ns):  q = c_p - (c_p - x) zz,
+ * zz = ((a - 1) u + 1)^2 / a.  The caller evaluates the log-probability of q
+ * (new_lp, same layout); accept replaces walker and log-probability where
+ * (0 + new_lp) - log_prob > log_uacc (log of the acceptance uniforms, taken
+ * by the caller) and, when `chain` is not NULL, copies each running chain's
+ * walkers to chain[r][step] of an (R, chain_steps, nwalkers) array.
+ * nwalkers <= 4096. */
+int gsx_stretch_propose(int64_t R, int32_t nwalkers, const int64_t* steps,
+                        int64_t step, int64_t block_row, int64_t block,
+                        int32_t split, double a, const double* coords,
+                        const int8_t* labels, const double* uz,
+                        const int64_t* partner, double* q, int64_t* n_active);
+int gsx_stretch_accept(int64_t R, int32_t nwalkers, const int64_t* steps,
+                       int64_t step, int64_t block_row, int64_t block,
+                       int32_t split, const int8_t* labels,
+                       const double* log_uacc, const double* q,
+                       const double* new_lp, double* coords, double* log_prob,
+                       double* chain, int64_t chain_steps);
 
 /* Batched FP32 summation of many realizations in one persistent launch.
  * Each item is one RandMeth / IncomprRandMeth call (generator.py:243-270,
