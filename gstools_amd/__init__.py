@@ -18,12 +18,17 @@ Public surface:
 * ``SRF`` -- a thin caller (positions -> isometrize -> generator -> mean);
 * ``Gaussian``, ``Exponential``, ``Matern``, ``Stable`` -- the spectral
   members of the covariance models the configurations use;
+* ``krige`` (``calc_field_krige``, ``calc_field_krige_and_variance`` and the
+  ``Krige`` / ``Simple`` / ``Ordinary`` / ``Universal`` / ``ExtDrift`` /
+  ``Detrended`` front end) and ``CondSRF`` -- kriging sums on the GPU;
 * ``RandMethSum`` (autograd), ``batch`` (many realizations per launch),
   ``sample_modes_batch`` (the modes of many realizations, sampled in
   lockstep, identical to one ``RandMeth`` each).
 """
 
-from .covmodel import CovModel, Exponential, Gaussian, Matern, Stable  # noqa: F401
+from . import krige  # noqa: F401
+from .condsrf import CondSRF  # noqa: F401
+from .covmodel import CovModel, Exponential, Gaussian, Matern, Spherical, Stable  # noqa: F401
 from .generator import Fourier, Generator, IncomprRandMeth, RandMeth  # noqa: F401
 from .rng import RNG, MasterRNG  # noqa: F401
 from .sampling import sample_ln_pdf_batch, sample_modes_batch  # noqa: F401

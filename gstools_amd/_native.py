@@ -73,6 +73,8 @@ EXPORTS = {
     "gsx_summate_adjoint_workspace": (_i64, [_i64, _i64, _i64]),
     "gsx_summate_adjoint_f64": (_i32, _ADJ),
     "gsx_summate_adjoint_f32": (_i32, _ADJ),
+    "gsx_krige_workspace": (_i64, [_i64, _i64, _i32]),
+    "gsx_krige_f64": (_i32, [_i64, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _i64, _vp]),
     "gsx_stretch_draws": (_i32, [_i64, _vp, _vp, _i64, _i32, _vp, _vp, _vp, _vp, _i32]),
     "gsx_stretch_propose": (_i32, [_i64, _i32, _vp, _i64, _i64, _i64, _i32, _f64, _vp, _vp, _vp, _vp, _vp, _vp]),
     "gsx_stretch_accept": (_i32, [_i64, _i32, _vp, _i64, _i64, _i64, _i32, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i64]),

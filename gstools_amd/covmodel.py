@@ -24,7 +24,7 @@ import copy
 import numpy as np
 import scipy.special as sps
 
-from .geometry import matrix_isometrize, set_angles, set_anis
+from .geometry import matrix_anisometrize, matrix_isometrize, set_angles, set_anis
 from .hankel import SymmetricFourierTransform
 
 __all__ = ["CovModel", "Gaussian", "Exponential", "Stable", "Matern", "set_len_anis", "rad_fac"]
@@ -178,6 +178,23 @@ class CovModel:
     def correlation(self, r):
         return self.cor(np.abs(np.asarray(r, dtype=np.double)) / self.len_rescaled)
 
+    def covariance(self, r):
+        """``var * correlation(r)`` (``covmodel/tools.py:65-67``)."""
+        return self.var * self.correlation(r)
+
+    def variogram(self, r):
+        """``var - covariance(r) + nugget`` (``covmodel/tools.py:61-63``)."""
+        return self.var - self.covariance(r) + self.nugget
+
+    def cov_nugget(self, r):
+        """Covariance with the nugget at r = 0: the sill there (``covmodel/base.py:313-320``)."""
+        r = np.asarray(np.abs(r), dtype=np.double)
+        r_gz = np.logical_not(np.isclose(r, 0))
+        res = np.empty_like(r, dtype=np.double)
+        res[r_gz] = self.covariance(r[r_gz])
+        res[np.logical_not(r_gz)] = self.sill
+        return res
+
     # spectrum -----------------------------------------------------------
     @property
     def needs_fourier_transform(self):
@@ -234,6 +251,10 @@ class CovModel:
     @property
     def field_dim(self):
         return self.dim
+
+    def anisometrize(self, pos):
+        pos = np.asarray(pos, dtype=np.double).reshape((self.dim, -1))
+        return np.dot(matrix_anisometrize(self.dim, self.angles, self.anis), pos)
 
     def isometrize(self, pos):
         pos = np.asarray(pos, dtype=np.double).reshape((self.field_dim, -1))
@@ -308,6 +329,14 @@ class Gaussian(CovModel):
 
     def _has_ppf(self):
         return self.dim in [1, 2]
+
+
+class Spherical(CovModel):
+    """Spherical model (``covmodel/models.py:745-781``); spectral density by the Hankel transform."""
+
+    def cor(self, h):
+        h = np.minimum(np.abs(h, dtype=np.double), 1.0)
+        return 1.0 - 1.5 * h + 0.5 * h**3
 
 
 class Exponential(CovModel):

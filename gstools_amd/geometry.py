@@ -74,6 +74,15 @@ def matrix_isometrize(dim, angles, anis):
     return np.matmul(matrix_isotropify(dim, anis), matrix_derotate(dim, angles))
 
 
+def matrix_anisotropify(dim, anis):
+    return np.diag(np.concatenate(([1.0], set_anis(dim, anis))))
+
+
+def matrix_anisometrize(dim, angles, anis):
+    """``rotate(angles) . diag(1, anis)``, the inverse of :func:`matrix_isometrize` (``geometric.py:297-316``)."""
+    return np.matmul(matrix_rotate(dim, angles), matrix_anisotropify(dim, anis))
+
+
 def generate_grid(pos):
     """Meshgrid ('ij') of a structured position tuple -> (d, P) (``geometric.py:340-356``)."""
     return np.asarray(np.meshgrid(*pos, indexing="ij"), dtype=np.double).reshape((len(pos), -1))

@@ -19,6 +19,10 @@
  *   (no reference counterpart: gradient of        gsx_summate_adjoint_f64 / _f32
  *    summate for calibration; formula is the
  *    derivative of generator.py:193-199)
+ *   calc_field_krige(_and_variance)(krig_mat,     gsx_krige_f64, gsx_krige_workspace
+ *     krig_vecs, cond)
+ *     src/gstools/krige/base.py:16-19, :42-61,
+ *     call :307-317
  *   emcee stretch-move draws of                   gsx_stretch_draws (host only)
  *     RNG.sample_ln_pdf for many chains,
  *     src/gstools/random/rng.py:38-104
@@ -168,6 +172,19 @@ int gsx_summate_adjoint_f32(int64_t d, int64_t N, int64_t P, const double* cov,
                             const double* pos, const float* g, double* gz1,
                             double* gz2, double* gk, void* workspace,
                             int64_t workspace_bytes, void* stream);
+
+/* Kriging sums, FP64: for every target point k (column of krig_vecs),
+ *   w = krig_mat . v_k,  field[k] = cond . w,  error[k] = v_k . w.
+ * krig_mat (K, K) and krig_vecs (K, P) row-major, cond (K,).  error == NULL
+ * computes the field only (calc_field_krige), otherwise both
+ * (calc_field_krige_and_variance; the caller forms max(sill - error, 0)).
+ * Replaces the external kernels called at src/gstools/krige/base.py:42-61.
+ * Workspace bytes from gsx_krige_workspace(K, P, error != NULL). */
+int64_t gsx_krige_workspace(int64_t K, int64_t P, int variance);
+int gsx_krige_f64(int64_t K, int64_t P, const double* krig_mat,
+                  const double* krig_vecs, const double* cond, double* field,
+                  double* error, void* workspace, int64_t workspace_bytes,
+                  void* stream);
 
 /* Host-only (no device memory, no stream): every random draw of the stretch-
  * move ensemble sampler that RNG.sample_ln_pdf runs through emcee

@@ -33,12 +33,18 @@ Pinning (each function is checked by the tests in ``tests/test_oracle.py``):
   ``tests/test_srf.py:259-275`` known values.
 * ``summate_adjoint``: central finite differences of ``summate`` in float64
   and the torch float64 autograd of the numpy formula.
+* ``krige`` (``krige_ref.c``, ``calc_field_krige(_and_variance)`` of
+  ``src/gstools/krige/base.py:42-61``): matrix identities computed by numpy
+  (``cond @ M @ V``, column sums of ``V * (M @ V)``) and simple kriging's
+  interpolation property (at a conditioning point the field is the
+  conditioning value and the variance term is the covariance).
 
 Only ``tests/``, ``__graft_entry__.smoke()`` and ``bench.py``'s CPU-baseline
 leg may import this package.
 """
 
 from .oracle import (  # noqa: F401
+    krige_c,
     lib_path,
     load_lib,
     max_threads,

@@ -1,8 +1,10 @@
 """ORACLE -- test infrastructure only (see the package docstring).
 
-Restates ``summate`` / ``summate_incompr`` of the reference's backend
-(formulas at ``src/gstools/field/generator.py:193-199`` and ``:479-494``,
-call sites ``:266-268`` and ``:554-560``).
+Restates ``summate`` / ``summate_incompr`` / ``summate_fourier`` of the
+reference's backend (formulas at ``src/gstools/field/generator.py:193-199``,
+``:479-494`` and ``:619-624``, call sites ``:266-268``, ``:554-560`` and
+``:681-688``) and its kriging sums (``src/gstools/krige/base.py:42-61``,
+called at ``:307-317``).
 """
 
 import ctypes
@@ -43,6 +45,8 @@ def load_lib():
         lib.oracle_summate_adjoint.restype = None
         lib.oracle_summate_fourier.argtypes = [i64, i64, i64, _dp, _dp, _dp, _dp, _dp, _dp, ctypes.c_int]
         lib.oracle_summate_fourier.restype = None
+        lib.oracle_krige.argtypes = [i64, i64, _dp, _dp, _dp, _dp, _dp]
+        lib.oracle_krige.restype = None
         lib.oracle_max_threads.restype = ctypes.c_int
         _LIB = lib
     return _LIB
@@ -94,6 +98,19 @@ def summate_fourier_c(spectrum_factor, modes, z1, z2, pos, nthreads=0):
     load_lib().oracle_summate_fourier(d, n, p, _ptr(f), _ptr(cov), _ptr(z1), _ptr(z2), _ptr(pos), _ptr(out),
                                       int(nthreads))
     return out
+
+
+def krige_c(krig_mat, krig_vecs, cond, variance=True):
+    """C/OpenMP restatement of ``calc_field_krige(_and_variance)`` -> field (P,) [, error (P,)]."""
+    m, v, c = _c(krig_mat), _c(krig_vecs), _c(cond)
+    if v.ndim != 2 or m.shape != (v.shape[0], v.shape[0]) or c.shape != (v.shape[0],):
+        raise ValueError("oracle: kriging shape mismatch")
+    k, p = v.shape
+    field = np.empty(p)
+    error = np.empty(p) if variance else None
+    load_lib().oracle_krige(k, p, _ptr(m), _ptr(v), _ptr(c), _ptr(field),
+                            _ptr(error) if variance else None)
+    return (field, error) if variance else field
 
 
 def summate_incompr_c(cov, z1, z2, pos, nthreads=0):
