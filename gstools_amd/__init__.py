@@ -21,12 +21,15 @@ Public surface:
 * ``krige`` (``calc_field_krige``, ``calc_field_krige_and_variance`` and the
   ``Krige`` / ``Simple`` / ``Ordinary`` / ``Universal`` / ``ExtDrift`` /
   ``Detrended`` front end) and ``CondSRF`` -- kriging sums on the GPU;
+* ``variogram`` (the ``unstructured`` / ``directional`` / ``structured`` /
+  ``ma_structured`` kernels) and ``vario_estimate``, ``vario_estimate_axis``,
+  ``standard_bins`` -- pair binning on the GPU;
 * ``RandMethSum`` (autograd), ``batch`` (many realizations per launch),
   ``sample_modes_batch`` (the modes of many realizations, sampled in
   lockstep, identical to one ``RandMeth`` each).
 """
 
-from . import krige  # noqa: F401
+from . import krige, variogram  # noqa: F401
 from .condsrf import CondSRF  # noqa: F401
 from .covmodel import CovModel, Exponential, Gaussian, Matern, Spherical, Stable  # noqa: F401
 from .generator import Fourier, Generator, IncomprRandMeth, RandMeth  # noqa: F401
@@ -44,5 +47,7 @@ from .summate import (  # noqa: F401
     summate_incompr,
     summate_incompr_device,
 )
+
+from .variogram import standard_bins, vario_estimate, vario_estimate_axis  # noqa: F401
 
 __version__ = "0.1.0"

@@ -75,6 +75,11 @@ EXPORTS = {
     "gsx_summate_adjoint_f32": (_i32, _ADJ),
     "gsx_krige_workspace": (_i64, [_i64, _i64, _i32]),
     "gsx_krige_f64": (_i32, [_i64, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _i64, _vp]),
+    "gsx_vario_workspace": (_i64, [_i64, _i64, _i64, _i64, _i64]),
+    "gsx_vario_unstructured_f64": (_i32, [_i64, _i64, _i64, _i64, _vp, _vp, _vp, _i32, _vp, _vp, _vp, _i64, _vp]),
+    "gsx_vario_directional_f64": (_i32, [_i64, _i64, _i64, _i64, _i64, _vp, _vp, _vp, _vp, _f64, _f64, _i32, _i32,
+                                         _vp, _vp, _vp, _i64, _vp]),
+    "gsx_vario_structured_f64": (_i32, [_i64, _i64, _vp, _vp, _i32, _vp, _vp, _vp, _i64, _vp]),
     "gsx_stretch_draws": (_i32, [_i64, _vp, _vp, _i64, _i32, _vp, _vp, _vp, _vp, _i32]),
     "gsx_stretch_propose": (_i32, [_i64, _i32, _vp, _i64, _i64, _i64, _i32, _f64, _vp, _vp, _vp, _vp, _vp, _vp]),
     "gsx_stretch_accept": (_i32, [_i64, _i32, _vp, _i64, _i64, _i64, _i32, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i64]),

@@ -1,0 +1,455 @@
+// Empirical variograms on gfx950: pair binning (unstructured / directional)
+// and axis lags (structured / masked structured).
+//
+// Reference contract: the external kernels called at
+// src/gstools/variogram/variogram.py:44-113 (gstools-cython >= 1):
+//   pairs j < k with edges[b] <= |x_j - x_k| < edges[b+1]; for directions a
+//   pair also needs acos(|s| / dist) < angles_tol (s = (x_k - x_j) . dir,
+//   distance-0 pairs pass) and, for bandwidth > 0, a band distance
+//   |(x_k - x_j) - s dir| < bandwidth; with separate_dirs the first matching
+//   direction takes the pair; every field m with neither value NaN adds
+//   est(f[m,k] - f[m,j]) (x^2 or sqrt|x|) and one count.  The comparisons
+//   are written exactly as there, so a NaN distance falls in every bin.
+//
+// Pair kernel: a persistent grid walks the upper triangle of 256 x 256 point
+// tiles (tile (tj, tk), tk >= tj, round robin over blocks).  Thread t owns
+// point j = 256 tj + t (coordinates and field values in registers); the k
+// tile's coordinates and field values are staged in LDS and read as
+// broadcasts.  Distances are FP64 with contraction off: the bits of the
+// reference's distance, hence the same bin.  Bins: binary search when the
+// edges are non-decreasing, a full scan otherwise.  Every thread accumulates
+// into its own LDS histogram column ([slot][thread], conflict-free, no
+// atomics); at the end each block sums its columns into per-block partials
+// and a second kernel sums the blocks in a fixed order: counts exact, sums
+// bitwise reproducible.  Slots are (direction, bin) pairs; a launch covers a
+// range of slots so that the histograms fit the LDS.
+//
+// Structured kernel: lag k along the first axis of an (n0, n1) array; block =
+// (lag, 256-column tile), each thread sums over i for its column, blocks
+// write per-tile partials reduced in a fixed order.
+#include "gsx_internal.h"
+
+namespace gsx {
+
+constexpr int kVT = 256;        // points per tile (threads per block)
+constexpr int kVFields = 8;     // fields per launch
+constexpr int kVMaxDim = 3;
+constexpr int kVBlocks = 2048;  // persistent blocks
+
+struct VarioArgs {
+    int64_t F, P, B, D;
+    int cressie;
+    int separate;
+    int directional;
+    double angles_tol, bandwidth;
+    int64_t h0, h1;  // slot range of this launch: slot = d * B + b
+    int64_t nt;      // tiles per side
+};
+
+__device__ __forceinline__ double vario_est(int cressie, double x) { return cressie ? sqrt(fabs(x)) : x * x; }
+
+// Largest b with e[b] <= x for non-decreasing e[0..B]; -1 if x < e[0].
+__device__ __forceinline__ int64_t upper_edge(const double* e, int64_t B, double x) {
+    int64_t lo = 0, hi = B + 1;  // answer in [lo - 1, hi - 1]
+    while (lo < hi) {
+        const int64_t mid = (lo + hi) >> 1;
+        if (e[mid] <= x)
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    return lo - 1;
+}
+
+template <int DIM>
+__global__ __launch_bounds__(kVT) void k_vario_pairs(VarioArgs a, const double* __restrict__ f,
+                                                     const double* __restrict__ edges,
+                                                     const double* __restrict__ pos,
+                                                     const double* __restrict__ dirs,
+                                                     double* __restrict__ part_sum,
+                                                     unsigned long long* __restrict__ part_cnt) {
+#pragma clang fp contract(off)
+    extern __shared__ __align__(16) unsigned char vlds[];
+    const int64_t NH = a.h1 - a.h0;
+    const int t = threadIdx.x;
+    double* s_edges = reinterpret_cast<double*>(vlds);
+    double* s_dirs = s_edges + (a.B + 1);
+    double* s_pos = s_dirs + a.D * DIM;                     // [DIM][kVT]
+    double* s_f = s_pos + DIM * kVT;                        // [F][kVT]
+    double* h_sum = s_f + a.F * kVT;                        // [NH][kVT]
+    unsigned* h_cnt = reinterpret_cast<unsigned*>(h_sum + NH * kVT);  // [NH][kVT]
+
+    for (int64_t i = t; i <= a.B; i += kVT) s_edges[i] = edges[i];
+    if (a.directional)
+        for (int64_t i = t; i < a.D * DIM; i += kVT) s_dirs[i] = dirs[i];
+    for (int64_t h = 0; h < NH; ++h) {
+        h_sum[h * kVT + t] = 0.0;
+        h_cnt[h * kVT + t] = 0u;
+    }
+    __syncthreads();
+    bool mono = true;
+    for (int64_t i = 0; i < a.B; ++i) mono = mono && (s_edges[i] <= s_edges[i + 1]);
+    // directions to test: all of those whose slots this launch records, and
+    // with separate_dirs also the earlier ones (they may take the pair)
+    const int64_t d_last = a.directional ? (a.h1 - 1) / a.B : 0;
+    const int64_t d_first = a.directional ? (a.separate ? 0 : a.h0 / a.B) : 0;
+
+    const int64_t ntp = a.nt * a.nt;
+    for (int64_t tt = blockIdx.x; tt < ntp; tt += gridDim.x) {
+        const int64_t tj = tt / a.nt, tk = tt - tj * a.nt;
+        if (tk < tj) continue;
+        const int64_t k0 = tk * kVT;
+        const int64_t kn = (a.P - k0 < kVT) ? a.P - k0 : kVT;
+        __syncthreads();  // previous tile's readers are done
+#pragma unroll
+        for (int c = 0; c < DIM; ++c) s_pos[c * kVT + t] = (t < kn) ? pos[c * a.P + k0 + t] : 0.0;
+        for (int64_t m = 0; m < a.F; ++m) s_f[m * kVT + t] = (t < kn) ? f[m * a.P + k0 + t] : 0.0;
+        __syncthreads();
+        const int64_t j = tj * kVT + t;
+        if (j >= a.P) continue;
+        double xj[DIM];
+#pragma unroll
+        for (int c = 0; c < DIM; ++c) xj[c] = pos[c * a.P + j];
+        double fj[kVFields];
+#pragma unroll
+        for (int m = 0; m < kVFields; ++m) fj[m] = (m < a.F) ? f[m * a.P + j] : 0.0;
+        const int kstart = (tj == tk) ? t + 1 : 0;
+        for (int kk = kstart; kk < kn; ++kk) {
+            double d2 = 0.0;
+            double dv[DIM];
+#pragma unroll
+            for (int c = 0; c < DIM; ++c) {
+                dv[c] = s_pos[c * kVT + kk] - xj[c];  // x_k - x_j
+                d2 += dv[c] * dv[c];
+            }
+            const double dist = sqrt(d2);
+            int64_t b_lo, b_hi;  // candidate bins [b_lo, b_hi)
+            if (mono && dist == dist) {
+                const int64_t b = upper_edge(s_edges, a.B, dist);
+                if (b < 0 || b >= a.B) continue;
+                b_lo = b;
+                b_hi = b + 1;
+            } else {
+                b_lo = 0;
+                b_hi = a.B;
+            }
+            for (int64_t b = b_lo; b < b_hi; ++b) {
+                if (dist < s_edges[b] || dist >= s_edges[b + 1]) continue;
+                for (int64_t d = d_first; d <= d_last; ++d) {
+                    if (a.directional) {
+                        const double* u = s_dirs + d * DIM;
+                        double sp = 0.0;
+#pragma unroll
+                        for (int c = 0; c < DIM; ++c) sp += dv[c] * u[c];
+                        bool in_band = true, in_angle = true;
+                        if (a.bandwidth > 0.0) {
+                            double bd = 0.0;
+#pragma unroll
+                            for (int c = 0; c < DIM; ++c) {
+                                const double r = dv[c] - sp * u[c];
+                                bd += r * r;
+                            }
+                            in_band = sqrt(bd) < a.bandwidth;
+                        }
+                        if (dist > 0.0) {
+                            const double q = fabs(sp) / dist;
+                            if (q < 1.0) in_angle = acos(q) < a.angles_tol;
+                        }
+                        if (!(in_band && in_angle)) continue;
+                    }
+                    const int64_t h = d * a.B + b - a.h0;
+                    if (h >= 0 && h < NH) {
+                        double s = 0.0;
+                        unsigned c = 0;
+#pragma unroll
+                        for (int m = 0; m < kVFields; ++m) {
+                            if (m >= a.F) break;
+                            const double fk = s_f[m * kVT + kk];
+                            if (fk != fk || fj[m] != fj[m]) continue;
+                            ++c;
+                            s += vario_est(a.cressie, fk - fj[m]);
+                        }
+                        h_sum[h * kVT + t] += s;
+                        h_cnt[h * kVT + t] += c;
+                    }
+                    if (a.separate) break;
+                }
+            }
+        }
+    }
+    __syncthreads();
+    // block partials: slot h summed over the threads in order
+    for (int64_t h = t; h < NH; h += kVT) {
+        double s = 0.0;
+        unsigned long long c = 0;
+        for (int i = 0; i < kVT; ++i) {
+            s += h_sum[h * kVT + i];
+            c += h_cnt[h * kVT + i];
+        }
+        part_sum[(int64_t)blockIdx.x * NH + h] = s;
+        part_cnt[(int64_t)blockIdx.x * NH + h] = c;
+    }
+}
+
+// Sum the block partials of slot h in block order; accumulate into the totals.
+__global__ void k_vario_collect(int64_t blocks, int64_t NH, int64_t h0, const double* __restrict__ part_sum,
+                                const unsigned long long* __restrict__ part_cnt, double* __restrict__ tot_sum,
+                                long long* __restrict__ tot_cnt, int accumulate) {
+    const int64_t h = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (h >= NH) return;
+    double s = 0.0;
+    unsigned long long c = 0;
+    for (int64_t b = 0; b < blocks; ++b) {
+        s += part_sum[b * NH + h];
+        c += part_cnt[b * NH + h];
+    }
+    if (accumulate) {
+        tot_sum[h0 + h] += s;
+        tot_cnt[h0 + h] += (long long)c;
+    } else {
+        tot_sum[h0 + h] = s;
+        tot_cnt[h0 + h] = (long long)c;
+    }
+}
+
+// gamma from sums and counts (Matheron or Cressie), N = max(count, 1)
+__global__ void k_vario_normalize(int64_t n, int cressie, double* __restrict__ gamma,
+                                  const long long* __restrict__ counts) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double cnt = (double)(counts[i] > 0 ? counts[i] : 1);
+    const double s = gamma[i];
+    if (cressie) {
+        const double q = s / cnt;
+        gamma[i] = 0.5 * (q * q * (q * q)) / (0.457 + 0.494 / cnt + 0.045 / (cnt * cnt));
+    } else {
+        gamma[i] = s / (2.0 * cnt);
+    }
+}
+
+// structured lags: block (lag k = blockIdx.y, column tile blockIdx.x)
+__global__ __launch_bounds__(256) void k_vario_lags(int64_t n0, int64_t n1, const double* __restrict__ f,
+                                                    const unsigned char* __restrict__ mask, int cressie,
+                                                    double* __restrict__ part_sum,
+                                                    unsigned long long* __restrict__ part_cnt) {
+    __shared__ double rs[256];
+    __shared__ unsigned long long rc[256];
+    const int64_t k = blockIdx.y;
+    const int64_t j = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    double s = 0.0;
+    unsigned long long c = 0;
+    if (k > 0 && j < n1) {
+        for (int64_t i = 0; i + k < n0; ++i) {
+            if (mask && (mask[(i + k) * n1 + j] || mask[i * n1 + j])) continue;
+            ++c;
+            s += vario_est(cressie, f[(i + k) * n1 + j] - f[i * n1 + j]);
+        }
+    }
+    rs[threadIdx.x] = s;
+    rc[threadIdx.x] = c;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+            rs[threadIdx.x] += rs[threadIdx.x + w];
+            rc[threadIdx.x] += rc[threadIdx.x + w];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        part_sum[k * gridDim.x + blockIdx.x] = rs[0];
+        part_cnt[k * gridDim.x + blockIdx.x] = rc[0];
+    }
+}
+
+__global__ void k_vario_lag_collect(int64_t n0, int64_t tiles, const double* __restrict__ part_sum,
+                                    const unsigned long long* __restrict__ part_cnt, double* __restrict__ gamma,
+                                    long long* __restrict__ counts) {
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n0) return;
+    double s = 0.0;
+    unsigned long long c = 0;
+    for (int64_t b = 0; b < tiles; ++b) {
+        s += part_sum[k * tiles + b];
+        c += part_cnt[k * tiles + b];
+    }
+    gamma[k] = s;
+    counts[k] = (long long)c;
+}
+
+inline size_t vario_lds_bytes(int64_t B, int64_t D, int dim, int64_t F, int64_t NH) {
+    return sizeof(double) * ((size_t)(B + 1) + (size_t)(D * dim) + (size_t)dim * kVT + (size_t)F * kVT +
+                             (size_t)NH * kVT) +
+           sizeof(unsigned) * (size_t)NH * kVT;
+}
+
+int vario_lds_limit(size_t* out) {
+    static size_t cached[64] = {0};
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return GSX_E_NO_DEVICE;
+    if (!cached[dev]) {
+        int v = 0;
+        if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess || v <= 0)
+            return GSX_E_NO_DEVICE;
+        cached[dev] = (size_t)v;
+    }
+    *out = cached[dev];
+    return GSX_OK;
+}
+
+// histogram slots one launch can hold
+inline int64_t vario_slots_per_launch(size_t limit, int64_t B, int64_t D, int dim) {
+    const size_t fixed = vario_lds_bytes(B, D, dim, kVFields, 0);
+    if (limit <= fixed) return 0;
+    return (int64_t)((limit - fixed) / ((sizeof(double) + sizeof(unsigned)) * kVT));
+}
+
+template <int DIM>
+int launch_pairs(const VarioArgs& a, size_t lds, int64_t blocks, hipStream_t s, const double* f,
+                 const double* edges, const double* pos, const double* dirs, double* part_sum,
+                 unsigned long long* part_cnt) {
+    if (lds > 65536 && hipFuncSetAttribute(reinterpret_cast<const void*>(&k_vario_pairs<DIM>),
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) != hipSuccess)
+        return GSX_E_LAUNCH;
+    hipLaunchKernelGGL((k_vario_pairs<DIM>), dim3((unsigned)blocks), dim3(kVT), lds, s, a, f, edges, pos, dirs,
+                       part_sum, part_cnt);
+    return launch_status();
+}
+
+inline int64_t vario_pair_blocks(int64_t P) {
+    const int64_t nt = ceil_div(P, (int64_t)kVT);
+    const int64_t ntri = nt * (nt + 1) / 2;
+    return ntri < kVBlocks ? (ntri > 0 ? ntri : 1) : kVBlocks;
+}
+
+int vario_pairs(int64_t F, int64_t P, int64_t dim, int64_t B, int64_t D, const double* f, const double* edges,
+                const double* pos, const double* dirs, double angles_tol, double bandwidth, int separate,
+                int estimator, int directional, double* gamma, int64_t* counts, void* ws, int64_t ws_bytes,
+                void* stream) {
+    if (F < 0 || P < 0 || dim < 1 || dim > kVMaxDim || B < 0 || D < 1 || (estimator != 0 && estimator != 1))
+        return GSX_E_INVALID;
+    const int64_t NHall = D * B;
+    if (NHall == 0) return GSX_OK;
+    if (!gamma || !counts || !edges || (directional && !dirs) || (P > 0 && (!pos || (F > 0 && !f))))
+        return GSX_E_INVALID;
+    size_t limit = 0;
+    int st = vario_lds_limit(&limit);
+    if (st != GSX_OK) return st;
+    const int64_t per = vario_slots_per_launch(limit, B, D, (int)dim);
+    if (per < 1) return GSX_E_UNSUPPORTED;
+    const int64_t blocks = vario_pair_blocks(P);
+    const int64_t chunk_slots = per < NHall ? per : NHall;
+    const int64_t need = blocks * chunk_slots * (int64_t)(sizeof(double) + sizeof(unsigned long long));
+    if (!ws || ws_bytes < need) return GSX_E_WORKSPACE;
+    // per-thread counts are 32-bit: bound the pairs one thread can see
+    const int64_t nt = ceil_div(P, (int64_t)kVT);
+    const int64_t tiles_per_block = ceil_div(nt * nt, blocks);
+    if ((double)tiles_per_block * kVT * (double)(F > 0 ? F : 1) > 4.0e9) return GSX_E_UNSUPPORTED;
+    hipStream_t s = as_stream(stream);
+    double* part_sum = static_cast<double*>(ws);
+    unsigned long long* part_cnt = reinterpret_cast<unsigned long long*>(part_sum + blocks * chunk_slots);
+    long long* tot_cnt = reinterpret_cast<long long*>(counts);
+    for (int64_t h0 = 0; h0 < NHall; h0 += chunk_slots) {
+        const int64_t h1 = (h0 + chunk_slots < NHall) ? h0 + chunk_slots : NHall;
+        for (int64_t m0 = 0; m0 < (F > 0 ? F : 1); m0 += kVFields) {
+            const int64_t fc = F - m0 < kVFields ? F - m0 : kVFields;
+            VarioArgs a;
+            a.F = F > 0 ? fc : 0;
+            a.P = P;
+            a.B = B;
+            a.D = directional ? D : 1;
+            a.cressie = estimator;
+            a.separate = separate;
+            a.directional = directional;
+            a.angles_tol = angles_tol;
+            a.bandwidth = bandwidth;
+            a.h0 = h0;
+            a.h1 = h1;
+            a.nt = nt;
+            const size_t lds = vario_lds_bytes(B, a.D, (int)dim, kVFields, h1 - h0);
+            if (P > 1) {
+                const double* fm = F > 0 ? f + m0 * P : nullptr;
+                if (dim == 1)
+                    st = launch_pairs<1>(a, lds, blocks, s, fm, edges, pos, dirs, part_sum, part_cnt);
+                else if (dim == 2)
+                    st = launch_pairs<2>(a, lds, blocks, s, fm, edges, pos, dirs, part_sum, part_cnt);
+                else
+                    st = launch_pairs<3>(a, lds, blocks, s, fm, edges, pos, dirs, part_sum, part_cnt);
+                if (st != GSX_OK) return st;
+                hipLaunchKernelGGL(k_vario_collect, dim3((unsigned)ceil_div(h1 - h0, (int64_t)256)), dim3(256), 0,
+                                   s, blocks, h1 - h0, h0, part_sum, part_cnt, gamma, tot_cnt, m0 > 0 ? 1 : 0);
+            } else {
+                hipLaunchKernelGGL(k_vario_collect, dim3((unsigned)ceil_div(h1 - h0, (int64_t)256)), dim3(256), 0,
+                                   s, (int64_t)0, h1 - h0, h0, part_sum, part_cnt, gamma, tot_cnt, 0);
+            }
+            st = launch_status();
+            if (st != GSX_OK) return st;
+        }
+    }
+    hipLaunchKernelGGL(k_vario_normalize, dim3((unsigned)ceil_div(NHall, (int64_t)256)), dim3(256), 0, s, NHall,
+                       estimator, gamma, tot_cnt);
+    return launch_status();
+}
+
+}  // namespace gsx
+
+extern "C" {
+
+int64_t gsx_vario_workspace(int64_t kind, int64_t P_or_n0, int64_t dim_or_n1, int64_t B, int64_t D) {
+    if (kind == 2) {  // structured: n0 lags x column tiles
+        if (P_or_n0 < 0 || dim_or_n1 < 0) return GSX_E_INVALID;
+        const int64_t tiles = gsx::ceil_div(dim_or_n1, (int64_t)256);
+        return P_or_n0 * (tiles > 0 ? tiles : 1) * 16;
+    }
+    if (P_or_n0 < 0 || dim_or_n1 < 1 || dim_or_n1 > gsx::kVMaxDim || B < 0 || D < 1) return GSX_E_INVALID;
+    size_t limit = 0;
+    int st = gsx::vario_lds_limit(&limit);
+    if (st != GSX_OK) return st;
+    const int64_t per = gsx::vario_slots_per_launch(limit, B, kind == 1 ? D : 1, (int)dim_or_n1);
+    if (per < 1) return GSX_E_UNSUPPORTED;
+    const int64_t slots = (kind == 1 ? D : 1) * B;
+    const int64_t chunk = per < slots ? per : slots;
+    return gsx::vario_pair_blocks(P_or_n0) * (chunk > 0 ? chunk : 1) * 16;
+}
+
+int gsx_vario_unstructured_f64(int64_t F, int64_t P, int64_t dim, int64_t B, const double* f, const double* edges,
+                               const double* pos, int estimator, double* gamma, int64_t* counts, void* workspace,
+                               int64_t workspace_bytes, void* stream) {
+    return gsx::vario_pairs(F, P, dim, B, 1, f, edges, pos, nullptr, 0.0, -1.0, 0, estimator, 0, gamma, counts,
+                            workspace, workspace_bytes, stream);
+}
+
+int gsx_vario_directional_f64(int64_t F, int64_t P, int64_t dim, int64_t B, int64_t D, const double* f,
+                              const double* edges, const double* pos, const double* directions, double angles_tol,
+                              double bandwidth, int separate_dirs, int estimator, double* gamma, int64_t* counts,
+                              void* workspace, int64_t workspace_bytes, void* stream) {
+    return gsx::vario_pairs(F, P, dim, B, D, f, edges, pos, directions, angles_tol, bandwidth, separate_dirs ? 1 : 0,
+                            estimator, 1, gamma, counts, workspace, workspace_bytes, stream);
+}
+
+int gsx_vario_structured_f64(int64_t n0, int64_t n1, const double* f, const uint8_t* mask, int estimator,
+                             double* gamma, int64_t* counts, void* workspace, int64_t workspace_bytes,
+                             void* stream) {
+    if (n0 < 0 || n1 < 0 || (estimator != 0 && estimator != 1)) return GSX_E_INVALID;
+    if (n0 == 0) return GSX_OK;
+    if (!gamma || !counts || (n1 > 0 && !f)) return GSX_E_INVALID;
+    const int64_t tiles = gsx::ceil_div(n1, (int64_t)256) > 0 ? gsx::ceil_div(n1, (int64_t)256) : 1;
+    if (n0 > 65535 || tiles > 0x7fffffffLL) return GSX_E_INVALID;
+    if (!workspace || workspace_bytes < n0 * tiles * 16) return GSX_E_WORKSPACE;
+    hipStream_t s = gsx::as_stream(stream);
+    double* part_sum = static_cast<double*>(workspace);
+    unsigned long long* part_cnt = reinterpret_cast<unsigned long long*>(part_sum + n0 * tiles);
+    hipLaunchKernelGGL(gsx::k_vario_lags, dim3((unsigned)tiles, (unsigned)n0), dim3(256), 0, s, n0, n1, f, mask,
+                       estimator, part_sum, part_cnt);
+    int st = gsx::launch_status();
+    if (st != GSX_OK) return st;
+    long long* cnt = reinterpret_cast<long long*>(counts);
+    hipLaunchKernelGGL(gsx::k_vario_lag_collect, dim3((unsigned)gsx::ceil_div(n0, (int64_t)256)), dim3(256), 0, s, n0,
+                       tiles, part_sum, part_cnt, gamma, cnt);
+    st = gsx::launch_status();
+    if (st != GSX_OK) return st;
+    hipLaunchKernelGGL(gsx::k_vario_normalize, dim3((unsigned)gsx::ceil_div(n0, (int64_t)256)), dim3(256), 0, s, n0,
+                       estimator, gamma, cnt);
+    return gsx::launch_status();
+}
+
+}  // extern "C"

@@ -23,6 +23,10 @@
  *     krig_vecs, cond)
  *     src/gstools/krige/base.py:16-19, :42-61,
  *     call :307-317
+ *   variogram kernels unstructured / directional  gsx_vario_unstructured_f64,
+ *     / structured / ma_structured                gsx_vario_directional_f64,
+ *     src/gstools/variogram/variogram.py:12-15,   gsx_vario_structured_f64,
+ *     :44-113, call :438-466, :560-565            gsx_vario_workspace
  *   emcee stretch-move draws of                   gsx_stretch_draws (host only)
  *     RNG.sample_ln_pdf for many chains,
  *     src/gstools/random/rng.py:38-104
@@ -185,6 +189,43 @@ int gsx_krige_f64(int64_t K, int64_t P, const double* krig_mat,
                   const double* krig_vecs, const double* cond, double* field,
                   double* error, void* workspace, int64_t workspace_bytes,
                   void* stream);
+
+/* Empirical variograms, FP64.  Fields f (F, P) row-major (several fields at
+ * the same points), pos (dim, P) row-major, dim 1..3, bin edges (B + 1).
+ * gamma (B) and counts (B) receive the estimate and the pair counts: pairs
+ * j < k with edges[b] <= |x_j - x_k| < edges[b+1] (Euclidean), every field
+ * whose two values are not NaN adds est(f_k - f_j) and one count; estimator
+ * 0 = Matheron (gamma = sum (f_k - f_j)^2 / (2 N)), 1 = Cressie
+ * (gamma = 0.5 (sum |f_k - f_j|^0.5 / N)^4 / (0.457 + 0.494/N + 0.045/N^2)),
+ * N = max(count, 1).  Directional: D unit directions (D, dim); a pair also
+ * needs acos(|s| / dist) < angles_tol, s = (x_k - x_j) . dir (distance-0
+ * pairs pass) and, if bandwidth > 0, a distance below bandwidth from the
+ * direction line; separate_dirs != 0 gives each pair to the first matching
+ * direction; gamma, counts are (D, B).  Structured: lags 0..n0-1 along the
+ * first axis of an (n0, n1) array; mask (n0, n1) bytes, non-zero = masked,
+ * or NULL; gamma, counts (n0).  Counts are exact; sums are reduced in a
+ * fixed order (bitwise reproducible).  Workspace bytes:
+ * gsx_vario_workspace(kind, P, dim, B, D) for kind 0 (unstructured) or 1
+ * (directional), gsx_vario_workspace(2, n0, n1, 0, 1) for structured.
+ * Replaces the external kernels of src/gstools/variogram/variogram.py:44-113. */
+int64_t gsx_vario_workspace(int64_t kind, int64_t P_or_n0, int64_t dim_or_n1,
+                            int64_t B, int64_t D);
+int gsx_vario_unstructured_f64(int64_t F, int64_t P, int64_t dim, int64_t B,
+                               const double* f, const double* edges,
+                               const double* pos, int estimator, double* gamma,
+                               int64_t* counts, void* workspace,
+                               int64_t workspace_bytes, void* stream);
+int gsx_vario_directional_f64(int64_t F, int64_t P, int64_t dim, int64_t B,
+                              int64_t D, const double* f, const double* edges,
+                              const double* pos, const double* directions,
+                              double angles_tol, double bandwidth,
+                              int separate_dirs, int estimator, double* gamma,
+                              int64_t* counts, void* workspace,
+                              int64_t workspace_bytes, void* stream);
+int gsx_vario_structured_f64(int64_t n0, int64_t n1, const double* f,
+                             const uint8_t* mask, int estimator, double* gamma,
+                             int64_t* counts, void* workspace,
+                             int64_t workspace_bytes, void* stream);
 
 /* Host-only (no device memory, no stream): every random draw of the stretch-
  * move ensemble sampler that RNG.sample_ln_pdf runs through emcee

@@ -38,6 +38,11 @@ Pinning (each function is checked by the tests in ``tests/test_oracle.py``):
   (``cond @ M @ V``, column sums of ``V * (M @ V)``) and simple kriging's
   interpolation property (at a conditioning point the field is the
   conditioning value and the variance term is the covariance).
+* variogram kernels (``variogram_ref.c``, ``variogram.py:44-113`` of the
+  reference): the reference's known values (``tests/test_variogram_*``),
+  brute force over explicit pair lists in numpy, the structured kernel
+  against the unstructured one on grids (axis lags), direction tests
+  against explicit angle computations.
 
 Only ``tests/``, ``__graft_entry__.smoke()`` and ``bench.py``'s CPU-baseline
 leg may import this package.
@@ -58,4 +63,7 @@ from .oracle import (  # noqa: F401
     summate_incompr_np,
     summate_mp,
     summate_np,
+    vario_directional_c,
+    vario_structured_c,
+    vario_unstructured_c,
 )

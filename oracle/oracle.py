@@ -47,6 +47,14 @@ def load_lib():
         lib.oracle_summate_fourier.restype = None
         lib.oracle_krige.argtypes = [i64, i64, _dp, _dp, _dp, _dp, _dp]
         lib.oracle_krige.restype = None
+        _lp = ctypes.POINTER(ctypes.c_int64)
+        lib.oracle_vario_unstructured.argtypes = [i64, i64, i64, i64, _dp, _dp, _dp, ctypes.c_int, _dp, _lp]
+        lib.oracle_vario_unstructured.restype = None
+        lib.oracle_vario_directional.argtypes = [i64, i64, i64, i64, i64, _dp, _dp, _dp, _dp, ctypes.c_double,
+                                                 ctypes.c_double, ctypes.c_int, ctypes.c_int, _dp, _lp]
+        lib.oracle_vario_directional.restype = None
+        lib.oracle_vario_structured.argtypes = [i64, i64, _dp, ctypes.c_void_p, ctypes.c_int, _dp, _lp]
+        lib.oracle_vario_structured.restype = None
         lib.oracle_max_threads.restype = ctypes.c_int
         _LIB = lib
     return _LIB
@@ -111,6 +119,57 @@ def krige_c(krig_mat, krig_vecs, cond, variance=True):
     load_lib().oracle_krige(k, p, _ptr(m), _ptr(v), _ptr(c), _ptr(field),
                             _ptr(error) if variance else None)
     return (field, error) if variance else field
+
+
+def _lptr(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_int64))
+
+
+def vario_unstructured_c(f, bin_edges, pos, estimator_type="m"):
+    """C restatement of the ``unstructured`` variogram kernel -> (gamma (B,), counts (B,))."""
+    f = np.atleast_2d(_c(f))
+    pos = _c(pos)
+    if pos.ndim == 1:
+        pos = pos.reshape(1, -1)
+    e = _c(bin_edges)
+    nf, npnt = f.shape
+    if pos.shape[1] != npnt:
+        raise ValueError("oracle: pos / field mismatch")
+    nb = max(len(e) - 1, 0)
+    gamma, counts = np.empty(nb), np.empty(nb, dtype=np.int64)
+    load_lib().oracle_vario_unstructured(nf, npnt, pos.shape[0], nb, _ptr(f), _ptr(e), _ptr(pos),
+                                         int(estimator_type == "c"), _ptr(gamma), _lptr(counts))
+    return gamma, counts
+
+
+def vario_directional_c(f, bin_edges, pos, direction, angles_tol=np.pi / 8, bandwidth=-1.0, separate_dirs=False,
+                        estimator_type="m"):
+    """C restatement of the ``directional`` kernel -> (gamma (D, B), counts (D, B))."""
+    f = np.atleast_2d(_c(f))
+    pos = _c(pos)
+    e = _c(bin_edges)
+    dirs = np.atleast_2d(_c(direction))
+    nf, npnt = f.shape
+    nb, nd = max(len(e) - 1, 0), dirs.shape[0]
+    gamma, counts = np.empty((nd, nb)), np.empty((nd, nb), dtype=np.int64)
+    load_lib().oracle_vario_directional(nf, npnt, pos.shape[0], nb, nd, _ptr(f), _ptr(e), _ptr(pos), _ptr(dirs),
+                                        float(angles_tol), float(bandwidth), int(bool(separate_dirs)),
+                                        int(estimator_type == "c"), _ptr(gamma), _lptr(counts))
+    return gamma, counts
+
+
+def vario_structured_c(f, mask=None, estimator_type="m"):
+    """C restatement of ``structured`` / ``ma_structured`` on an (n0, n1) array -> (gamma, counts)."""
+    f = _c(f)
+    if f.ndim == 1:
+        f = f.reshape(-1, 1)
+    n0, n1 = f.shape
+    m = None if mask is None else np.ascontiguousarray(np.asarray(mask, dtype=bool).reshape(n0, n1),
+                                                       dtype=np.uint8)
+    gamma, counts = np.empty(n0), np.empty(n0, dtype=np.int64)
+    load_lib().oracle_vario_structured(n0, n1, _ptr(f), None if m is None else m.ctypes.data, int(estimator_type == "c"),
+                                       _ptr(gamma), _lptr(counts))
+    return gamma, counts
 
 
 def summate_incompr_c(cov, z1, z2, pos, nthreads=0):

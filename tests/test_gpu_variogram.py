@@ -1,0 +1,127 @@
+"""GPU variograms: ``gsx_vario_*`` against the oracle, and the reference's
+variogram scenarios with the real kernels.
+
+Bars: counts identical to the oracle's (integers; the distances carry the
+reference's rounding, so every pair lands in the same bin); estimates within
+1e-12 relative (the sums are the same terms added in another order).
+Covered: 1-3 dimensions, stacked fields beyond one launch's field chunk,
+NaN values and positions, non-monotone edges, more (direction, bin) slots
+than one launch holds, directional tolerance / bandwidth / separated
+directions, Cressie, structured and masked lags.
+"""
+
+import sys
+
+import numpy as np
+import pytest
+
+import gstools_amd as gs
+import oracle
+from gstools_amd import variogram as v
+
+import vario_cases as cases
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def _close(g, g_ref):
+    assert np.allclose(g, g_ref, rtol=1e-12, atol=1e-300)
+
+
+@pytest.mark.parametrize("dim,p,nf,est", [(1, 1000, 1, "m"), (2, 2500, 3, "c"), (3, 3001, 11, "m"), (2, 2, 1, "m"),
+                                          (3, 1, 1, "m")])
+def test_unstructured_matches_oracle(dim, p, nf, est):
+    _say(f"vario unstructured dim={dim} P={p} F={nf} est={est}")
+    rs = np.random.RandomState(p + dim)
+    pos = rs.uniform(0, 20, size=(dim, p))
+    f = rs.normal(size=(nf, p))
+    f[0, :5] = np.nan
+    edges = np.linspace(0, 12, 15)
+    g, c = v.unstructured(f, edges, pos, est)
+    g_ref, c_ref = oracle.vario_unstructured_c(f, edges, pos, est)
+    assert np.array_equal(c, c_ref)
+    _close(g, g_ref)
+    g2, c2 = v.unstructured(f, edges, pos, est)
+    assert np.array_equal(g, g2) and np.array_equal(c, c2)  # fixed reduction order
+
+
+def test_edges_nan_positions_and_many_bins():
+    _say("vario odd edges / nan positions / many bins")
+    rs = np.random.RandomState(3)
+    pos = rs.uniform(0, 10, size=(2, 700))
+    pos[0, 3] = np.nan  # its pairs have a NaN distance: counted in every bin
+    f = rs.normal(size=700)
+    for edges in (np.array([0.0, 2.0, 1.0, 5.0, 4.0, 9.0]), np.linspace(0, 14, 400), np.array([1.0, 1.0, 3.0])):
+        g, c = v.unstructured(f, edges, pos)
+        g_ref, c_ref = oracle.vario_unstructured_c(f, edges, pos)
+        assert np.array_equal(c, c_ref)
+        _close(g, g_ref)
+
+
+@pytest.mark.parametrize("separate,bandwidth", [(False, -1.0), (True, -1.0), (False, 1.5), (True, 0.8)])
+def test_directional_matches_oracle(separate, bandwidth):
+    _say(f"vario directional separate={separate} bandwidth={bandwidth}")
+    rs = np.random.RandomState(5)
+    pos = rs.uniform(0, 10, size=(3, 1500))
+    f = rs.normal(size=(2, 1500))
+    dirs = rs.normal(size=(4, 3))
+    dirs /= np.linalg.norm(dirs, axis=1)[:, None]
+    edges = np.linspace(0, 9, 10)
+    g, c = v.directional(f, edges, pos, dirs, np.pi / 6, bandwidth, separate, "m")
+    g_ref, c_ref = oracle.vario_directional_c(f, edges, pos, dirs, np.pi / 6, bandwidth, separate, "m")
+    assert np.array_equal(c, c_ref)
+    _close(g, g_ref)
+
+
+def test_directional_slot_chunks():
+    """(direction, bin) slots beyond one launch's LDS: launches over slot ranges."""
+    _say("vario directional many slots")
+    rs = np.random.RandomState(6)
+    pos = rs.uniform(0, 10, size=(2, 900))
+    f = rs.normal(size=900)
+    dirs = np.stack([np.cos(np.linspace(0, np.pi, 9)), np.sin(np.linspace(0, np.pi, 9))], 1)
+    edges = np.linspace(0, 12, 61)
+    for separate in (False, True):
+        g, c = v.directional(f, edges, pos, dirs, np.pi / 16, -1.0, separate, "c")
+        g_ref, c_ref = oracle.vario_directional_c(f, edges, pos, dirs, np.pi / 16, -1.0, separate, "c")
+        assert np.array_equal(c, c_ref)
+        _close(g, g_ref)
+
+
+@pytest.mark.parametrize("n0,n1,masked,est", [(10, 1, False, "m"), (300, 5000, False, "c"), (257, 1031, True, "m")])
+def test_structured_matches_oracle(n0, n1, masked, est):
+    _say(f"vario structured {n0}x{n1} masked={masked}")
+    rs = np.random.RandomState(n0)
+    f = rs.normal(size=(n0, n1))
+    mask = rs.uniform(size=(n0, n1)) < 0.1 if masked else None
+    g = v.ma_structured(f, mask, est) if masked else v.structured(f, est)
+    g_ref, c_ref = oracle.vario_structured_c(f, mask, est)
+    _close(g, g_ref)
+    g2, c2 = v._lags(f, mask, est)
+    assert np.array_equal(c2.cpu().numpy(), c_ref)
+
+
+@pytest.mark.parametrize("scenario", ["known_values", "uncorrelated", "sampling", "assertions", "direction_assertions"])
+def test_reference_scenarios(scenario):
+    _say(f"vario scenario {scenario}")
+    getattr(cases, scenario)(gs)
+
+
+def test_reference_fields_from_srf():
+    """``tests/test_variogram_unstructured.py:13-17``: the 10^3 Exponential SRF field, seed 123456."""
+    _say("vario srf fields")
+    model = gs.Exponential(dim=3, len_scale=[12, 6, 3])
+    srf = gs.SRF(model, seed=123456)
+    field = srf((range(10), range(10), range(10)), mesh_type="structured")
+
+    def smooth1d(x, seed):
+        return gs.SRF(gs.Exponential(dim=1, var=2, len_scale=10))(x, seed=seed)
+
+    cases.stacked_and_no_data(gs, smooth1d)
+    cases.direction_axis(gs, field)
+    cases.direction_angle(gs, field)

@@ -1,0 +1,185 @@
+"""Variograms on the CPU: the oracle's variogram kernels and the host front end.
+
+``oracle.vario_*_c`` (C restatement of the external ``unstructured`` /
+``directional`` / ``structured`` / ``ma_structured`` kernels,
+``src/gstools/variogram/variogram.py:44-113``) is pinned by the reference's
+literature known values, by brute force over explicit pair lists (numpy
+distances and ``searchsorted`` bins), by the structured kernel agreeing with
+the unstructured one on integer grids, by direction tests against explicit
+angles, and by planted slips.  The host front end runs the reference's
+variogram scenarios (``tests/vario_cases.py``) with the oracle in place of
+the device kernels -- test infrastructure only; the product calls
+``gsx_vario_*``, compared with this oracle in ``test_gpu_variogram.py``.
+"""
+
+import numpy as np
+import pytest
+import torch
+
+import gstools_amd as gs
+import oracle
+from gstools_amd import variogram as vmod
+
+import vario_cases as cases
+
+Z = cases.Z_LIT
+
+
+def test_oracle_known_values():
+    x = np.arange(1, 11, dtype=np.double)
+    g, c = oracle.vario_unstructured_c(Z, np.arange(1, 11, dtype=np.double), x)
+    assert g[0] == pytest.approx(0.4917, abs=5e-5) and g[1] == pytest.approx(0.7625, abs=5e-5)
+    assert list(c[:3]) == [9, 8, 7]
+    g, _ = oracle.vario_structured_c(Z, estimator_type="c")
+    assert g[1] == pytest.approx(1.546 / 2.0, abs=5e-4)
+    m = np.zeros(10, bool)
+    m[0] = True
+    g, c = oracle.vario_structured_c(Z, mask=m)
+    assert g[1] == pytest.approx(0.4906, abs=5e-5) and g[2] == pytest.approx(0.7107, abs=5e-5)
+    assert list(c[:3]) == [0, 8, 7]
+
+
+def _brute(f, edges, pos, cressie=False):
+    p = pos.shape[1]
+    j, k = np.triu_indices(p, 1)
+    d = np.sqrt(((pos[:, j] - pos[:, k]) ** 2).sum(0))
+    b = np.searchsorted(edges, d, side="right") - 1
+    ok = (b >= 0) & (b < len(edges) - 1)
+    sums, counts = np.zeros(len(edges) - 1), np.zeros(len(edges) - 1, dtype=np.int64)
+    for fm in np.atleast_2d(f):
+        diff = fm[k] - fm[j]
+        good = ok & ~np.isnan(diff)
+        val = np.sqrt(np.abs(diff)) if cressie else diff ** 2
+        np.add.at(sums, b[good], val[good])
+        np.add.at(counts, b[good], 1)
+    n = np.maximum(counts, 1)
+    if cressie:
+        return 0.5 * (sums / n) ** 4 / (0.457 + 0.494 / n + 0.045 / n ** 2), counts
+    return sums / (2 * n), counts
+
+
+@pytest.mark.parametrize("dim,cressie", [(1, False), (2, True), (3, False)])
+def test_oracle_brute_force(dim, cressie):
+    rs = np.random.RandomState(dim)
+    pos = rs.uniform(0, 10, size=(dim, 300))
+    f = rs.normal(size=(2, 300))
+    f[1, :7] = np.nan
+    edges = np.linspace(0, 6, 9)
+    g, c = oracle.vario_unstructured_c(f, edges, pos, "c" if cressie else "m")
+    gb, cb = _brute(f, edges, pos, cressie)
+    assert np.array_equal(c, cb)
+    assert np.allclose(g, gb, rtol=1e-12, atol=0)
+
+
+def test_oracle_structured_equals_unstructured_on_grid():
+    rs = np.random.RandomState(4)
+    f = rs.normal(size=(20, 6))
+    g_s, c_s = oracle.vario_structured_c(f)
+    # along axis 0 of a 20 x 6 grid with unit spacing: one column at a time
+    edges = np.arange(20) + 0.5  # bin b holds lag b + 1
+    sums = np.zeros(19)
+    counts = np.zeros(19, dtype=np.int64)
+    for col in range(6):
+        g, c = oracle.vario_unstructured_c(f[:, col], edges, np.arange(20.0))
+        sums += g * 2 * np.maximum(c, 1)
+        counts += c
+    assert np.array_equal(counts, c_s[1:])
+    assert np.allclose(sums / (2 * np.maximum(counts, 1)), g_s[1:], rtol=1e-13)
+
+
+def test_oracle_directional_against_explicit_angles():
+    rs = np.random.RandomState(8)
+    pos = rs.uniform(0, 10, size=(2, 120))
+    f = rs.normal(size=120)
+    edges = np.linspace(0, 8, 5)
+    dirs = np.array([[1.0, 0.0], [np.sqrt(0.5), np.sqrt(0.5)]])
+    tol = np.pi / 8
+    g, c = oracle.vario_directional_c(f, edges, pos, dirs, tol, -1.0, False)
+    j, k = np.triu_indices(120, 1)
+    dv = pos[:, k] - pos[:, j]
+    d = np.hypot(*dv)
+    b = np.searchsorted(edges, d, side="right") - 1
+    for di, u in enumerate(dirs):
+        ang = np.arccos(np.minimum(np.abs(u @ dv) / d, 1.0))
+        sel = (ang < tol) & (b >= 0) & (b < 4)
+        assert np.array_equal(c[di], np.bincount(b[sel], minlength=4))
+    # separate directions: each pair at most once
+    gs_, cs = oracle.vario_directional_c(f, edges, pos, dirs, np.pi / 4, -1.0, True)
+    g_all, _ = oracle.vario_unstructured_c(f, edges, pos)
+    assert np.all(cs.sum(0) <= oracle.vario_unstructured_c(f, edges, pos)[1])
+    del gs_, g_all
+
+
+def test_planted_slips_are_caught():
+    """Inclusive upper edges, or pairs counted in both orders, miss the known counts/values."""
+    x = np.arange(1, 11, dtype=np.double)
+    edges = np.arange(1, 11, dtype=np.double)
+    j, k = np.triu_indices(10, 1)
+    d = np.abs(x[k] - x[j])
+    incl = np.searchsorted(edges, d, side="left") - 1  # edges[b] < d <= edges[b+1]
+    counts_slip = np.bincount(incl[(incl >= 0) & (incl < 9)], minlength=9)
+    assert list(counts_slip[:3]) != [9, 8, 7]
+    g, c = oracle.vario_unstructured_c(Z, edges, x)
+    both = np.bincount(np.concatenate([np.searchsorted(edges, d, side="right") - 1] * 2), minlength=10)[:9]
+    assert list(both[:3]) != list(c[:3])
+
+
+@pytest.fixture
+def oracle_kernels(monkeypatch):
+    """Replace the device kernels by the oracle (host-logic tests only)."""
+
+    def pairs(f, bin_edges, pos, estimator_type, direction=None, angles_tol=np.pi / 8, bandwidth=-1.0,
+              separate_dirs=False):
+        f = np.atleast_2d(np.asarray(f, dtype=np.double))
+        pos = np.atleast_2d(np.asarray(pos, dtype=np.double))
+        if direction is None:
+            g, c = oracle.vario_unstructured_c(f, bin_edges, pos, estimator_type)
+        else:
+            g, c = oracle.vario_directional_c(f, bin_edges, pos, direction, angles_tol, bandwidth, separate_dirs,
+                                              estimator_type)
+        return torch.from_numpy(g), torch.from_numpy(c)
+
+    def lags(f, mask, estimator_type):
+        g, c = oracle.vario_structured_c(f, mask, estimator_type)
+        return torch.from_numpy(g), torch.from_numpy(c)
+
+    monkeypatch.setattr(vmod, "_pairs", pairs)
+    monkeypatch.setattr(vmod, "_lags", lags)
+
+
+def _smooth1d(x, seed):
+    rs = np.random.RandomState(seed)
+    k, ph = rs.normal(size=30) * 0.2, rs.uniform(0, 2 * np.pi, 30)
+    return np.sqrt(2.0 / 30) * np.cos(np.outer(x, k) + ph).sum(1)
+
+
+def _field3d():
+    rs = np.random.RandomState(123456)
+    g = np.meshgrid(*[np.arange(10.0)] * 3, indexing="ij")
+    k, ph = rs.normal(size=(3, 40)) * 0.3, rs.uniform(0, 2 * np.pi, 40)
+    return np.sqrt(2.0 / 40) * np.cos(np.tensordot(np.stack(g, -1), k, 1) + ph).sum(-1)
+
+
+@pytest.mark.parametrize("scenario", ["known_values", "uncorrelated", "sampling", "assertions",
+                                      "direction_assertions"])
+def test_reference_scenarios(oracle_kernels, scenario):
+    getattr(cases, scenario)(gs)
+
+
+def test_reference_fields(oracle_kernels):
+    cases.stacked_and_no_data(gs, _smooth1d)
+    cases.direction_axis(gs, _field3d())
+    cases.direction_angle(gs, _field3d())
+
+
+def test_standard_bins_and_unsupported(oracle_kernels):
+    pos = np.random.RandomState(1).uniform(0, 30, size=(2, 100))
+    edges = gs.standard_bins(pos, dim=2)
+    diam = np.linalg.norm(pos.max(1) - pos.min(1))
+    assert len(edges) == int(np.ceil(2 * np.log2(100) + 1)) + 1 and edges[-1] == pytest.approx(diam / 3)
+    bc, g = gs.vario_estimate(pos, np.ones(100))
+    assert np.allclose(bc, (edges[:-1] + edges[1:]) / 2) and np.all(g == 0)
+    with pytest.raises(NotImplementedError):
+        gs.vario_estimate(pos, np.ones(100), latlon=True)
+    with pytest.raises(NotImplementedError):
+        gs.vario_estimate(pos, np.ones(100), trend=lambda *x: 0.0)
