@@ -16,8 +16,10 @@ baseline (the oracle's C/OpenMP restatement of the reference loop) timed on
 a bounded sample of the same workload.
 
 Extra sections (``--extras``, on by default): structured 512^3 Matern grid
-on the matrix cores, the periodic Fourier method on a 4096^2 grid, forward +
-adjoint on the 2D Stable configuration and the 256-realization batch; each prints its own JSON object on stderr and
+on the matrix cores, the periodic Fourier method on a 4096^2 grid, ordinary
+kriging sums (1000 conditioning points onto 1e6 targets), variogram pair
+binning (1e6 points), forward + adjoint
+on the 2D Stable configuration and the 256-realization batch; each prints its own JSON object on stderr and
 under ``gpurun_out/`` when that folder exists.
 """
 
@@ -184,6 +186,114 @@ def bench_fourier(torch, steps, warmup):
     res["fp64_direct"] = {"value": 128 * side * float(nm) / wall64, "measured_on": "128 of 4096 x-rows"}
     res["speedup_bf16x3_vs_fp64"] = res["bf16x3"]["value"] / res["fp64_direct"]["value"]
     return res
+
+
+def bench_krige(torch, steps, warmup):
+    """Ordinary kriging sums: 3D Exponential, 1000 conditioning points onto 1e6 targets, FP64."""
+    import oracle
+    from gstools_amd import Exponential, krige as gkrige
+
+    log("bench: kriging sums (K 1001, 1e6 targets)")
+    model = Exponential(dim=3, var=2.0, len_scale=50.0)
+    rs = np.random.RandomState(20170519)
+    k_cond, n_pts = 1000, 1_000_000
+    cond_pos = rs.uniform(0, 1000, size=(3, k_cond))
+    cond_val = rs.normal(size=k_cond)
+    krig = gkrige.Ordinary(model, cond_pos, cond_val)
+    dev = torch.device("cuda", 0)
+    mat = torch.from_numpy(krig._krige_mat).to(dev)
+    cond = torch.from_numpy(krig._krige_cond).to(dev)
+    # right-hand sides (krige_size, P) built on the device: covariance rows plus
+    # the unbiasedness row, exactly what the host front end builds per chunk
+    pts = torch.from_numpy(rs.uniform(0, 1000, size=(3, n_pts))).to(dev)
+    cpos = torch.from_numpy(krig._krige_pos).to(dev)
+    vecs = torch.empty((k_cond + 1, n_pts), dtype=torch.float64, device=dev)
+    for c0 in range(0, n_pts, 200_000):
+        d = torch.cdist(cpos.T.contiguous(), pts[:, c0:c0 + 200_000].T.contiguous())
+        vecs[:k_cond, c0:c0 + 200_000] = model.var * torch.exp(-d / model.len_rescaled)
+    vecs[k_cond] = 1.0
+    del d
+    kk = k_cond + 1
+    flops = 2.0 * kk * kk * n_pts
+    both = lambda: gkrige.krige_sum_device(mat, vecs, cond, variance=True)
+    wall_v, ker_v = time_kernel(both, steps, warmup, torch)
+    f, e = both()
+    only = lambda: gkrige.krige_sum_device(mat, vecs, cond, variance=False)
+    wall_f, ker_f = time_kernel(only, steps, warmup, torch)
+    sub = np.random.RandomState(4).choice(n_pts, 2000, replace=False)
+    vs = vecs[:, torch.from_numpy(sub).to(dev)].cpu().numpy()
+    t0 = time.perf_counter()
+    f_ref, e_ref = oracle.krige_c(krig._krige_mat, vs, krig._krige_cond)
+    t_cpu = time.perf_counter() - t0
+    fe = f.cpu().numpy()[sub]
+    ee = e.cpu().numpy()[sub]
+    scale = np.abs(krig._krige_mat) @ np.abs(vs)
+    tflops = flops / ker_v / 1e12
+    return {
+        "workload": "ordinary kriging sums, 3D Exponential var 2 len 50, 1000 conditioning points (+1 unbiased row) "
+                    "onto 1e6 targets uniform in [0,1000]^3, fp64, krig_vecs resident in HBM",
+        "cond_points": kk, "targets": n_pts,
+        "field_and_variance_ms": wall_v * 1e3, "gemm_tflops": tflops,
+        "roofline_frac_fp64_mfma": tflops / FP64_VALU_PEAK_TFLOPS,
+        "field_only_ms": wall_f * 1e3,
+        "field_only_hbm_gbs": 8.0 * kk * n_pts / ker_f / 1e9,
+        "max_err_over_bound_field": float(np.max(np.abs(fe - f_ref) / (np.abs(krig._krige_cond) @ scale))),
+        "max_err_over_bound_error": float(np.max(np.abs(ee - e_ref) / np.sum(np.abs(vs) * scale, axis=0))),
+        "cpu_oracle_points_per_s": 2000 / t_cpu,
+        "gpu_points_per_s": n_pts / wall_v,
+    }
+
+
+def bench_variogram(torch, steps, warmup):
+    """Unstructured 3D variogram of 1e6 points (20 bins), plus directional and axis-lag kernels."""
+    import oracle
+    from gstools_amd import variogram as gv
+
+    log("bench: variogram pair binning (1e6 points, 20 bins)")
+    rs = np.random.RandomState(19970221)
+    n_pts = 1_000_000
+    pos = rs.uniform(0, 1000, size=(3, n_pts))
+    field = rs.normal(size=(1, n_pts))
+    edges = np.linspace(0, 577.0, 21)
+    dev = torch.device("cuda", 0)
+    pt = torch.from_numpy(pos).to(dev)
+    ft = torch.from_numpy(field).to(dev)
+    et = torch.from_numpy(edges).to(dev)
+    pairs = n_pts * (n_pts - 1) / 2
+    run = lambda: gv._pairs(ft, et, pt, "m")
+    wall, ker = time_kernel(run, max(2, steps // 2), 1, torch)
+    g, c = run()
+    # size-independent check: with one bin over [0, inf) every pair is counted once
+    _, call = gv._pairs(ft, torch.tensor([0.0, np.inf], dtype=torch.float64, device=dev), pt, "m")
+    # parity on a 3000-point subset against the oracle (the reference loop over bins x pairs)
+    sub = rs.choice(n_pts, 3000, replace=False)
+    gs_, cs_ = gv.unstructured(field[:, sub], edges, pos[:, sub])
+    t0 = time.perf_counter()
+    g_ref, c_ref = oracle.vario_unstructured_c(field[:, sub], edges, pos[:, sub])
+    t_cpu = time.perf_counter() - t0
+    dirs = np.array([[1.0, 0, 0], [0, 1.0, 0], [0, 0, 1.0]])
+    dt = torch.from_numpy(dirs).to(dev)
+    drun = lambda: gv._pairs(ft[:, :200_000], et, pt[:, :200_000], "m", dt, np.pi / 8, 20.0, True)
+    dwall, _ = time_kernel(drun, 2, 1, torch)
+    lag_f = torch.from_numpy(rs.normal(size=(1024, 100_000))).to(dev)
+    lrun = lambda: gv._lags(lag_f, None, "m")
+    lwall, _ = time_kernel(lrun, 3, 1, torch)
+    sub_pairs = 3000 * 2999 / 2
+    return {
+        "workload": "unstructured 3D variogram, 1e6 points uniform in [0,1000]^3, one field, 20 bins to 577, "
+                    "Matheron, fp64",
+        "pairs": pairs, "ms": wall * 1e3, "pairs_per_s": pairs / wall,
+        "all_pairs_counted_once": bool(int(call.sum().item()) == int(pairs)),
+        "subset_counts_equal_oracle": bool(np.array_equal(cs_, c_ref)),
+        "subset_rel_max_vs_oracle": float(np.max(np.abs(gs_ - g_ref) / np.maximum(np.abs(g_ref), 1e-300))),
+        "cpu_oracle_pairs_per_s": sub_pairs / t_cpu,
+        "cpu_oracle_note": "the reference loop visits every pair once per bin (20 bins); C/OpenMP restatement, "
+                           "all host threads",
+        "directional_3dirs_200k_points_ms": dwall * 1e3,
+        "directional_pairs_per_s": 200_000 * 199_999 / 2 / dwall,
+        "axis_lags_1024x100000_ms": lwall * 1e3,
+        "axis_lag_pairs_per_s": (1024 * 1023 / 2) * 100_000 / lwall,
+    }
 
 
 def bench_adjoint(torch, steps, warmup):
@@ -418,7 +528,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--extras", type=int, default=1, help="also run the other configurations")
-    ap.add_argument("--only", choices=["fp32", "fp64", "mfma", "fourier", "adjoint", "batch", "copy"],
+    ap.add_argument("--only", choices=["fp32", "fp64", "mfma", "fourier", "krige", "variogram", "adjoint", "batch",
+                                       "copy"],
                     help="run one section only (for profiling) and print its JSON")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -440,8 +551,10 @@ def main():
     dev = torch.device("cuda", 0)
     k_ext = max(3, args.steps // 4)
     w_ext = max(1, args.warmup)
-    if args.only in ("mfma", "fourier", "adjoint", "batch", "copy"):
+    if args.only in ("mfma", "fourier", "krige", "variogram", "adjoint", "batch", "copy"):
         fn = {"mfma": lambda: bench_mfma_grid(torch, k_ext, w_ext),
+              "variogram": lambda: bench_variogram(torch, k_ext, w_ext),
+              "krige": lambda: bench_krige(torch, k_ext, w_ext),
               "fourier": lambda: bench_fourier(torch, k_ext, w_ext),
               "adjoint": lambda: bench_adjoint(torch, k_ext, w_ext),
               "batch": lambda: bench_batch(torch, k_ext, w_ext, *batch_inputs),
@@ -525,7 +638,9 @@ def main():
     }
     if args.extras:
         extras = {}
-        sections = (("mfma_grid", bench_mfma_grid), ("fourier", bench_fourier), ("adjoint", bench_adjoint),
+        sections = (("mfma_grid", bench_mfma_grid), ("fourier", bench_fourier), ("krige", bench_krige),
+                    ("variogram", bench_variogram),
+                    ("adjoint", bench_adjoint),
                     ("batch", lambda t, k, w: bench_batch(t, k, w, *batch_inputs)))
         for name, fn in sections:
             try:

@@ -296,9 +296,9 @@ int vario_lds_limit(size_t* out) {
     return GSX_OK;
 }
 
-// histogram slots one launch can hold
-inline int64_t vario_slots_per_launch(size_t limit, int64_t B, int64_t D, int dim) {
-    const size_t fixed = vario_lds_bytes(B, D, dim, kVFields, 0);
+// histogram slots one launch can hold, for fc fields per launch
+inline int64_t vario_slots_per_launch(size_t limit, int64_t B, int64_t D, int dim, int64_t fc) {
+    const size_t fixed = vario_lds_bytes(B, D, dim, fc, 0);
     if (limit <= fixed) return 0;
     return (int64_t)((limit - fixed) / ((sizeof(double) + sizeof(unsigned)) * kVT));
 }
@@ -334,7 +334,8 @@ int vario_pairs(int64_t F, int64_t P, int64_t dim, int64_t B, int64_t D, const d
     size_t limit = 0;
     int st = vario_lds_limit(&limit);
     if (st != GSX_OK) return st;
-    const int64_t per = vario_slots_per_launch(limit, B, D, (int)dim);
+    const int64_t fc_max = F < kVFields ? (F > 0 ? F : 1) : kVFields;  // fields staged per launch
+    const int64_t per = vario_slots_per_launch(limit, B, directional ? D : 1, (int)dim, fc_max);
     if (per < 1) return GSX_E_UNSUPPORTED;
     const int64_t blocks = vario_pair_blocks(P);
     const int64_t chunk_slots = per < NHall ? per : NHall;
@@ -365,7 +366,7 @@ int vario_pairs(int64_t F, int64_t P, int64_t dim, int64_t B, int64_t D, const d
             a.h0 = h0;
             a.h1 = h1;
             a.nt = nt;
-            const size_t lds = vario_lds_bytes(B, a.D, (int)dim, kVFields, h1 - h0);
+            const size_t lds = vario_lds_bytes(B, a.D, (int)dim, fc_max, h1 - h0);
             if (P > 1) {
                 const double* fm = F > 0 ? f + m0 * P : nullptr;
                 if (dim == 1)
@@ -401,14 +402,9 @@ int64_t gsx_vario_workspace(int64_t kind, int64_t P_or_n0, int64_t dim_or_n1, in
         return P_or_n0 * (tiles > 0 ? tiles : 1) * 16;
     }
     if (P_or_n0 < 0 || dim_or_n1 < 1 || dim_or_n1 > gsx::kVMaxDim || B < 0 || D < 1) return GSX_E_INVALID;
-    size_t limit = 0;
-    int st = gsx::vario_lds_limit(&limit);
-    if (st != GSX_OK) return st;
-    const int64_t per = gsx::vario_slots_per_launch(limit, B, kind == 1 ? D : 1, (int)dim_or_n1);
-    if (per < 1) return GSX_E_UNSUPPORTED;
+    // per-block partials of every (direction, bin) slot: enough for any split into launches
     const int64_t slots = (kind == 1 ? D : 1) * B;
-    const int64_t chunk = per < slots ? per : slots;
-    return gsx::vario_pair_blocks(P_or_n0) * (chunk > 0 ? chunk : 1) * 16;
+    return gsx::vario_pair_blocks(P_or_n0) * (slots > 0 ? slots : 1) * 16;
 }
 
 int gsx_vario_unstructured_f64(int64_t F, int64_t P, int64_t dim, int64_t B, const double* f, const double* edges,

@@ -46,12 +46,20 @@ def _ws(nbytes, dev):
 
 
 # ------------------------------------------------------------- kernels ----
+def _tensor(x, name, dev, ndim):
+    """float64 device tensor of ``ndim`` dimensions (leading axes added) from numpy or torch input."""
+    t = _as_f64(x if isinstance(x, torch.Tensor) else np.asarray(x, dtype=np.double), name, dev)
+    while t.dim() < ndim:
+        t = t.unsqueeze(0)
+    return t.contiguous()
+
+
 def _pairs(f, bin_edges, pos, estimator_type, direction=None, angles_tol=np.pi / 8, bandwidth=-1.0,
            separate_dirs=False):
     dev = _device()
-    ft = _as_f64(np.atleast_2d(np.asarray(f, dtype=np.double)), "f", dev)
-    pt = _as_f64(np.atleast_2d(np.asarray(pos, dtype=np.double)), "pos", dev)
-    et = _as_f64(np.atleast_1d(np.asarray(bin_edges, dtype=np.double)), "bin_edges", dev)
+    ft = _tensor(f, "f", dev, 2)
+    pt = _tensor(pos, "pos", dev, 2)
+    et = _tensor(bin_edges, "bin_edges", dev, 1)
     nf, npnt = ft.shape
     dim = pt.shape[0]
     if pt.shape[1] != npnt:
@@ -68,7 +76,7 @@ def _pairs(f, bin_edges, pos, estimator_type, direction=None, angles_tol=np.pi /
         _native.check(lib.gsx_vario_unstructured_f64(nf, npnt, dim, nb, _ptr(ft), _ptr(et), _ptr(pt), est, _ptr(gamma),
                                                      _ptr(counts), _ptr(ws), ws.numel(), _stream()), "unstructured")
         return gamma, counts
-    dt = _as_f64(np.atleast_2d(np.asarray(direction, dtype=np.double)), "direction", dev)
+    dt = _tensor(direction, "direction", dev, 2)
     nd = dt.shape[0]
     if dt.shape[1] != dim:
         raise ValueError(f"Can't interpret directions: {direction}")
@@ -99,7 +107,7 @@ def directional(f, bin_edges, pos, direction, angles_tol=np.pi / 8.0, bandwidth=
 
 def _lags(f, mask, estimator_type):
     dev = _device()
-    ft = _as_f64(np.asarray(f, dtype=np.double), "f", dev)
+    ft = _tensor(f, "f", dev, 1)
     if ft.dim() == 1:
         ft = ft.reshape(-1, 1)
     n0, n1 = ft.shape
