@@ -74,6 +74,7 @@ EXPORTS = {
     "gsx_summate_adjoint_f64": (_i32, _ADJ),
     "gsx_summate_adjoint_f32": (_i32, _ADJ),
     "gsx_krige_workspace": (_i64, [_i64, _i64, _i32]),
+    "gsx_krige_cov_f64": (_i32, [_i32, _f64, _f64, _f64, _f64, _i64, _i64, _i64, _vp, _vp, _vp, _i64, _vp]),
     "gsx_krige_f64": (_i32, [_i64, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _i64, _vp]),
     "gsx_vario_workspace": (_i64, [_i64, _i64, _i64, _i64, _i64]),
     "gsx_vario_unstructured_f64": (_i32, [_i64, _i64, _i64, _i64, _vp, _vp, _vp, _i32, _vp, _vp, _vp, _i64, _vp]),

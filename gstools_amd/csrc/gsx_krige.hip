@@ -20,6 +20,13 @@
 // Field-only form: field = (M^T cond) . v_k, a weight vector (K) followed by
 // an HBM-bound pass over V (no GEMM).
 //
+// Right-hand sides: k_krige_cov writes the covariance rows of V,
+// C(|c_i - x_p|) for conditioning points c_i and targets x_p, for models
+// with closed-form correlations (the reference evaluates
+// model.cov_nugget / model.covariance on scipy's cdist, krige/base.py:357-388;
+// the distance here is rounded the same way: differences, squares and sums
+// in dimension order, no fused multiply-add).
+//
 // MFMA operand layouts (v_mfma_f64_16x16x4, 64 lanes, g = lane / 16):
 //   A: lane holds row (lane % 16), K element g
 //   B: lane holds column (lane % 16), K element g
@@ -184,6 +191,50 @@ __global__ __launch_bounds__(256) void k_krige_field(int64_t K, int64_t P, const
     if (q < P) field[q] = f1;
 }
 
+struct CovModelArgs {
+    int kind;           // 0 Gaussian exp(-h^2), 1 Exponential exp(-h), 2 Stable exp(-h^alpha),
+                        // 3 Spherical 1 - 1.5 h + 0.5 h^3 (h <= 1)
+    double var;         // variance
+    double len;         // rescaled length scale: h = r / len
+    double alpha;       // Stable shape
+    double sill_at_0;   // >= 0: value where r <= 1e-8 (cov_nugget); < 0: none
+};
+
+__device__ __forceinline__ double cov_of(const CovModelArgs& m, double r) {
+    if (m.sill_at_0 >= 0.0 && r <= 1e-8) return m.sill_at_0;
+    const double h = r / m.len;
+    double c;
+    switch (m.kind) {
+        case 0: c = exp(-(h * h)); break;
+        case 1: c = exp(-h); break;
+        case 2: c = exp(-pow(h, m.alpha)); break;
+        default: {
+            const double g = fmin(h, 1.0);
+            c = 1.0 - 1.5 * g + 0.5 * (g * g * g);
+        }
+    }
+    return m.var * c;
+}
+
+// out[i * ld + p] = C(|c_i - x_p|), i < K, p < P
+template <int DIM>
+__global__ __launch_bounds__(256) void k_krige_cov(CovModelArgs m, int64_t K, int64_t P,
+                                                   const double* __restrict__ cpos,
+                                                   const double* __restrict__ pos, double* __restrict__ out,
+                                                   int64_t ld) {
+#pragma clang fp contract(off)
+    const int64_t p = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    const int64_t i = blockIdx.y;
+    if (p >= P) return;
+    double d2 = 0.0;
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) {
+        const double t = cpos[d * K + i] - pos[d * P + p];
+        d2 += t * t;
+    }
+    out[i * ld + p] = cov_of(m, sqrt(d2));
+}
+
 __global__ void k_fill_zero_f64(double* __restrict__ p, int64_t n) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = 0.0;
@@ -241,6 +292,25 @@ int gsx_krige_f64(int64_t K, int64_t P, const double* krig_mat, const double* kr
     if (st != GSX_OK) return st;
     hipLaunchKernelGGL(gsx::k_krige_reduce, dim3((unsigned)gsx::ceil_div(P, (int64_t)256)), dim3(256), 0, s, tiles,
                        P, part_f, part_e, field, error);
+    return gsx::launch_status();
+}
+
+int gsx_krige_cov_f64(int model_kind, double var, double len_rescaled, double alpha, double sill_at_zero,
+                      int64_t dim, int64_t K, int64_t P, const double* cond_pos, const double* pos, double* out,
+                      int64_t ld_out, void* stream) {
+    if (model_kind < 0 || model_kind > 3 || dim < 1 || dim > 3 || K < 0 || P < 0 || ld_out < P) return GSX_E_INVALID;
+    if (!(len_rescaled > 0.0)) return GSX_E_INVALID;
+    if (K == 0 || P == 0) return GSX_OK;
+    if (K > 65535) return GSX_E_INVALID;
+    if (!cond_pos || !pos || !out) return GSX_E_INVALID;
+    gsx::CovModelArgs m{model_kind, var, len_rescaled, alpha, sill_at_zero};
+    hipStream_t s = gsx::as_stream(stream);
+    const dim3 grid((unsigned)gsx::ceil_div(P, (int64_t)256), (unsigned)K);
+    switch (dim) {
+        case 1: hipLaunchKernelGGL((gsx::k_krige_cov<1>), grid, dim3(256), 0, s, m, K, P, cond_pos, pos, out, ld_out); break;
+        case 2: hipLaunchKernelGGL((gsx::k_krige_cov<2>), grid, dim3(256), 0, s, m, K, P, cond_pos, pos, out, ld_out); break;
+        default: hipLaunchKernelGGL((gsx::k_krige_cov<3>), grid, dim3(256), 0, s, m, K, P, cond_pos, pos, out, ld_out);
+    }
     return gsx::launch_status();
 }
 

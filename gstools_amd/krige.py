@@ -12,7 +12,10 @@ weighted column sum for the field alone).
 (``krige/base.py:64-753``, methods ``krige/methods.py``): the kriging matrix
 with measurement errors, unbiasedness and drift rows, its (pseudo-)inverse,
 the right-hand sides per chunk of target points, constant or callable mean,
-trend, ``exact`` / ``cond_err``, ``only_mean`` and ``get_mean``.  The
+trend, ``exact`` / ``cond_err``, ``only_mean`` and ``get_mean``.  For the
+Gaussian, Exponential, Stable and Spherical models the covariance rows of the
+right-hand sides are computed on the GPU (``gsx_krige_cov_f64``); other
+models are evaluated on the host exactly as the reference does.  The
 subclasses :class:`Simple`, :class:`Ordinary`, :class:`Universal`,
 :class:`ExtDrift` and :class:`Detrended` set the same defaults as the
 reference.  Not restated: data normalizers and variogram fitting
@@ -20,7 +23,6 @@ reference.  Not restated: data normalizers and variogram fitting
 raise ``NotImplementedError``).
 """
 
-import ctypes
 from itertools import combinations_with_replacement
 
 import numpy as np
@@ -33,7 +35,7 @@ from .geometry import generate_grid
 from .summate import _as_f64, _device, _ptr, _stream
 
 __all__ = [
-    "calc_field_krige", "calc_field_krige_and_variance", "krige_sum_device",
+    "calc_field_krige", "calc_field_krige_and_variance", "krige_sum_device", "krige_cov_device",
     "Krige", "Simple", "Ordinary", "Universal", "ExtDrift", "Detrended",
 ]
 
@@ -81,6 +83,45 @@ def calc_field_krige_and_variance(krig_mat, krig_vecs, cond, num_threads=None):
     _check_numpy(krig_mat=krig_mat, krig_vecs=krig_vecs, cond=cond)
     field, error = krige_sum_device(krig_mat, krig_vecs, cond, variance=True)
     return field.cpu().numpy(), error.cpu().numpy()
+
+
+def krige_cov_device(model, cond_pos, pos, exact=False, out=None, row0=0):
+    """Covariance rows C(|c_i - x_p|) on the GPU for closed-form models -> tensor (K, P), or None.
+
+    ``cond_pos`` (dim, K) and ``pos`` (dim, P) are isometrized positions;
+    with ``exact`` the value at distance 0 is the sill (``cov_nugget``).
+    Returns None when the model has no device form (the caller evaluates
+    it on the host as the reference does).  With ``out`` the rows are written
+    to ``out[row0:row0 + K]``.
+    """
+    kind = _device_cov_kind(model)
+    if kind is None:
+        return None
+    dev = _device()
+    c = _as_f64(cond_pos, "cond_pos", dev)
+    p = _as_f64(pos, "pos", dev)
+    k, n = c.shape[1], p.shape[1]
+    if k > 65535:
+        return None
+    if out is None:
+        out = torch.empty((k, n), dtype=torch.float64, device=dev)
+        row0 = 0
+    dst = out[row0:row0 + k]
+    alpha = float(getattr(model, "alpha", 0.0)) if kind == 2 else 0.0
+    sill0 = float(model.sill) if exact else -1.0
+    _native.check(_native.lib().gsx_krige_cov_f64(kind, float(model.var), float(model.len_rescaled), alpha, sill0,
+                                                  c.shape[0], k, n, _ptr(c.contiguous()), _ptr(p.contiguous()),
+                                                  _ptr(dst), out.shape[1], _stream()), "krige_cov")
+    return dst
+
+
+def _device_cov_kind(model):
+    from . import covmodel as cm
+
+    for cls, kind in ((cm.Gaussian, 0), (cm.Exponential, 1), (cm.Stable, 2), (cm.Spherical, 3)):
+        if type(model) is cls:
+            return kind
+    return None
 
 
 # ---------------------------------------------------------------- helpers ---
@@ -207,9 +248,13 @@ class Krige:
             chunk_size = pnt_cnt if chunk_size is None else int(chunk_size)
             chunk_no = int(np.ceil(pnt_cnt / chunk_size)) if pnt_cnt else 0
             ext_drift = self._pre_ext_drift(pnt_cnt, ext_drift)
+            on_device = not only_mean and _device_cov_kind(self.model) is not None and self.cond_no <= 65535
             for i in range(chunk_no):
                 sl = (i * chunk_size, min(pnt_cnt, (i + 1) * chunk_size))
-                k_vec = self._get_krige_vecs(iso_pos, sl, ext_drift, only_mean)
+                if on_device:
+                    k_vec = self._get_krige_vecs_device(iso_pos, sl, ext_drift)
+                else:
+                    k_vec = self._get_krige_vecs(iso_pos, sl, ext_drift, only_mean)
                 self._summate(field, krige_var, slice(*sl), k_vec, return_var)
         field = np.reshape(field, shape)
         if post_process:
@@ -290,6 +335,28 @@ class Krige:
         if self.ext_drift_no > 0:
             ext_size = self.krige_size - self.ext_drift_no
             res[ext_size:, :] = ext_drift[:, slice(*chunk_slice)]
+        return res
+
+    def _get_krige_vecs_device(self, pos, chunk_slice, ext_drift=None):
+        """``_get_krige_vecs`` with the covariance rows computed on the GPU (a device tensor)."""
+        dev = _device()
+        chunk_pos = np.ascontiguousarray(pos[:, slice(*chunk_slice)])
+        res = torch.empty((self.krige_size, chunk_pos.shape[1]), dtype=torch.float64, device=dev)
+        krige_cov_device(self.model, self._krige_pos, chunk_pos, exact=self.exact, out=res, row0=0)
+        extra = self.krige_size - self.cond_no
+        if extra:
+            rows = np.empty((extra, chunk_pos.shape[1]), dtype=np.double)
+            off = 0
+            if self.unbiased:
+                rows[0] = 1
+                off = 1
+            if self.int_drift_no > 0:
+                drift_pos = self.model.anisometrize(pos)[:, slice(*chunk_slice)]
+                for i, f in enumerate(self.drift_functions):
+                    rows[off + i] = f(*drift_pos)
+            if self.ext_drift_no > 0:
+                rows[extra - self.ext_drift_no:] = ext_drift[:, slice(*chunk_slice)]
+            res[self.cond_no:] = torch.from_numpy(rows).to(dev)
         return res
 
     def _pre_ext_drift(self, pnt_cnt, ext_drift=None, set_cond=False):

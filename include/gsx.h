@@ -185,6 +185,18 @@ int gsx_summate_adjoint_f32(int64_t d, int64_t N, int64_t P, const double* cov,
  * Replaces the external kernels called at src/gstools/krige/base.py:42-61.
  * Workspace bytes from gsx_krige_workspace(K, P, error != NULL). */
 int64_t gsx_krige_workspace(int64_t K, int64_t P, int variance);
+/* Covariance rows of krig_vecs: out[i * ld_out + p] = C(|c_i - x_p|) for
+ * conditioning points cond_pos (dim, K) and targets pos (dim, P), both
+ * isometrized, row-major; distances rounded like scipy's cdist.  C =
+ * var * cor(r / len_rescaled) for model_kind 0 Gaussian exp(-h^2),
+ * 1 Exponential exp(-h), 2 Stable exp(-h^alpha), 3 Spherical; with
+ * sill_at_zero >= 0 the value at r <= 1e-8 is sill_at_zero (cov_nugget).
+ * K <= 65535.  Replaces the host evaluation at src/gstools/krige/base.py:374-378. */
+int gsx_krige_cov_f64(int model_kind, double var, double len_rescaled,
+                      double alpha, double sill_at_zero, int64_t dim,
+                      int64_t K, int64_t P, const double* cond_pos,
+                      const double* pos, double* out, int64_t ld_out,
+                      void* stream);
 int gsx_krige_f64(int64_t K, int64_t P, const double* krig_mat,
                   const double* krig_vecs, const double* cond, double* field,
                   double* error, void* workspace, int64_t workspace_bytes,

@@ -89,6 +89,27 @@ def test_kriging_system_sizes():
     assert np.all(np.abs((model.sill - var) - e_ref) <= 1e-13 * be + 1e-12 * model.sill)
 
 
+@pytest.mark.parametrize("exact", [False, True])
+def test_device_covariance_rows(exact):
+    """``gsx_krige_cov_f64`` against the host ``covariance`` / ``cov_nugget`` on scipy's cdist."""
+    from scipy.spatial.distance import cdist
+
+    from gstools_amd.krige import krige_cov_device
+
+    _say(f"krige covariance rows exact={exact}")
+    rs = np.random.RandomState(11)
+    for model in (gs.Gaussian(dim=3, var=1.7, len_scale=4.0, nugget=0.2), gs.Exponential(dim=2, var=2.0, len_scale=3.0),
+                  gs.Stable(dim=3, var=0.8, len_scale=5.0, alpha=1.3), gs.Spherical(dim=1, var=1.1, len_scale=6.0)):
+        d = model.dim
+        cpos = rs.uniform(0, 10, size=(d, 37))
+        pos = np.concatenate([rs.uniform(0, 10, size=(d, 500)), cpos[:, :5]], axis=1)  # includes r = 0
+        got = krige_cov_device(model, cpos, pos, exact=exact).cpu().numpy()
+        fn = model.cov_nugget if exact else model.covariance
+        ref = fn(cdist(cpos.T, pos.T))
+        assert np.allclose(got, ref, rtol=4e-16 * 8, atol=1e-300), type(model).__name__
+    assert krige_cov_device(gs.Matern(dim=2), cpos[:2], pos[:2]) is None  # host path for other models
+
+
 @pytest.mark.parametrize("scenario", ["simple", "ordinary", "universal", "detrended", "extdrift", "pseudo",
                                       "error"])
 def test_reference_krige_scenarios(scenario):

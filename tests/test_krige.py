@@ -79,6 +79,7 @@ def oracle_sums(monkeypatch):
         return torch.from_numpy(res)
 
     monkeypatch.setattr(krige_mod, "krige_sum_device", fake)
+    monkeypatch.setattr(krige_mod, "_device_cov_kind", lambda model: None)  # host right-hand sides
 
 
 @pytest.mark.parametrize("scenario", ["simple", "ordinary", "universal", "detrended", "extdrift", "pseudo",
