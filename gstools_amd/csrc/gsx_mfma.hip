@@ -187,7 +187,8 @@ __device__ __forceinline__ float2 cmul(float2 a, float2 b) {
     return {a.x * b.x - a.y * b.y, __builtin_fmaf(a.x, b.y, a.y * b.x)};
 }
 
-template <bool FP16, int G2>
+// E1ONE: the grid has a one-point middle axis (2D grids), E1 == 1, so B is E2.
+template <bool FP16, int G2, bool E1ONE>
 __global__ __launch_bounds__(kMWaves * 64, 2) void k_mfma_grid(MfmaGeom g, double alpha,
                                                                const uint32_t* __restrict__ a_hi,
                                                                const uint32_t* __restrict__ a_lo,
@@ -292,7 +293,7 @@ __global__ __launch_bounds__(kMWaves * 64, 2) void k_mfma_grid(MfmaGeom g, doubl
 #pragma unroll
         for (int cf = 0; cf < kMCF; ++cf) {
             float2 yf[4];
-            {
+            if constexpr (!E1ONE) {
                 const float4* p = reinterpret_cast<const float4*>(e1p[cf] + moff);
                 const float4 q0 = p[0], q1 = p[1];
                 yf[0] = {q0.x, q0.y};
@@ -311,7 +312,7 @@ __global__ __launch_bounds__(kMWaves * 64, 2) void k_mfma_grid(MfmaGeom g, doubl
             uint32_t h4[4], l4[4];
 #pragma unroll
             for (int t = 0; t < 4; ++t) {
-                const float2 v = cmul(yf[t], zf[t]);
+                const float2 v = E1ONE ? zf[t] : cmul(yf[t], zf[t]);
                 split2<FP16>(v.x, v.y, h4[t], l4[t]);
             }
             bh[cf] = u32x4{h4[0], h4[1], h4[2], h4[3]};
@@ -530,8 +531,12 @@ int mfma_launch(const MfmaGeom& g, int64_t d, int64_t N, const double* ax0, cons
         hipLaunchKernelGGL((k_mfma_grid3<FP16>), dim3((unsigned)blocks), dim3(kMWaves * 64), 0, s, g, alpha,
                            a_hi, a_lo, e1, e2, out);
     } else {
-        hipLaunchKernelGGL((k_mfma_grid<FP16, 16>), dim3((unsigned)blocks), dim3(kMWaves * 64), 0, s, g, alpha,
-                           a_hi, a_lo, e1, e2, out);
+        if (d == 2)
+            hipLaunchKernelGGL((k_mfma_grid<FP16, 16, true>), dim3((unsigned)blocks), dim3(kMWaves * 64), 0, s, g,
+                               alpha, a_hi, a_lo, e1, e2, out);
+        else
+            hipLaunchKernelGGL((k_mfma_grid<FP16, 16, false>), dim3((unsigned)blocks), dim3(kMWaves * 64), 0, s, g,
+                               alpha, a_hi, a_lo, e1, e2, out);
     }
     return launch_status();
 }
