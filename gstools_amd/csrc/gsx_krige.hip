@@ -30,7 +30,8 @@
 // MFMA operand layouts (v_mfma_f64_16x16x4, 64 lanes, g = lane / 16):
 //   A: lane holds row (lane % 16), K element g
 //   B: lane holds column (lane % 16), K element g
-//   C: lane holds column (lane % 16), rows 4g .. 4g+3
+//   C: lane holds column (lane % 16), rows g, g+4, g+8, g+12 (register r holds
+//      row g + 4r; unlike the 16-bit 16x16 MFMAs, whose lanes hold rows 4g..4g+3)
 #include "gsx_internal.h"
 
 namespace gsx {
@@ -120,7 +121,7 @@ __global__ __launch_bounds__(256) void k_krige_gemm(int64_t K, int64_t P, const 
         for (int rt = 0; rt < 2; ++rt)
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                const int64_t i = i0 + wr * 32 + rt * 16 + 4 * g + r;
+                const int64_t i = i0 + wr * 32 + rt * 16 + g + 4 * r;
                 if (i < K && p < P) {
                     const double w = acc[rt][ct][r];
                     f += cond[i] * w;

@@ -106,7 +106,8 @@ def test_device_covariance_rows(exact):
         got = krige_cov_device(model, cpos, pos, exact=exact).cpu().numpy()
         fn = model.cov_nugget if exact else model.covariance
         ref = fn(cdist(cpos.T, pos.T))
-        assert np.allclose(got, ref, rtol=4e-16 * 8, atol=1e-300), type(model).__name__
+        # absolute bar: the Spherical polynomial cancels to 0 at the range
+        assert np.max(np.abs(got - ref)) <= 8 * 2.3e-16 * model.sill, type(model).__name__
     assert krige_cov_device(gs.Matern(dim=2), cpos[:2], pos[:2]) is None  # host path for other models
 
 
