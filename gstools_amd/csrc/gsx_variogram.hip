@@ -61,7 +61,18 @@ __device__ __forceinline__ int64_t upper_edge(const double* e, int64_t B, double
     return lo - 1;
 }
 
-template <int DIM>
+// Smallest-index form of the bin search for non-decreasing edges: g in [-1, B]
+// with e[g] <= x < e[g+1] (e[-1] = -inf, e[B+1] = +inf), starting from a guess
+// (uniform edges: the guess is exact but for rounding, so the loops run 0-1 times).
+__device__ __forceinline__ int bin_from_guess(const double* e, int B, double x, int g) {
+    while (g >= 0 && e[g] > x) --g;
+    while (g < B && e[g + 1] <= x) ++g;
+    return g;
+}
+
+// FAST: one field, no directions; a block whose edges are non-decreasing runs a
+// straight-line pair loop (the general loop covers everything else).
+template <int DIM, bool FAST>
 __global__ __launch_bounds__(kVT) void k_vario_pairs(VarioArgs a, const double* __restrict__ f,
                                                      const double* __restrict__ edges,
                                                      const double* __restrict__ pos,
@@ -89,6 +100,13 @@ __global__ __launch_bounds__(kVT) void k_vario_pairs(VarioArgs a, const double* 
     __syncthreads();
     bool mono = true;
     for (int64_t i = 0; i < a.B; ++i) mono = mono && (s_edges[i] <= s_edges[i + 1]);
+    // near-uniform edges: bin guess from the spacing (exactness comes from the edge checks)
+    const int Bi = (int)a.B, NHi = (int)NH, h0i = (int)a.h0;
+    const double e0 = s_edges[0];
+    const double w = Bi > 0 ? (s_edges[Bi] - e0) / Bi : 0.0;
+    bool uniform = mono && Bi > 0 && w > 0.0 && w < 1.0e300;
+    for (int i = 0; uniform && i <= Bi; ++i) uniform = fabs(s_edges[i] - (e0 + i * w)) <= 0.25 * w;
+    const double inv_w = uniform ? 1.0 / w : 0.0;
     // directions to test: all of those whose slots this launch records, and
     // with separate_dirs also the earlier ones (they may take the pair)
     const int64_t d_last = a.directional ? (a.h1 - 1) / a.B : 0;
@@ -114,6 +132,48 @@ __global__ __launch_bounds__(kVT) void k_vario_pairs(VarioArgs a, const double* 
 #pragma unroll
         for (int m = 0; m < kVFields; ++m) fj[m] = (m < a.F) ? f[m * a.P + j] : 0.0;
         const int kstart = (tj == tk) ? t + 1 : 0;
+        if constexpr (FAST) {
+            if (mono) {
+                const double fj0 = fj[0];
+                const bool fj_ok = (a.F > 0) && (fj0 == fj0);
+                for (int kk = kstart; kk < (int)kn; ++kk) {
+                    double d2 = 0.0;
+#pragma unroll
+                    for (int c = 0; c < DIM; ++c) {
+                        const double dd = s_pos[c * kVT + kk] - xj[c];
+                        d2 += dd * dd;
+                    }
+                    const double dist = sqrt(d2);
+                    const double fk = s_f[kk];
+                    const bool ok = fj_ok && (fk == fk);
+                    const double val = ok ? vario_est(a.cressie, fk - fj0) : 0.0;
+                    if (dist != dist) {  // NaN distance: the reference's comparisons admit it to every bin
+                        for (int b = 0; b < Bi; ++b) {
+                            const int slot = b - h0i;
+                            if ((unsigned)slot < (unsigned)NHi && ok) {
+                                h_sum[slot * kVT + t] += val;
+                                h_cnt[slot * kVT + t] += 1u;
+                            }
+                        }
+                        continue;
+                    }
+                    int g;
+                    if (uniform) {
+                        const double q = (dist - e0) * inv_w;
+                        g = q < 0.0 ? -1 : (q >= (double)Bi ? Bi : (int)q);
+                        g = bin_from_guess(s_edges, Bi, dist, g);
+                    } else {
+                        g = (int)upper_edge(s_edges, a.B, dist);
+                    }
+                    const int slot = g - h0i;
+                    if (g >= 0 && g < Bi && (unsigned)slot < (unsigned)NHi && ok) {
+                        h_sum[slot * kVT + t] += val;
+                        h_cnt[slot * kVT + t] += 1u;
+                    }
+                }
+                continue;
+            }
+        }
         for (int kk = kstart; kk < kn; ++kk) {
             double d2 = 0.0;
             double dv[DIM];
@@ -303,16 +363,25 @@ inline int64_t vario_slots_per_launch(size_t limit, int64_t B, int64_t D, int di
     return (int64_t)((limit - fixed) / ((sizeof(double) + sizeof(unsigned)) * kVT));
 }
 
+template <int DIM, bool FAST>
+int launch_pairs_t(const VarioArgs& a, size_t lds, int64_t blocks, hipStream_t s, const double* f,
+                   const double* edges, const double* pos, const double* dirs, double* part_sum,
+                   unsigned long long* part_cnt) {
+    if (lds > 65536 && hipFuncSetAttribute(reinterpret_cast<const void*>(&k_vario_pairs<DIM, FAST>),
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) != hipSuccess)
+        return GSX_E_LAUNCH;
+    hipLaunchKernelGGL((k_vario_pairs<DIM, FAST>), dim3((unsigned)blocks), dim3(kVT), lds, s, a, f, edges, pos,
+                       dirs, part_sum, part_cnt);
+    return launch_status();
+}
+
 template <int DIM>
 int launch_pairs(const VarioArgs& a, size_t lds, int64_t blocks, hipStream_t s, const double* f,
                  const double* edges, const double* pos, const double* dirs, double* part_sum,
                  unsigned long long* part_cnt) {
-    if (lds > 65536 && hipFuncSetAttribute(reinterpret_cast<const void*>(&k_vario_pairs<DIM>),
-                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) != hipSuccess)
-        return GSX_E_LAUNCH;
-    hipLaunchKernelGGL((k_vario_pairs<DIM>), dim3((unsigned)blocks), dim3(kVT), lds, s, a, f, edges, pos, dirs,
-                       part_sum, part_cnt);
-    return launch_status();
+    if (!a.directional && a.F <= 1)
+        return launch_pairs_t<DIM, true>(a, lds, blocks, s, f, edges, pos, dirs, part_sum, part_cnt);
+    return launch_pairs_t<DIM, false>(a, lds, blocks, s, f, edges, pos, dirs, part_sum, part_cnt);
 }
 
 inline int64_t vario_pair_blocks(int64_t P) {
@@ -329,6 +398,7 @@ int vario_pairs(int64_t F, int64_t P, int64_t dim, int64_t B, int64_t D, const d
         return GSX_E_INVALID;
     const int64_t NHall = D * B;
     if (NHall == 0) return GSX_OK;
+    if (B > 0x3fffffff || NHall > 0x3fffffff) return GSX_E_INVALID;
     if (!gamma || !counts || !edges || (directional && !dirs) || (P > 0 && (!pos || (F > 0 && !f))))
         return GSX_E_INVALID;
     size_t limit = 0;
