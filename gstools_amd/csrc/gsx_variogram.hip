@@ -61,6 +61,25 @@ __device__ __forceinline__ int64_t upper_edge(const double* e, int64_t B, double
     return lo - 1;
 }
 
+// T(e) = the smallest double x >= 0 with sqrt(x) >= e (sqrt correctly rounded,
+// hence monotone): sqrt(d2) >= e exactly when d2 >= T(e), so bins can be found on
+// squared distances with the reference's sqrt-then-compare decisions.  e <= 0
+// gives -1 (every d2 >= 0 passes), e = +inf gives +inf, NaN stays NaN.
+__device__ double sq_threshold(double e) {
+    if (e != e) return e;
+    if (e <= 0.0) return -1.0;
+    double x = e * e;
+    while (sqrt(x) < e) x = nextafter(x, __builtin_inf());
+    for (;;) {
+        const double y = nextafter(x, 0.0);
+        if (y >= 0.0 && sqrt(y) >= e && y < x)
+            x = y;
+        else
+            break;
+    }
+    return x;
+}
+
 // Smallest-index form of the bin search for non-decreasing edges: g in [-1, B]
 // with e[g] <= x < e[g+1] (e[-1] = -inf, e[B+1] = +inf), starting from a guess
 // (uniform edges: the guess is exact but for rounding, so the loops run 0-1 times).
@@ -72,7 +91,9 @@ __device__ __forceinline__ int bin_from_guess(const double* e, int B, double x, 
 
 // FAST: one field, no directions; a block whose edges are non-decreasing runs a
 // straight-line pair loop (the general loop covers everything else).
-template <int DIM, bool FAST>
+// CRESSIE selects the estimator of the FAST loop at compile time (a runtime flag
+// would be if-converted into computing both estimators for every pair).
+template <int DIM, bool FAST, bool CRESSIE>
 __global__ __launch_bounds__(kVT) void k_vario_pairs(VarioArgs a, const double* __restrict__ f,
                                                      const double* __restrict__ edges,
                                                      const double* __restrict__ pos,
@@ -84,16 +105,21 @@ __global__ __launch_bounds__(kVT) void k_vario_pairs(VarioArgs a, const double* 
     const int64_t NH = a.h1 - a.h0;
     const int t = threadIdx.x;
     double* s_edges = reinterpret_cast<double*>(vlds);
-    double* s_dirs = s_edges + (a.B + 1);
+    double* s_thr = s_edges + (a.B + 1);  // squared-distance thresholds of the edges
+    double* s_dirs = s_thr + (a.B + 1);
     double* s_pos = s_dirs + a.D * DIM;                     // [DIM][kVT]
     double* s_f = s_pos + DIM * kVT;                        // [F][kVT]
-    double* h_sum = s_f + a.F * kVT;                        // [NH][kVT]
-    unsigned* h_cnt = reinterpret_cast<unsigned*>(h_sum + NH * kVT);  // [NH][kVT]
+    double* h_sum = s_f + a.F * kVT;                        // [NH + 1][kVT], row NH scratch
+    unsigned* h_cnt = reinterpret_cast<unsigned*>(h_sum + (NH + 1) * kVT);  // [NH + 1][kVT]
 
-    for (int64_t i = t; i <= a.B; i += kVT) s_edges[i] = edges[i];
+    for (int64_t i = t; i <= a.B; i += kVT) {
+        const double e = edges[i];
+        s_edges[i] = e;
+        s_thr[i] = sq_threshold(e);
+    }
     if (a.directional)
         for (int64_t i = t; i < a.D * DIM; i += kVT) s_dirs[i] = dirs[i];
-    for (int64_t h = 0; h < NH; ++h) {
+    for (int64_t h = 0; h <= NH; ++h) {
         h_sum[h * kVT + t] = 0.0;
         h_cnt[h * kVT + t] = 0u;
     }
@@ -107,11 +133,15 @@ __global__ __launch_bounds__(kVT) void k_vario_pairs(VarioArgs a, const double* 
     bool uniform = mono && Bi > 0 && w > 0.0 && w < 1.0e300;
     for (int i = 0; uniform && i <= Bi; ++i) uniform = fabs(s_edges[i] - (e0 + i * w)) <= 0.25 * w;
     const double inv_w = uniform ? 1.0 / w : 0.0;
+    const float e0f = (float)e0, inv_wf = (float)inv_w;
+    (void)inv_w;
     // directions to test: all of those whose slots this launch records, and
     // with separate_dirs also the earlier ones (they may take the pair)
     const int64_t d_last = a.directional ? (a.h1 - 1) / a.B : 0;
     const int64_t d_first = a.directional ? (a.separate ? 0 : a.h0 / a.B) : 0;
 
+    double nan_sum = 0.0;  // NaN-distance pairs of this thread (every bin takes them)
+    unsigned nan_cnt = 0u;
     const int64_t ntp = a.nt * a.nt;
     for (int64_t tt = blockIdx.x; tt < ntp; tt += gridDim.x) {
         const int64_t tj = tt / a.nt, tk = tt - tj * a.nt;
@@ -136,39 +166,75 @@ __global__ __launch_bounds__(kVT) void k_vario_pairs(VarioArgs a, const double* 
             if (mono) {
                 const double fj0 = fj[0];
                 const bool fj_ok = (a.F > 0) && (fj0 == fj0);
-                for (int kk = kstart; kk < (int)kn; ++kk) {
-                    double d2 = 0.0;
+                const int kend = (int)kn;
+                // four pairs per step: their bin searches overlap, and after
+                // merging equal slots the four histogram updates are independent
+                for (int k4 = kstart; k4 < kend; k4 += 4) {
+                    int slot[4];
+                    double val[4];
+                    unsigned cnt[4];
 #pragma unroll
-                    for (int c = 0; c < DIM; ++c) {
-                        const double dd = s_pos[c * kVT + kk] - xj[c];
-                        d2 += dd * dd;
-                    }
-                    const double dist = sqrt(d2);
-                    const double fk = s_f[kk];
-                    const bool ok = fj_ok && (fk == fk);
-                    const double val = ok ? vario_est(a.cressie, fk - fj0) : 0.0;
-                    if (dist != dist) {  // NaN distance: the reference's comparisons admit it to every bin
-                        for (int b = 0; b < Bi; ++b) {
-                            const int slot = b - h0i;
-                            if ((unsigned)slot < (unsigned)NHi && ok) {
-                                h_sum[slot * kVT + t] += val;
-                                h_cnt[slot * kVT + t] += 1u;
-                            }
+                    for (int u = 0; u < 4; ++u) {
+                        const int kk = k4 + u;
+                        const bool live = kk < kend;
+                        const int kc = live ? kk : kstart;
+                        double d2 = 0.0;
+#pragma unroll
+                        for (int c = 0; c < DIM; ++c) {
+                            const double dd = s_pos[c * kVT + kc] - xj[c];
+                            d2 += dd * dd;
                         }
-                        continue;
+                        const double fk = s_f[kc];
+                        const bool ok = live && fj_ok && (fk == fk);
+                        const double x = fk - fj0;
+                        const double v = ok ? (CRESSIE ? sqrt(fabs(x)) : x * x) : 0.0;
+                        const bool dnan = d2 != d2;  // passes the reference's comparisons for every bin
+                        nan_sum += (dnan && ok) ? v : 0.0;
+                        nan_cnt += (dnan && ok) ? 1u : 0u;
+                        int g;
+                        if (uniform) {
+                            const float q = (__builtin_sqrtf((float)d2) - e0f) * inv_wf;
+                            g = (q >= 0.0f) ? ((q < (float)Bi) ? (int)q : Bi) : -1;
+                            const double tdown = s_thr[g < 0 ? 0 : g];
+                            g = (g >= 0 && tdown > d2) ? g - 1 : g;
+                            const double tup = s_thr[g + 1 > Bi ? Bi : g + 1];
+                            g = (g < Bi && tup <= d2) ? g + 1 : g;
+                            const double t0 = s_thr[g < 0 ? 0 : g];
+                            const double t1 = s_thr[g + 1 > Bi ? Bi : g + 1];
+                            const bool exact = dnan || ((g < 0 || t0 <= d2) && (g >= Bi || t1 > d2));
+                            if (!exact) g = (int)upper_edge(s_thr, a.B, d2);
+                        } else {
+                            g = (int)upper_edge(s_thr, a.B, d2);
+                        }
+                        const int sl = g - h0i;
+                        const bool in = ok && !dnan && g >= 0 && g < Bi && (unsigned)sl < (unsigned)NHi;
+                        slot[u] = in ? sl : NHi;  // row NH: a scratch row
+                        val[u] = in ? v : 0.0;
+                        cnt[u] = in ? 1u : 0u;
                     }
-                    int g;
-                    if (uniform) {
-                        const double q = (dist - e0) * inv_w;
-                        g = q < 0.0 ? -1 : (q >= (double)Bi ? Bi : (int)q);
-                        g = bin_from_guess(s_edges, Bi, dist, g);
-                    } else {
-                        g = (int)upper_edge(s_edges, a.B, dist);
+                    // merge equal slots into the first occurrence (fixed order)
+#pragma unroll
+                    for (int u = 1; u < 4; ++u)
+#pragma unroll
+                        for (int w2 = 0; w2 < u; ++w2)
+                            if (slot[u] != NHi && slot[u] == slot[w2]) {
+                                val[w2] += val[u];
+                                cnt[w2] += cnt[u];
+                                slot[u] = NHi;
+                                val[u] = 0.0;
+                                cnt[u] = 0u;
+                            }
+                    double hs[4];
+                    unsigned hc[4];
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) {
+                        hs[u] = h_sum[slot[u] * kVT + t];
+                        hc[u] = h_cnt[slot[u] * kVT + t];
                     }
-                    const int slot = g - h0i;
-                    if (g >= 0 && g < Bi && (unsigned)slot < (unsigned)NHi && ok) {
-                        h_sum[slot * kVT + t] += val;
-                        h_cnt[slot * kVT + t] += 1u;
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) {
+                        h_sum[slot[u] * kVT + t] = hs[u] + val[u];
+                        h_cnt[slot[u] * kVT + t] = hc[u] + cnt[u];
                     }
                 }
                 continue;
@@ -234,6 +300,15 @@ __global__ __launch_bounds__(kVT) void k_vario_pairs(VarioArgs a, const double* 
                     }
                     if (a.separate) break;
                 }
+            }
+        }
+    }
+    if (nan_cnt) {
+        for (int h = 0; h < NHi; ++h) {
+            const int b = h + h0i;
+            if (b < Bi) {
+                h_sum[h * kVT + t] += nan_sum;
+                h_cnt[h * kVT + t] += nan_cnt;
             }
         }
     }
@@ -336,10 +411,11 @@ __global__ void k_vario_lag_collect(int64_t n0, int64_t tiles, const double* __r
     counts[k] = (long long)c;
 }
 
+// LDS bytes of a pair launch with NH histogram slots (+ one scratch row)
 inline size_t vario_lds_bytes(int64_t B, int64_t D, int dim, int64_t F, int64_t NH) {
-    return sizeof(double) * ((size_t)(B + 1) + (size_t)(D * dim) + (size_t)dim * kVT + (size_t)F * kVT +
-                             (size_t)NH * kVT) +
-           sizeof(unsigned) * (size_t)NH * kVT;
+    return sizeof(double) * (2 * (size_t)(B + 1) + (size_t)(D * dim) + (size_t)dim * kVT + (size_t)F * kVT +
+                             (size_t)(NH + 1) * kVT) +
+           sizeof(unsigned) * (size_t)(NH + 1) * kVT;
 }
 
 int vario_lds_limit(size_t* out) {
@@ -363,15 +439,16 @@ inline int64_t vario_slots_per_launch(size_t limit, int64_t B, int64_t D, int di
     return (int64_t)((limit - fixed) / ((sizeof(double) + sizeof(unsigned)) * kVT));
 }
 
-template <int DIM, bool FAST>
+template <int DIM, bool FAST, bool CRESSIE>
 int launch_pairs_t(const VarioArgs& a, size_t lds, int64_t blocks, hipStream_t s, const double* f,
                    const double* edges, const double* pos, const double* dirs, double* part_sum,
                    unsigned long long* part_cnt) {
-    if (lds > 65536 && hipFuncSetAttribute(reinterpret_cast<const void*>(&k_vario_pairs<DIM, FAST>),
-                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) != hipSuccess)
+    if (lds > 65536 &&
+        hipFuncSetAttribute(reinterpret_cast<const void*>(&k_vario_pairs<DIM, FAST, CRESSIE>),
+                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) != hipSuccess)
         return GSX_E_LAUNCH;
-    hipLaunchKernelGGL((k_vario_pairs<DIM, FAST>), dim3((unsigned)blocks), dim3(kVT), lds, s, a, f, edges, pos,
-                       dirs, part_sum, part_cnt);
+    hipLaunchKernelGGL((k_vario_pairs<DIM, FAST, CRESSIE>), dim3((unsigned)blocks), dim3(kVT), lds, s, a, f, edges,
+                       pos, dirs, part_sum, part_cnt);
     return launch_status();
 }
 
@@ -379,9 +456,12 @@ template <int DIM>
 int launch_pairs(const VarioArgs& a, size_t lds, int64_t blocks, hipStream_t s, const double* f,
                  const double* edges, const double* pos, const double* dirs, double* part_sum,
                  unsigned long long* part_cnt) {
-    if (!a.directional && a.F <= 1)
-        return launch_pairs_t<DIM, true>(a, lds, blocks, s, f, edges, pos, dirs, part_sum, part_cnt);
-    return launch_pairs_t<DIM, false>(a, lds, blocks, s, f, edges, pos, dirs, part_sum, part_cnt);
+    if (!a.directional && a.F <= 1) {
+        if (a.cressie)
+            return launch_pairs_t<DIM, true, true>(a, lds, blocks, s, f, edges, pos, dirs, part_sum, part_cnt);
+        return launch_pairs_t<DIM, true, false>(a, lds, blocks, s, f, edges, pos, dirs, part_sum, part_cnt);
+    }
+    return launch_pairs_t<DIM, false, false>(a, lds, blocks, s, f, edges, pos, dirs, part_sum, part_cnt);
 }
 
 inline int64_t vario_pair_blocks(int64_t P) {

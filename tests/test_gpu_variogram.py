@@ -63,6 +63,22 @@ def test_edges_nan_positions_and_many_bins():
         _close(g, g_ref)
 
 
+def test_edges_on_pair_distances():
+    """Edges placed exactly on pair distances (the reference's rounding): those pairs go to the upper bin."""
+    _say("vario edges on pair distances")
+    rs = np.random.RandomState(12)
+    pos = np.round(rs.uniform(0, 10, size=(3, 400)), 3)
+    f = rs.normal(size=400)
+    j, k = np.triu_indices(400, 1)
+    d = np.sqrt(((pos[:, j] - pos[:, k]) ** 2).sum(0))
+    edges = np.unique(np.concatenate([[0.0], rs.choice(d, 30, replace=False), [20.0]]))
+    for e in (edges, np.linspace(0.0, d[17], 25), np.concatenate([[d[5]], d[5] + np.arange(1, 12) * 0.3])):
+        g, c = v.unstructured(f, e, pos)
+        g_ref, c_ref = oracle.vario_unstructured_c(f, e, pos)
+        assert np.array_equal(c, c_ref)
+        _close(g, g_ref)
+
+
 @pytest.mark.parametrize("separate,bandwidth", [(False, -1.0), (True, -1.0), (False, 1.5), (True, 0.8)])
 def test_directional_matches_oracle(separate, bandwidth):
     _say(f"vario directional separate={separate} bandwidth={bandwidth}")

@@ -124,6 +124,32 @@ def test_planted_slips_are_caught():
     assert list(both[:3]) != list(c[:3])
 
 
+def _sq_threshold(e):
+    """Smallest double x with sqrt(x) >= e (the squared-distance thresholds of the pair kernel)."""
+    if e <= 0:
+        return -1.0
+    x = e * e
+    while np.sqrt(x) < e:
+        x = np.nextafter(x, np.inf)
+    while True:
+        y = np.nextafter(x, 0.0)
+        if y >= 0 and np.sqrt(y) >= e and y < x:
+            x = y
+        else:
+            return x
+
+
+def test_squared_distance_thresholds_decide_like_sqrt():
+    """sqrt(d2) >= e exactly when d2 >= T(e): bins found on squared distances are the reference's."""
+    rs = np.random.RandomState(2)
+    edges = np.concatenate([rs.uniform(0, 50, 300), np.sqrt(rs.uniform(0, 2500, 300)), [1.0, 2.0, 0.5, 1e-300]])
+    for e in edges:
+        t = _sq_threshold(e)
+        assert np.sqrt(t) >= e and not np.sqrt(np.nextafter(t, 0.0)) >= e
+        for x in (t, np.nextafter(t, 0.0), np.nextafter(t, np.inf), e * e):
+            assert (np.sqrt(x) >= e) == (x >= t)
+
+
 @pytest.fixture
 def oracle_kernels(monkeypatch):
     """Replace the device kernels by the oracle (host-logic tests only)."""
