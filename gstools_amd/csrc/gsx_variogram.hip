@@ -27,6 +27,8 @@
 // Structured kernel: lag k along the first axis of an (n0, n1) array; block =
 // (lag, 256-column tile), each thread sums over i for its column, blocks
 // write per-tile partials reduced in a fixed order.
+#include <type_traits>
+
 #include "gsx_internal.h"
 
 namespace gsx {
@@ -130,7 +132,11 @@ __global__ __launch_bounds__(kVT) void k_vario_pairs(VarioArgs a, const double* 
     const int Bi = (int)a.B, NHi = (int)NH, h0i = (int)a.h0;
     const double e0 = s_edges[0];
     const double w = Bi > 0 ? (s_edges[Bi] - e0) / Bi : 0.0;
-    bool uniform = mono && Bi > 0 && w > 0.0 && w < 1.0e300;
+    // uniform: every edge within w/4 of e0 + i w, and w not below 4e-6 of the edge
+    // magnitudes (an FP32 distance then guesses the bin to within one); the FP32
+    // guess also needs the last edge below 1e18 (larger squared distances overflow)
+    const double emax = fmax(fabs(e0), fabs(s_edges[Bi > 0 ? Bi : 0]));
+    bool uniform = mono && Bi > 0 && w > 0.0 && emax < 1.0e18 && w >= 4.0e-6 * emax;
     for (int i = 0; uniform && i <= Bi; ++i) uniform = fabs(s_edges[i] - (e0 + i * w)) <= 0.25 * w;
     const double inv_w = uniform ? 1.0 / w : 0.0;
     const float e0f = (float)e0, inv_wf = (float)inv_w;
@@ -169,74 +175,79 @@ __global__ __launch_bounds__(kVT) void k_vario_pairs(VarioArgs a, const double* 
                 const int kend = (int)kn;
                 // four pairs per step: their bin searches overlap, and after
                 // merging equal slots the four histogram updates are independent
-                for (int k4 = kstart; k4 < kend; k4 += 4) {
-                    int slot[4];
-                    double val[4];
-                    unsigned cnt[4];
+                auto run = [&](auto uni) {
+                    constexpr bool UNI = decltype(uni)::value;
+                    for (int k4 = kstart; k4 < kend; k4 += 4) {
+                        int slot[4];
+                        double val[4];
+                        unsigned cnt[4];
 #pragma unroll
-                    for (int u = 0; u < 4; ++u) {
-                        const int kk = k4 + u;
-                        const bool live = kk < kend;
-                        const int kc = live ? kk : kstart;
-                        double d2 = 0.0;
+                        for (int u = 0; u < 4; ++u) {
+                            const int kk = k4 + u;
+                            const bool live = kk < kend;
+                            const int kc = live ? kk : kstart;
+                            double d2 = 0.0;
 #pragma unroll
-                        for (int c = 0; c < DIM; ++c) {
-                            const double dd = s_pos[c * kVT + kc] - xj[c];
-                            d2 += dd * dd;
-                        }
-                        const double fk = s_f[kc];
-                        const bool ok = live && fj_ok && (fk == fk);
-                        const double x = fk - fj0;
-                        const double v = ok ? (CRESSIE ? sqrt(fabs(x)) : x * x) : 0.0;
-                        const bool dnan = d2 != d2;  // passes the reference's comparisons for every bin
-                        nan_sum += (dnan && ok) ? v : 0.0;
-                        nan_cnt += (dnan && ok) ? 1u : 0u;
-                        int g;
-                        if (uniform) {
-                            const float q = (__builtin_sqrtf((float)d2) - e0f) * inv_wf;
-                            g = (q >= 0.0f) ? ((q < (float)Bi) ? (int)q : Bi) : -1;
-                            const double tdown = s_thr[g < 0 ? 0 : g];
-                            g = (g >= 0 && tdown > d2) ? g - 1 : g;
-                            const double tup = s_thr[g + 1 > Bi ? Bi : g + 1];
-                            g = (g < Bi && tup <= d2) ? g + 1 : g;
-                            const double t0 = s_thr[g < 0 ? 0 : g];
-                            const double t1 = s_thr[g + 1 > Bi ? Bi : g + 1];
-                            const bool exact = dnan || ((g < 0 || t0 <= d2) && (g >= Bi || t1 > d2));
-                            if (!exact) g = (int)upper_edge(s_thr, a.B, d2);
-                        } else {
-                            g = (int)upper_edge(s_thr, a.B, d2);
-                        }
-                        const int sl = g - h0i;
-                        const bool in = ok && !dnan && g >= 0 && g < Bi && (unsigned)sl < (unsigned)NHi;
-                        slot[u] = in ? sl : NHi;  // row NH: a scratch row
-                        val[u] = in ? v : 0.0;
-                        cnt[u] = in ? 1u : 0u;
-                    }
-                    // merge equal slots into the first occurrence (fixed order)
-#pragma unroll
-                    for (int u = 1; u < 4; ++u)
-#pragma unroll
-                        for (int w2 = 0; w2 < u; ++w2)
-                            if (slot[u] != NHi && slot[u] == slot[w2]) {
-                                val[w2] += val[u];
-                                cnt[w2] += cnt[u];
-                                slot[u] = NHi;
-                                val[u] = 0.0;
-                                cnt[u] = 0u;
+                            for (int c = 0; c < DIM; ++c) {
+                                const double dd = s_pos[c * kVT + kc] - xj[c];
+                                d2 += dd * dd;
                             }
-                    double hs[4];
-                    unsigned hc[4];
+                            const double fk = s_f[kc];
+                            const bool ok = live && fj_ok && (fk == fk);
+                            const double x = fk - fj0;
+                            const double v = ok ? (CRESSIE ? sqrt(fabs(x)) : x * x) : 0.0;
+                            const bool dnan = d2 != d2;  // passes the reference's comparisons for every bin
+                            nan_sum += (dnan && ok) ? v : 0.0;
+                            nan_cnt += (dnan && ok) ? 1u : 0u;
+                            int g;
+                            if constexpr (UNI) {
+                                // guess from an FP32 distance: off by at most one bin for
+                                // the edges `uniform` admits; one step each way is exact
+                                const float q = (__builtin_amdgcn_sqrtf((float)d2) - e0f) * inv_wf;
+                                g = (q >= 0.0f) ? ((q < (float)Bi) ? (int)q : Bi) : -1;
+                                const double tdown = s_thr[g < 0 ? 0 : g];
+                                g = (g >= 0 && tdown > d2) ? g - 1 : g;
+                                const double tup = s_thr[g + 1 > Bi ? Bi : g + 1];
+                                g = (g < Bi && tup <= d2) ? g + 1 : g;
+                            } else {
+                                g = (int)upper_edge(s_thr, a.B, d2);
+                            }
+                            const int sl = g - h0i;
+                            const bool in = ok && !dnan && g >= 0 && g < Bi && (unsigned)sl < (unsigned)NHi;
+                            slot[u] = in ? sl : NHi;  // row NH: a scratch row
+                            val[u] = in ? v : 0.0;
+                            cnt[u] = in ? 1u : 0u;
+                        }
+                        // merge equal slots into the first occurrence (fixed order, selects only)
 #pragma unroll
-                    for (int u = 0; u < 4; ++u) {
-                        hs[u] = h_sum[slot[u] * kVT + t];
-                        hc[u] = h_cnt[slot[u] * kVT + t];
-                    }
+                        for (int u = 1; u < 4; ++u)
 #pragma unroll
-                    for (int u = 0; u < 4; ++u) {
-                        h_sum[slot[u] * kVT + t] = hs[u] + val[u];
-                        h_cnt[slot[u] * kVT + t] = hc[u] + cnt[u];
+                            for (int w2 = 0; w2 < u; ++w2) {
+                                const bool m = slot[u] != NHi && slot[u] == slot[w2];
+                                val[w2] += m ? val[u] : 0.0;
+                                cnt[w2] += m ? cnt[u] : 0u;
+                                slot[u] = m ? NHi : slot[u];
+                                val[u] = m ? 0.0 : val[u];
+                                cnt[u] = m ? 0u : cnt[u];
+                            }
+                        double hs[4];
+                        unsigned hc[4];
+#pragma unroll
+                        for (int u = 0; u < 4; ++u) {
+                            hs[u] = h_sum[slot[u] * kVT + t];
+                            hc[u] = h_cnt[slot[u] * kVT + t];
+                        }
+#pragma unroll
+                        for (int u = 0; u < 4; ++u) {
+                            h_sum[slot[u] * kVT + t] = hs[u] + val[u];
+                            h_cnt[slot[u] * kVT + t] = hc[u] + cnt[u];
+                        }
                     }
-                }
+                };
+                if (uniform)
+                    run(std::integral_constant<bool, true>{});
+                else
+                    run(std::integral_constant<bool, false>{});
                 continue;
             }
         }
