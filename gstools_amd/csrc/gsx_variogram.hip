@@ -136,7 +136,9 @@ __global__ __launch_bounds__(kVT) void k_vario_pairs(VarioArgs a, const double* 
     // magnitudes (an FP32 distance then guesses the bin to within one); the FP32
     // guess also needs the last edge below 1e18 (larger squared distances overflow)
     const double emax = fmax(fabs(e0), fabs(s_edges[Bi > 0 ? Bi : 0]));
-    bool uniform = mono && Bi > 0 && w > 0.0 && emax < 1.0e18 && w >= 4.0e-6 * emax;
+    // (w >= 1e-15: squared distances that underflow FP32 lie within 1e-19 of 0,
+    // far below one bin)
+    bool uniform = mono && Bi > 0 && w >= 1.0e-15 && emax < 1.0e18 && w >= 4.0e-6 * emax;
     for (int i = 0; uniform && i <= Bi; ++i) uniform = fabs(s_edges[i] - (e0 + i * w)) <= 0.25 * w;
     const double inv_w = uniform ? 1.0 / w : 0.0;
     const float e0f = (float)e0, inv_wf = (float)inv_w;
