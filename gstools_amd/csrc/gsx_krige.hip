@@ -44,7 +44,8 @@ constexpr int kKrK = 16;      // reduction depth per LDS stage
 constexpr int kKrApad = kKrK + 1;
 constexpr int kKrBpad = kKrPts + 2;
 
-__global__ __launch_bounds__(256) void k_krige_gemm(int64_t K, int64_t P, const double* __restrict__ M,
+__global__ __launch_bounds__(256) void k_krige_gemm(int64_t K, int64_t P, int64_t row_tiles, int64_t col_tiles,
+                                                    const double* __restrict__ M,
                                                     const double* __restrict__ V,
                                                     const double* __restrict__ cond,
                                                     double* __restrict__ part_f, double* __restrict__ part_e) {
@@ -56,8 +57,22 @@ __global__ __launch_bounds__(256) void k_krige_gemm(int64_t K, int64_t P, const 
     const int lane = t & 63, wave = t >> 6;
     const int wr = wave >> 1, wc = wave & 1;
     const int g = lane >> 4, l16 = lane & 15;
-    const int64_t i0 = (int64_t)blockIdx.y * kKrRows;
-    const int64_t p0 = (int64_t)blockIdx.x * kKrPts;
+    // XCD-aware order: dispatch puts block b on XCD b % 8; the blocks of one XCD
+    // take all row tiles of a column tile in turn, so that column's slice of V
+    // (K x 128 doubles) is reused from that XCD's L2
+    int64_t rt_idx, ct_idx;
+    {
+        // the grid has R * C8 blocks, C8 = col_tiles rounded up to a multiple of
+        // 8: every XCD owns the column tiles ct == xcd (mod 8) with their R row
+        // tiles (a bijection onto row tiles x [0, C8)); padding blocks leave
+        const int64_t R = row_tiles;
+        const int64_t b = blockIdx.x, xcd = b & 7, slot = b >> 3;
+        rt_idx = slot % R;
+        ct_idx = (slot / R) * 8 + xcd;
+    }
+    if (ct_idx >= col_tiles) return;  // whole block: before any barrier
+    const int64_t i0 = rt_idx * kKrRows;
+    const int64_t p0 = ct_idx * kKrPts;
 
     // staging assignment: A 64 x 16 (4 per thread), B 16 x 128 (8 per thread)
     const int ar = t >> 2, ac = (t & 3) * 4;
@@ -147,8 +162,8 @@ __global__ __launch_bounds__(256) void k_krige_gemm(int64_t K, int64_t P, const 
     if (t < kKrPts) {
         const int64_t p = p0 + t;
         if (p < P) {
-            part_f[(int64_t)blockIdx.y * P + p] = red[0][0][t] + red[1][0][t];
-            part_e[(int64_t)blockIdx.y * P + p] = red[0][1][t] + red[1][1][t];
+            part_f[rt_idx * P + p] = red[0][0][t] + red[1][0][t];
+            part_e[rt_idx * P + p] = red[0][1][t] + red[1][1][t];
         }
     }
 }
@@ -284,10 +299,11 @@ int gsx_krige_f64(int64_t K, int64_t P, const double* krig_mat, const double* kr
     }
     const int64_t tiles = gsx::krige_row_tiles(K);
     const int64_t col_blocks = gsx::ceil_div(P, (int64_t)gsx::kKrPts);
-    if (tiles > 65535 || col_blocks > 0x7fffffffLL) return GSX_E_INVALID;
+    if (tiles * (col_blocks + 8) > 0x7fffffffLL) return GSX_E_INVALID;
     double* part_f = ws;
     double* part_e = ws + tiles * P;
-    hipLaunchKernelGGL(gsx::k_krige_gemm, dim3((unsigned)col_blocks, (unsigned)tiles), dim3(256), 0, s, K, P,
+    const int64_t col8 = gsx::ceil_div(col_blocks, (int64_t)8) * 8;
+    hipLaunchKernelGGL(gsx::k_krige_gemm, dim3((unsigned)(col8 * tiles)), dim3(256), 0, s, K, P, tiles, col_blocks,
                        krig_mat, krig_vecs, cond, part_f, part_e);
     int st = gsx::launch_status();
     if (st != GSX_OK) return st;
