@@ -34,6 +34,7 @@
 namespace gsx {
 
 constexpr int kVT = 256;        // points per tile (threads per block)
+constexpr int kVPad = 4;        // zero doubles after the staged field rows
 constexpr int kVFields = 8;     // fields per launch
 constexpr int kVMaxDim = 3;
 constexpr int kVBlocks = 2048;  // persistent blocks
@@ -107,11 +108,17 @@ __global__ __launch_bounds__(kVT) void k_vario_pairs(VarioArgs a, const double* 
     const int64_t NH = a.h1 - a.h0;
     const int t = threadIdx.x;
     double* s_edges = reinterpret_cast<double*>(vlds);
-    double* s_thr = s_edges + (a.B + 1);  // squared-distance thresholds of the edges
-    double* s_dirs = s_thr + (a.B + 1);
+    // squared-distance thresholds of the edges, padded: s_thp[0] = -inf,
+    // s_thp[i + 1] = T(e_i), s_thp[B + 2] = +inf, so that bins -1..B read
+    // their two bounding thresholds at one address with no clamping
+    double* s_thp = s_edges + (a.B + 1);
+    double* s_thr = s_thp + 1;
+    double* s_dirs = s_thp + (a.B + 3);
+    // staged tile: coordinates then field values, rows of kVT, then kVPad zeros;
+    // a read up to 3 entries past a row lands in the next row or in the pad
     double* s_pos = s_dirs + a.D * DIM;                     // [DIM][kVT]
     double* s_f = s_pos + DIM * kVT;                        // [F][kVT]
-    double* h_sum = s_f + a.F * kVT;                        // [NH + 1][kVT], row NH scratch
+    double* h_sum = s_f + a.F * kVT + kVPad;                // [NH + 1][kVT], row NH scratch
     unsigned* h_cnt = reinterpret_cast<unsigned*>(h_sum + (NH + 1) * kVT);  // [NH + 1][kVT]
 
     for (int64_t i = t; i <= a.B; i += kVT) {
@@ -119,8 +126,13 @@ __global__ __launch_bounds__(kVT) void k_vario_pairs(VarioArgs a, const double* 
         s_edges[i] = e;
         s_thr[i] = sq_threshold(e);
     }
+    if (t == 0) {
+        s_thp[0] = -__builtin_inf();
+        s_thp[a.B + 2] = __builtin_inf();
+    }
     if (a.directional)
         for (int64_t i = t; i < a.D * DIM; i += kVT) s_dirs[i] = dirs[i];
+    if (t < kVPad) s_f[a.F * kVT + t] = 0.0;
     for (int64_t h = 0; h <= NH; ++h) {
         h_sum[h * kVT + t] = 0.0;
         h_cnt[h * kVT + t] = 0u;
@@ -141,8 +153,8 @@ __global__ __launch_bounds__(kVT) void k_vario_pairs(VarioArgs a, const double* 
     bool uniform = mono && Bi > 0 && w >= 1.0e-15 && emax < 1.0e18 && w >= 4.0e-6 * emax;
     for (int i = 0; uniform && i <= Bi; ++i) uniform = fabs(s_edges[i] - (e0 + i * w)) <= 0.25 * w;
     const double inv_w = uniform ? 1.0 / w : 0.0;
-    const float e0f = (float)e0, inv_wf = (float)inv_w;
-    (void)inv_w;
+    // guess q = d / w - e0 / w in one FP32 fma
+    const float inv_wf = (float)inv_w, qoff = (float)(-e0 * inv_w);
     // directions to test: all of those whose slots this launch records, and
     // with separate_dirs also the earlier ones (they may take the pair)
     const int64_t d_last = a.directional ? (a.h1 - 1) / a.B : 0;
@@ -157,15 +169,30 @@ __global__ __launch_bounds__(kVT) void k_vario_pairs(VarioArgs a, const double* 
         const int64_t k0 = tk * kVT;
         const int64_t kn = (a.P - k0 < kVT) ? a.P - k0 : kVT;
         __syncthreads();  // previous tile's readers are done
+        // a non-finite coordinate (inf - inf is a NaN distance) or first-field value
+        // among the staged points
+        bool stage_nan = false;
 #pragma unroll
-        for (int c = 0; c < DIM; ++c) s_pos[c * kVT + t] = (t < kn) ? pos[c * a.P + k0 + t] : 0.0;
-        for (int64_t m = 0; m < a.F; ++m) s_f[m * kVT + t] = (t < kn) ? f[m * a.P + k0 + t] : 0.0;
-        __syncthreads();
+        for (int c = 0; c < DIM; ++c) {
+            const double v = (t < kn) ? pos[c * a.P + k0 + t] : 0.0;
+            stage_nan = stage_nan || !__builtin_isfinite(v);
+            s_pos[c * kVT + t] = v;
+        }
+        for (int64_t m = 0; m < a.F; ++m) {
+            const double v = (t < kn) ? f[m * a.P + k0 + t] : 0.0;
+            stage_nan = stage_nan || (m == 0 && !__builtin_isfinite(v));
+            s_f[m * kVT + t] = v;
+        }
+        const bool tile_nan = __syncthreads_or(stage_nan ? 1 : 0) != 0;
         const int64_t j = tj * kVT + t;
         if (j >= a.P) continue;
         double xj[DIM];
+        bool xj_nan = false;
 #pragma unroll
-        for (int c = 0; c < DIM; ++c) xj[c] = pos[c * a.P + j];
+        for (int c = 0; c < DIM; ++c) {
+            xj[c] = pos[c * a.P + j];
+            xj_nan = xj_nan || !__builtin_isfinite(xj[c]);
+        }
         double fj[kVFields];
 #pragma unroll
         for (int m = 0; m < kVFields; ++m) fj[m] = (m < a.F) ? f[m * a.P + j] : 0.0;
@@ -175,63 +202,71 @@ __global__ __launch_bounds__(kVT) void k_vario_pairs(VarioArgs a, const double* 
                 const double fj0 = fj[0];
                 const bool fj_ok = (a.F > 0) && (fj0 == fj0);
                 const int kend = (int)kn;
-                // four pairs per step: their bin searches overlap, and after
-                // merging equal slots the four histogram updates are independent
-                auto run = [&](auto uni) {
+                // Four pairs per step.  Every pair updates one histogram slot; pairs
+                // that are not counted (out of range, outside this launch's slots,
+                // NaN) go to the scratch row NH, so values and counts need no
+                // selects.  The four slots are read together; a later pair in the
+                // same slot as an earlier one adds onto that pair's new value
+                // (the latest earlier match), and the writes go out in pair order,
+                // so the last write to a slot holds every pair's contribution.
+                // CLEAN: every coordinate and field value of this wave's points and
+                // of the staged tile is finite, so no pair has a NaN distance or a
+                // NaN field value.
+                auto run = [&](auto uni, auto clean) {
                     constexpr bool UNI = decltype(uni)::value;
+                    constexpr bool CLEAN = decltype(clean)::value;
                     for (int k4 = kstart; k4 < kend; k4 += 4) {
                         int slot[4];
                         double val[4];
-                        unsigned cnt[4];
 #pragma unroll
                         for (int u = 0; u < 4; ++u) {
+                            // past kend (at most 3 entries) the reads see staged values,
+                            // zeros or the pad, all finite when CLEAN; such pairs go to
+                            // the scratch row
                             const int kk = k4 + u;
                             const bool live = kk < kend;
-                            const int kc = live ? kk : kstart;
                             double d2 = 0.0;
 #pragma unroll
                             for (int c = 0; c < DIM; ++c) {
-                                const double dd = s_pos[c * kVT + kc] - xj[c];
+                                const double dd = s_pos[c * kVT + kk] - xj[c];
                                 d2 += dd * dd;
                             }
-                            const double fk = s_f[kc];
-                            const bool ok = live && fj_ok && (fk == fk);
+                            const double fk = s_f[kk];
                             const double x = fk - fj0;
-                            const double v = ok ? (CRESSIE ? sqrt(fabs(x)) : x * x) : 0.0;
-                            const bool dnan = d2 != d2;  // passes the reference's comparisons for every bin
-                            nan_sum += (dnan && ok) ? v : 0.0;
-                            nan_cnt += (dnan && ok) ? 1u : 0u;
+                            const double v = CRESSIE ? sqrt(fabs(x)) : x * x;
+                            bool ok = live;
+                            if constexpr (!CLEAN) {
+                                ok = ok && fj_ok && (fk == fk);
+                                const bool dnan = d2 != d2;  // in every bin by the reference's comparisons
+                                nan_sum += (dnan && ok) ? v : 0.0;
+                                nan_cnt += (dnan && ok) ? 1u : 0u;
+                                ok = ok && !dnan;
+                            }
                             int g;
                             if constexpr (UNI) {
-                                // guess from an FP32 distance: off by at most one bin for
-                                // the edges `uniform` admits; one step each way is exact
-                                const float q = (__builtin_amdgcn_sqrtf((float)d2) - e0f) * inv_wf;
-                                g = (q >= 0.0f) ? ((q < (float)Bi) ? (int)q : Bi) : -1;
-                                const double tdown = s_thr[g < 0 ? 0 : g];
-                                g = (g >= 0 && tdown > d2) ? g - 1 : g;
-                                const double tup = s_thr[g + 1 > Bi ? Bi : g + 1];
-                                g = (g < Bi && tup <= d2) ? g + 1 : g;
+                                // guess from an FP32 distance, within 0.1 bin of (d - e0) / w
+                                // for the edges `uniform` admits (every edge within w/4 of
+                                // uniform), so the truncated guess is within one bin of the
+                                // true one and one threshold step is exact; thresholds are
+                                // non-decreasing, so at most one of dec / inc holds
+                                float q = __builtin_fmaf(__builtin_amdgcn_sqrtf((float)d2), inv_wf, qoff);
+                                if constexpr (!CLEAN) q = (d2 == d2) ? q : -1.0f;
+                                q = __builtin_amdgcn_fmed3f(q, -1.0f, (float)Bi);
+                                const int g0 = (int)q;  // -1 .. B
+                                const double tdown = s_thp[g0 + 1], tup = s_thp[g0 + 2];  // T(e_g0), T(e_g0+1)
+                                g = g0 - (tdown > d2 ? 1 : 0) + (tup <= d2 ? 1 : 0);
                             } else {
                                 g = (int)upper_edge(s_thr, a.B, d2);
                             }
-                            const int sl = g - h0i;
-                            const bool in = ok && !dnan && g >= 0 && g < Bi && (unsigned)sl < (unsigned)NHi;
-                            slot[u] = in ? sl : NHi;  // row NH: a scratch row
-                            val[u] = in ? v : 0.0;
-                            cnt[u] = in ? 1u : 0u;
+                            // g is made opaque so the search stays unconditional: otherwise
+                            // the compiler moves it, loads included, under a branch on `ok`
+                            // and the four pairs' threshold reads no longer overlap
+                            asm volatile("" : "+v"(g));
+                            // launch slots lie in [0, B): one unsigned test covers the bin range
+                            const unsigned sl = (unsigned)(g - h0i);
+                            slot[u] = (ok && sl < (unsigned)NHi) ? (int)sl : NHi;
+                            val[u] = v;
                         }
-                        // merge equal slots into the first occurrence (fixed order, selects only)
-#pragma unroll
-                        for (int u = 1; u < 4; ++u)
-#pragma unroll
-                            for (int w2 = 0; w2 < u; ++w2) {
-                                const bool m = slot[u] != NHi && slot[u] == slot[w2];
-                                val[w2] += m ? val[u] : 0.0;
-                                cnt[w2] += m ? cnt[u] : 0u;
-                                slot[u] = m ? NHi : slot[u];
-                                val[u] = m ? 0.0 : val[u];
-                                cnt[u] = m ? 0u : cnt[u];
-                            }
                         double hs[4];
                         unsigned hc[4];
 #pragma unroll
@@ -240,16 +275,32 @@ __global__ __launch_bounds__(kVT) void k_vario_pairs(VarioArgs a, const double* 
                             hc[u] = h_cnt[slot[u] * kVT + t];
                         }
 #pragma unroll
+                        for (int u = 1; u < 4; ++u)
+#pragma unroll
+                            for (int w2 = 0; w2 < u; ++w2) {
+                                const bool m = slot[u] == slot[w2];
+                                hs[u] = m ? hs[w2] + val[w2] : hs[u];
+                                hc[u] = m ? hc[w2] + 1u : hc[u];
+                            }
+#pragma unroll
                         for (int u = 0; u < 4; ++u) {
                             h_sum[slot[u] * kVT + t] = hs[u] + val[u];
-                            h_cnt[slot[u] * kVT + t] = hc[u] + cnt[u];
+                            h_cnt[slot[u] * kVT + t] = hc[u] + 1u;
                         }
                     }
                 };
-                if (uniform)
-                    run(std::integral_constant<bool, true>{});
-                else
-                    run(std::integral_constant<bool, false>{});
+                const bool clean = !tile_nan && !__any((xj_nan || !__builtin_isfinite(fj0) || a.F < 1) ? 1 : 0);
+                if (uniform) {
+                    if (clean)
+                        run(std::integral_constant<bool, true>{}, std::integral_constant<bool, true>{});
+                    else
+                        run(std::integral_constant<bool, true>{}, std::integral_constant<bool, false>{});
+                } else {
+                    if (clean)
+                        run(std::integral_constant<bool, false>{}, std::integral_constant<bool, true>{});
+                    else
+                        run(std::integral_constant<bool, false>{}, std::integral_constant<bool, false>{});
+                }
                 continue;
             }
         }
@@ -426,7 +477,7 @@ __global__ void k_vario_lag_collect(int64_t n0, int64_t tiles, const double* __r
 
 // LDS bytes of a pair launch with NH histogram slots (+ one scratch row)
 inline size_t vario_lds_bytes(int64_t B, int64_t D, int dim, int64_t F, int64_t NH) {
-    return sizeof(double) * (2 * (size_t)(B + 1) + (size_t)(D * dim) + (size_t)dim * kVT + (size_t)F * kVT +
+    return sizeof(double) * (2 * (size_t)(B + 1) + 2 + (size_t)(D * dim) + (size_t)dim * kVT + (size_t)F * kVT + kVPad +
                              (size_t)(NH + 1) * kVT) +
            sizeof(unsigned) * (size_t)(NH + 1) * kVT;
 }

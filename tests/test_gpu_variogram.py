@@ -63,6 +63,24 @@ def test_edges_nan_positions_and_many_bins():
         _close(g, g_ref)
 
 
+def test_infinite_positions_and_values():
+    """Infinite coordinates and field values: inf - inf is a NaN distance (every bin), one infinite
+    coordinate an infinite distance (no bin); an infinite field value makes its bins' sums inf or NaN."""
+    _say("vario infinite positions / values")
+    rs = np.random.RandomState(21)
+    pos = rs.uniform(0, 10, size=(3, 900))
+    pos[1, 7] = np.inf
+    pos[1, 300] = np.inf  # with point 7: a NaN distance
+    pos[2, 650] = -np.inf
+    f = rs.normal(size=900)
+    for fv in (f, np.where(np.arange(900) == 500, np.inf, f)):
+        for edges in (np.linspace(0, 12, 15), np.array([0.0, 1.0, 4.0, 4.5, 11.0])):
+            g, c = v.unstructured(fv, edges, pos)
+            g_ref, c_ref = oracle.vario_unstructured_c(fv, edges, pos)
+            assert np.array_equal(c, c_ref)
+            np.testing.assert_allclose(g, g_ref, rtol=1e-12, atol=1e-300, equal_nan=True)
+
+
 def test_edges_on_pair_distances():
     """Edges placed exactly on pair distances (the reference's rounding): those pairs go to the upper bin."""
     _say("vario edges on pair distances")
