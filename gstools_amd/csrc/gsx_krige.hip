@@ -42,7 +42,9 @@ constexpr int kKrRows = 64;   // W rows per block
 constexpr int kKrPts = 128;   // points per block
 constexpr int kKrK = 16;      // reduction depth per LDS stage
 constexpr int kKrApad = kKrK + 1;
-constexpr int kKrBpad = kKrPts + 2;
+// B rows padded by 16 doubles: the operand reads of K groups g and g + 1
+// (rows 144 doubles apart) fall in disjoint LDS banks
+constexpr int kKrBpad = kKrPts + 16;
 
 __global__ __launch_bounds__(256) void k_krige_gemm(int64_t K, int64_t P, int64_t row_tiles, int64_t col_tiles,
                                                     const double* __restrict__ M,
@@ -74,29 +76,32 @@ __global__ __launch_bounds__(256) void k_krige_gemm(int64_t K, int64_t P, int64_
     const int64_t i0 = rt_idx * kKrRows;
     const int64_t p0 = ct_idx * kKrPts;
 
-    // staging assignment: A 64 x 16 (4 per thread), B 16 x 128 (8 per thread)
-    const int ar = t >> 2, ac = (t & 3) * 4;
-    const int br = t >> 4, bc = (t & 15) * 8;
-    double pa[4], pb[8];
+    // staging assignment: A 64 x kKrK (kKrK / 4 consecutive per thread); B kKrK x 128
+    // (kKrK / 2 per thread: point t % 128 of rows t / 128 + 2u, so a wave's loads
+    // and LDS stores are unit-stride and the stores conflict-free)
+    constexpr int kAU = kKrK / 4, kBU = kKrK / 2;
+    const int ar = t >> 2, ac = (t & 3) * kAU;
+    const int br = t >> 7, bc = t & 127;
+    double pa[kAU], pb[kBU];
     auto load = [&](int64_t j0) {
         const int64_t row = i0 + ar;
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
+        for (int u = 0; u < kAU; ++u) {
             const int64_t col = j0 + ac + u;
             pa[u] = (row < K && col < K) ? M[row * K + col] : 0.0;
         }
-        const int64_t j = j0 + br;
+        const int64_t p = p0 + bc;
 #pragma unroll
-        for (int u = 0; u < 8; ++u) {
-            const int64_t p = p0 + bc + u;
+        for (int u = 0; u < kBU; ++u) {
+            const int64_t j = j0 + br + 2 * u;
             pb[u] = (j < K && p < P) ? V[j * P + p] : 0.0;
         }
     };
     auto store = [&]() {
 #pragma unroll
-        for (int u = 0; u < 4; ++u) As[ar * kKrApad + ac + u] = pa[u];
+        for (int u = 0; u < kAU; ++u) As[ar * kKrApad + ac + u] = pa[u];
 #pragma unroll
-        for (int u = 0; u < 8; ++u) Bs[br * kKrBpad + bc + u] = pb[u];
+        for (int u = 0; u < kBU; ++u) Bs[(br + 2 * u) * kKrBpad + bc] = pb[u];
     };
 
     f64x4 acc[2][4];
