@@ -354,10 +354,11 @@ __global__ __launch_bounds__(kMWaves * 64, 2) void k_mfma_grid(MfmaGeom g, doubl
 // Each K step stages, through double-buffered LDS: the A tile (hi and lo,
 // 128 rows x 16 element pairs) and the E1 / E2 slices of the block's 16 i1
 // and 16 i2 values (16 modes each), all loaded one step ahead into
-// registers by the whole block.  Layouts are XOR-swizzled so that the 16
-// lanes reading 16 rows hit 16 distinct 16-byte bank groups:
-//   A row r (64 B = 4 chunks):   chunk c stored at c ^ ((r >> 2) & 3)
-//   E row r (128 B = 8 chunks):  chunk c stored at c ^ ((r >> 1) & 7)
+// registers by the whole block.  Layouts are XOR-swizzled so that the 8
+// lanes of one 16-byte LDS access phase (32 banks x 4 B = 8 bank groups of
+// 16 B) hit 8 distinct groups, for the reads (8 rows) and the stores:
+//   A row r (64 B = 4 chunks, two rows per 128 B):  chunk c stored at c ^ ((r >> 1) & 3)
+//   E row r (128 B = 8 chunks):                      chunk c stored at c ^ (r & 7)
 struct alignas(16) Stage3 {
     uint32_t ah[kMBM * 16];
     uint32_t al[kMBM * 16];
@@ -420,12 +421,12 @@ __global__ __launch_bounds__(kMWaves * 64, 2) void k_mfma_grid3(MfmaGeom g, doub
         for (int h = 0; h < 2; ++h) {
             const int c = tid + h * 256;
             const int r = c >> 2;
-            const int pos = r * 16 + (((c & 3) ^ ((r >> 2) & 3)) * 4);
+            const int pos = r * 16 + (((c & 3) ^ ((r >> 1) & 3)) * 4);
             *reinterpret_cast<u32x4*>(&st[buf].ah[pos]) = pa[h][0];
             *reinterpret_cast<u32x4*>(&st[buf].al[pos]) = pa[h][1];
         }
         float* es = is_e1 ? st[buf].e1 : st[buf].e2;
-        *reinterpret_cast<float4*>(es + erow * 32 + ((ech ^ ((erow >> 1) & 7)) * 4)) = pe;
+        *reinterpret_cast<float4*>(es + erow * 32 + ((ech ^ (erow & 7)) * 4)) = pe;
     };
 
     f32x4 acc[kMRF][kMCF];
@@ -438,7 +439,7 @@ __global__ __launch_bounds__(kMWaves * 64, 2) void k_mfma_grid3(MfmaGeom g, doub
     load(0);
     store(0);
     __syncthreads();
-    const int zsw = (l16 >> 1) & 7;
+    const int zsw = l16 & 7;
     for (int64_t s = 0; s < steps; ++s) {
         const int buf = (int)(s & 1);
         if (s + 1 < steps) load(s + 1);
@@ -458,7 +459,7 @@ __global__ __launch_bounds__(kMWaves * 64, 2) void k_mfma_grid3(MfmaGeom g, doub
 #pragma unroll
         for (int cf = 0; cf < kMCF; ++cf) {
             const int er = wave * kMCF + cf;
-            const int ysw = (er >> 1) & 7;
+            const int ysw = er & 7;
             const float* yr = st[buf].e1 + er * 32;
             const float4 q0 = *reinterpret_cast<const float4*>(yr + (((2 * grp) ^ ysw) * 4));
             const float4 q1 = *reinterpret_cast<const float4*>(yr + (((2 * grp + 1) ^ ysw) * 4));
@@ -473,7 +474,7 @@ __global__ __launch_bounds__(kMWaves * 64, 2) void k_mfma_grid3(MfmaGeom g, doub
             bl[cf] = u32x4{l4[0], l4[1], l4[2], l4[3]};
         }
 
-        const int asw = (l16 >> 2) & 3;
+        const int asw = (l16 >> 1) & 3;
 #pragma unroll
         for (int rf = 0; rf < kMRF; ++rf) {
             const int pos = (rf * 16 + l16) * 16 + ((grp ^ asw) * 4);
