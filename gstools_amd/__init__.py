@@ -29,10 +29,11 @@ Public surface:
   lockstep, identical to one ``RandMeth`` each).
 """
 
-from . import krige, variogram  # noqa: F401
+from . import geometry, krige, variogram  # noqa: F401
 from .condsrf import CondSRF  # noqa: F401
 from .covmodel import CovModel, Exponential, Gaussian, Matern, Spherical, Stable  # noqa: F401
 from .generator import Fourier, Generator, IncomprRandMeth, RandMeth  # noqa: F401
+from .geometry import DEGREE_SCALE, EARTH_RADIUS, KM_SCALE, RADIAN_SCALE  # noqa: F401
 from .rng import RNG, MasterRNG  # noqa: F401
 from .sampling import sample_ln_pdf_batch, sample_modes_batch  # noqa: F401
 from .srf import SRF  # noqa: F401

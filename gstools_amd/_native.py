@@ -78,6 +78,7 @@ EXPORTS = {
     "gsx_krige_f64": (_i32, [_i64, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _i64, _vp]),
     "gsx_vario_workspace": (_i64, [_i64, _i64, _i64, _i64, _i64]),
     "gsx_vario_unstructured_f64": (_i32, [_i64, _i64, _i64, _i64, _vp, _vp, _vp, _i32, _vp, _vp, _vp, _i64, _vp]),
+    "gsx_vario_unstructured_haversine_f64": (_i32, [_i64, _i64, _i64, _vp, _vp, _vp, _i32, _vp, _vp, _vp, _i64, _vp]),
     "gsx_vario_directional_f64": (_i32, [_i64, _i64, _i64, _i64, _i64, _vp, _vp, _vp, _vp, _f64, _f64, _i32, _i32,
                                          _vp, _vp, _vp, _i64, _vp]),
     "gsx_vario_structured_f64": (_i32, [_i64, _i64, _vp, _vp, _i32, _vp, _vp, _vp, _i64, _vp]),

@@ -49,6 +49,23 @@ struct VarioArgs {
     int64_t nt;      // tiles per side
 };
 
+constexpr double kDeg2Rad = 3.14159265358979323846 / 180.0;
+
+// Great-circle angle (radians) of two (lat, lon) points in degrees, the haversine
+// form of the backend's distance_type 'h' (selected at
+// src/gstools/variogram/variogram.py:446 for latlon=True), in the same operation
+// order as the oracle; coslat_* = cos(lat_* * pi / 180), taken once per point
+// (the same bits as per pair).  Point j precedes point k.
+__device__ __forceinline__ double haversine(double lat_j, double lon_j, double coslat_j, double lat_k, double lon_k,
+                                            double coslat_k) {
+#pragma clang fp contract(off)
+    const double diff_lat = (lat_k - lat_j) * kDeg2Rad;
+    const double diff_lon = (lon_k - lon_j) * kDeg2Rad;
+    const double sl = sin(diff_lat / 2.0), so = sin(diff_lon / 2.0);
+    const double arg = sl * sl + coslat_j * coslat_k * (so * so);
+    return 2.0 * atan2(sqrt(arg), sqrt(1.0 - arg));
+}
+
 __device__ __forceinline__ double vario_est(int cressie, double x) { return cressie ? sqrt(fabs(x)) : x * x; }
 
 // Largest b with e[b] <= x for non-decreasing e[0..B]; -1 if x < e[0].
@@ -96,7 +113,9 @@ __device__ __forceinline__ int bin_from_guess(const double* e, int B, double x, 
 // straight-line pair loop (the general loop covers everything else).
 // CRESSIE selects the estimator of the FAST loop at compile time (a runtime flag
 // would be if-converted into computing both estimators for every pair).
-template <int DIM, bool FAST, bool CRESSIE>
+// HAV: (lat, lon) points in degrees, great-circle distances (general loop, DIM 2,
+// a third staged row holds cos(lat)).
+template <int DIM, bool FAST, bool CRESSIE, bool HAV = false>
 __global__ __launch_bounds__(kVT) void k_vario_pairs(VarioArgs a, const double* __restrict__ f,
                                                      const double* __restrict__ edges,
                                                      const double* __restrict__ pos,
@@ -114,10 +133,12 @@ __global__ __launch_bounds__(kVT) void k_vario_pairs(VarioArgs a, const double* 
     double* s_thp = s_edges + (a.B + 1);
     double* s_thr = s_thp + 1;
     double* s_dirs = s_thp + (a.B + 3);
+    static_assert(!HAV || (DIM == 2 && !FAST), "haversine distances: 2D, general loop");
+    constexpr int NPOS = HAV ? DIM + 1 : DIM;  // staged coordinate rows
     // staged tile: coordinates then field values, rows of kVT, then kVPad zeros;
     // a read up to 3 entries past a row lands in the next row or in the pad
-    double* s_pos = s_dirs + a.D * DIM;                     // [DIM][kVT]
-    double* s_f = s_pos + DIM * kVT;                        // [F][kVT]
+    double* s_pos = s_dirs + a.D * DIM;                     // [NPOS][kVT]
+    double* s_f = s_pos + NPOS * kVT;                       // [F][kVT]
     double* h_sum = s_f + a.F * kVT + kVPad;                // [NH + 1][kVT], row NH scratch
     unsigned* h_cnt = reinterpret_cast<unsigned*>(h_sum + (NH + 1) * kVT);  // [NH + 1][kVT]
 
@@ -178,6 +199,7 @@ __global__ __launch_bounds__(kVT) void k_vario_pairs(VarioArgs a, const double* 
             stage_nan = stage_nan || !__builtin_isfinite(v);
             s_pos[c * kVT + t] = v;
         }
+        if constexpr (HAV) s_pos[DIM * kVT + t] = cos(s_pos[t] * kDeg2Rad);
         for (int64_t m = 0; m < a.F; ++m) {
             const double v = (t < kn) ? f[m * a.P + k0 + t] : 0.0;
             stage_nan = stage_nan || (m == 0 && !__builtin_isfinite(v));
@@ -304,15 +326,23 @@ __global__ __launch_bounds__(kVT) void k_vario_pairs(VarioArgs a, const double* 
                 continue;
             }
         }
+        const double coslat_j = HAV ? cos(xj[0] * kDeg2Rad) : 0.0;
         for (int kk = kstart; kk < kn; ++kk) {
-            double d2 = 0.0;
             double dv[DIM];
+            double dist;
+            if constexpr (HAV) {
+                dist = haversine(xj[0], xj[1], coslat_j, s_pos[kk], s_pos[kVT + kk], s_pos[2 * kVT + kk]);
 #pragma unroll
-            for (int c = 0; c < DIM; ++c) {
-                dv[c] = s_pos[c * kVT + kk] - xj[c];  // x_k - x_j
-                d2 += dv[c] * dv[c];
+                for (int c = 0; c < DIM; ++c) dv[c] = 0.0;  // (directions are not allowed for lat-lon)
+            } else {
+                double d2 = 0.0;
+#pragma unroll
+                for (int c = 0; c < DIM; ++c) {
+                    dv[c] = s_pos[c * kVT + kk] - xj[c];  // x_k - x_j
+                    d2 += dv[c] * dv[c];
+                }
+                dist = sqrt(d2);
             }
-            const double dist = sqrt(d2);
             int64_t b_lo, b_hi;  // candidate bins [b_lo, b_hi)
             if (mono && dist == dist) {
                 const int64_t b = upper_edge(s_edges, a.B, dist);
@@ -503,16 +533,16 @@ inline int64_t vario_slots_per_launch(size_t limit, int64_t B, int64_t D, int di
     return (int64_t)((limit - fixed) / ((sizeof(double) + sizeof(unsigned)) * kVT));
 }
 
-template <int DIM, bool FAST, bool CRESSIE>
+template <int DIM, bool FAST, bool CRESSIE, bool HAV = false>
 int launch_pairs_t(const VarioArgs& a, size_t lds, int64_t blocks, hipStream_t s, const double* f,
                    const double* edges, const double* pos, const double* dirs, double* part_sum,
                    unsigned long long* part_cnt) {
     if (lds > 65536 &&
-        hipFuncSetAttribute(reinterpret_cast<const void*>(&k_vario_pairs<DIM, FAST, CRESSIE>),
+        hipFuncSetAttribute(reinterpret_cast<const void*>(&k_vario_pairs<DIM, FAST, CRESSIE, HAV>),
                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) != hipSuccess)
         return GSX_E_LAUNCH;
-    hipLaunchKernelGGL((k_vario_pairs<DIM, FAST, CRESSIE>), dim3((unsigned)blocks), dim3(kVT), lds, s, a, f, edges,
-                       pos, dirs, part_sum, part_cnt);
+    hipLaunchKernelGGL((k_vario_pairs<DIM, FAST, CRESSIE, HAV>), dim3((unsigned)blocks), dim3(kVT), lds, s, a, f,
+                       edges, pos, dirs, part_sum, part_cnt);
     return launch_status();
 }
 
@@ -537,9 +567,11 @@ inline int64_t vario_pair_blocks(int64_t P) {
 int vario_pairs(int64_t F, int64_t P, int64_t dim, int64_t B, int64_t D, const double* f, const double* edges,
                 const double* pos, const double* dirs, double angles_tol, double bandwidth, int separate,
                 int estimator, int directional, double* gamma, int64_t* counts, void* ws, int64_t ws_bytes,
-                void* stream) {
+                void* stream, bool haversine = false) {
     if (F < 0 || P < 0 || dim < 1 || dim > kVMaxDim || B < 0 || D < 1 || (estimator != 0 && estimator != 1))
         return GSX_E_INVALID;
+    if (haversine && (dim != 2 || directional)) return GSX_E_INVALID;
+    const int dim_stage = haversine ? 3 : (int)dim;  // staged coordinate rows
     const int64_t NHall = D * B;
     if (NHall == 0) return GSX_OK;
     if (B > 0x3fffffff || NHall > 0x3fffffff) return GSX_E_INVALID;
@@ -549,7 +581,7 @@ int vario_pairs(int64_t F, int64_t P, int64_t dim, int64_t B, int64_t D, const d
     int st = vario_lds_limit(&limit);
     if (st != GSX_OK) return st;
     const int64_t fc_max = F < kVFields ? (F > 0 ? F : 1) : kVFields;  // fields staged per launch
-    const int64_t per = vario_slots_per_launch(limit, B, directional ? D : 1, (int)dim, fc_max);
+    const int64_t per = vario_slots_per_launch(limit, B, directional ? D : 1, dim_stage, fc_max);
     if (per < 1) return GSX_E_UNSUPPORTED;
     const int64_t blocks = vario_pair_blocks(P);
     const int64_t chunk_slots = per < NHall ? per : NHall;
@@ -580,10 +612,13 @@ int vario_pairs(int64_t F, int64_t P, int64_t dim, int64_t B, int64_t D, const d
             a.h0 = h0;
             a.h1 = h1;
             a.nt = nt;
-            const size_t lds = vario_lds_bytes(B, a.D, (int)dim, fc_max, h1 - h0);
+            const size_t lds = vario_lds_bytes(B, a.D, dim_stage, fc_max, h1 - h0);
             if (P > 1) {
                 const double* fm = F > 0 ? f + m0 * P : nullptr;
-                if (dim == 1)
+                if (haversine)
+                    st = launch_pairs_t<2, false, false, true>(a, lds, blocks, s, fm, edges, pos, dirs, part_sum,
+                                                               part_cnt);
+                else if (dim == 1)
                     st = launch_pairs<1>(a, lds, blocks, s, fm, edges, pos, dirs, part_sum, part_cnt);
                 else if (dim == 2)
                     st = launch_pairs<2>(a, lds, blocks, s, fm, edges, pos, dirs, part_sum, part_cnt);
@@ -626,6 +661,13 @@ int gsx_vario_unstructured_f64(int64_t F, int64_t P, int64_t dim, int64_t B, con
                                int64_t workspace_bytes, void* stream) {
     return gsx::vario_pairs(F, P, dim, B, 1, f, edges, pos, nullptr, 0.0, -1.0, 0, estimator, 0, gamma, counts,
                             workspace, workspace_bytes, stream);
+}
+
+int gsx_vario_unstructured_haversine_f64(int64_t F, int64_t P, int64_t B, const double* f, const double* edges,
+                                         const double* latlon, int estimator, double* gamma, int64_t* counts,
+                                         void* workspace, int64_t workspace_bytes, void* stream) {
+    return gsx::vario_pairs(F, P, 2, B, 1, f, edges, latlon, nullptr, 0.0, -1.0, 0, estimator, 0, gamma, counts,
+                            workspace, workspace_bytes, stream, true);
 }
 
 int gsx_vario_directional_f64(int64_t F, int64_t P, int64_t dim, int64_t B, int64_t D, const double* f,

@@ -12,8 +12,17 @@ import numpy as np
 __all__ = [
     "set_angles", "set_anis", "no_of_angles", "rotation_planes", "givens_rotation",
     "matrix_rotate", "matrix_derotate", "matrix_isotropify", "matrix_isometrize",
-    "generate_grid",
+    "generate_grid", "latlon2pos", "chordal_to_great_circle",
+    "EARTH_RADIUS", "KM_SCALE", "DEGREE_SCALE", "RADIAN_SCALE",
 ]
+
+# Geographic scales (``src/gstools/tools/__init__.py:112-121``): the earth's
+# radius in km, and the factors turning great-circle angles (radians) into km,
+# degrees or radians.
+EARTH_RADIUS = 6371.0
+KM_SCALE = 6371.0
+DEGREE_SCALE = 57.29577951308232
+RADIAN_SCALE = 1.0
 
 
 def no_of_angles(dim):
@@ -86,3 +95,17 @@ def matrix_anisometrize(dim, angles, anis):
 def generate_grid(pos):
     """Meshgrid ('ij') of a structured position tuple -> (d, P) (``geometric.py:340-356``)."""
     return np.asarray(np.meshgrid(*pos, indexing="ij"), dtype=np.double).reshape((len(pos), -1))
+
+
+def latlon2pos(latlon, radius=1.0, dtype=np.double):
+    """(lat, lon) in degrees -> 3D points on a sphere of ``radius`` (``geometric.py:628-667``)."""
+    latlon = np.asarray(latlon, dtype=dtype).reshape((2, -1))
+    lat, lon = np.deg2rad(latlon[:2])
+    return np.array((radius * np.cos(lat) * np.cos(lon), radius * np.cos(lat) * np.sin(lon),
+                     radius * np.sin(lat) * np.ones_like(lon)), dtype=dtype)
+
+
+def chordal_to_great_circle(dist, radius=1.0):
+    """Chordal distance -> great-circle distance on a sphere, clipped to [0, 2 radius] (``geometric.py:712-733``)."""
+    diameter = 2 * radius
+    return diameter * np.arcsin(np.maximum(np.minimum(np.divide(dist, diameter), 1), 0))

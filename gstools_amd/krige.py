@@ -161,11 +161,12 @@ def get_drift_functions(dim, drift_type):
 
 
 def eval_func(func_val, pos, dim, broadcast=False):
-    """Mean / trend at positions (d, n): callable evaluated, constant broadcast (``tools/misc.py``)."""
+    """Mean / trend at positions (d, n) (``tools/misc.py:62-141``): a callable is evaluated,
+    ``None`` is 0, any other value is a constant (its first element, as for a scalar field)."""
     pos = np.asarray(pos, dtype=np.double).reshape(dim, -1)
     if callable(func_val):
         return np.asarray(func_val(*pos), dtype=np.double).reshape(-1) * np.ones(pos.shape[1])
-    val = 0.0 if func_val is None else float(func_val)
+    val = 0.0 if func_val is None else float(np.asarray(func_val, dtype=np.double).ravel()[0])
     return np.full(pos.shape[1], val) if broadcast else val
 
 

@@ -11,17 +11,18 @@ fixed order).
 :func:`standard_bins` restate the front end (``variogram.py:151-470``,
 ``:473-565``, ``binning.py:25-104``): stacked fields, masks and ``no_data``,
 structured and unstructured positions, sampling, directions from vectors or
-angles with tolerance and bandwidth, Matheron / Cressie estimators, counts.
-Not restated: lat-lon (great-circle) variograms, normalizers, trends and
-mean removal (``latlon=True``, ``normalizer``, ``trend``, ``mean`` or
-``fit_normalizer`` other than the defaults raise ``NotImplementedError``).
+angles with tolerance and bandwidth, Matheron / Cressie estimators, counts,
+lat-lon data (great-circle distances, ``geo_scale``), and mean / trend
+removal.  Not restated: normalizers (``normalizer`` or ``fit_normalizer``
+other than the defaults raise ``NotImplementedError``).
 """
 
 import numpy as np
 import torch
 
 from . import _native
-from .geometry import generate_grid
+from .geometry import RADIAN_SCALE, chordal_to_great_circle, generate_grid, latlon2pos
+from .krige import eval_func
 from .summate import _as_f64, _device, _ptr, _stream
 
 __all__ = [
@@ -55,7 +56,7 @@ def _tensor(x, name, dev, ndim):
 
 
 def _pairs(f, bin_edges, pos, estimator_type, direction=None, angles_tol=np.pi / 8, bandwidth=-1.0,
-           separate_dirs=False):
+           separate_dirs=False, haversine=False):
     dev = _device()
     ft = _tensor(f, "f", dev, 2)
     pt = _tensor(pos, "pos", dev, 2)
@@ -69,6 +70,16 @@ def _pairs(f, bin_edges, pos, estimator_type, direction=None, angles_tol=np.pi /
     nb = max(et.numel() - 1, 0)
     est = _estimator_code(estimator_type)
     lib = _native.lib()
+    if haversine:
+        if dim != 2 or direction is not None:
+            raise ValueError("variogram: great-circle distances need (lat, lon) positions and no direction")
+        gamma = torch.zeros(nb, dtype=torch.float64, device=dev)
+        counts = torch.zeros(nb, dtype=torch.int64, device=dev)
+        ws = _ws(lib.gsx_vario_workspace(0, npnt, 2, nb, 1), dev)
+        _native.check(lib.gsx_vario_unstructured_haversine_f64(nf, npnt, nb, _ptr(ft), _ptr(et), _ptr(pt), est,
+                                                               _ptr(gamma), _ptr(counts), _ptr(ws), ws.numel(),
+                                                               _stream()), "unstructured (haversine)")
+        return gamma, counts
     if direction is None:
         gamma = torch.zeros(nb, dtype=torch.float64, device=dev)
         counts = torch.zeros(nb, dtype=torch.int64, device=dev)
@@ -91,10 +102,14 @@ def _pairs(f, bin_edges, pos, estimator_type, direction=None, angles_tol=np.pi /
 
 
 def unstructured(f, bin_edges, pos, estimator_type="m", distance_type="e", num_threads=None):
-    """Drop-in for the backend ``unstructured`` -> (gamma (B,), counts (B,))."""
-    if distance_type != "e":
-        raise NotImplementedError("variogram: only Euclidean distances are part of this build")
-    gamma, counts = _pairs(f, bin_edges, pos, estimator_type)
+    """Drop-in for the backend ``unstructured`` -> (gamma (B,), counts (B,)).
+
+    ``distance_type`` "e": Euclidean distances; "h": ``pos`` is (lat, lon) in
+    degrees and a pair's distance its great-circle angle in radians.
+    """
+    if distance_type not in ("e", "h"):
+        raise ValueError(f"Unknown distance type: {distance_type}")
+    gamma, counts = _pairs(f, bin_edges, pos, estimator_type, haversine=distance_type == "h")
     return gamma.cpu().numpy(), counts.cpu().numpy()
 
 
@@ -210,10 +225,12 @@ def _separate_dirs_test(direction, angles_tol):
     return bool(sep)
 
 
-def standard_bins(pos=None, dim=2, latlon=False, mesh_type="unstructured", bin_no=None, max_dist=None, **kwargs):
-    """Sturges-rule bins up to a third of the bounding-box diagonal (``binning.py:29-104``)."""
-    if latlon:
-        raise NotImplementedError("standard_bins: lat-lon data is not part of this build")
+def standard_bins(pos=None, dim=2, latlon=False, mesh_type="unstructured", bin_no=None, max_dist=None,
+                  geo_scale=RADIAN_SCALE):
+    """Sturges-rule bins up to a third of the bounding-box diagonal (``binning.py:29-104``);
+    lat-lon points are taken on a sphere of radius ``geo_scale`` and the diagonal turned
+    into a great-circle distance."""
+    dim = 2 if latlon else int(dim)
     if bin_no is None or max_dist is None:
         if pos is None:
             raise ValueError("standard_bins: no pos tuple given.")
@@ -221,9 +238,11 @@ def standard_bins(pos=None, dim=2, latlon=False, mesh_type="unstructured", bin_n
             pos = generate_grid(_struct_pos_dim(pos, dim))
         else:
             pos = np.asarray(pos, dtype=np.double).reshape(dim, -1)
+        pos = latlon2pos(pos, radius=geo_scale) if latlon else pos
         pnt_cnt = len(pos[0])
         box = np.asarray([[np.min(axis), np.max(axis)] for axis in pos])
         diam = np.linalg.norm(box[:, 0] - box[:, 1])
+        diam = chordal_to_great_circle(diam, geo_scale) if latlon else diam
         bin_no = int(np.ceil(2 * np.log2(pnt_cnt) + 1)) if bin_no is None else int(bin_no)
         max_dist = diam / 3 if max_dist is None else float(max_dist)
     return np.linspace(0, max_dist, num=bin_no + 1, dtype=np.double)
@@ -233,12 +252,10 @@ def standard_bins(pos=None, dim=2, latlon=False, mesh_type="unstructured", bin_n
 def vario_estimate(pos, field, bin_edges=None, sampling_size=None, sampling_seed=None, estimator="matheron",
                    latlon=False, direction=None, angles=None, angles_tol=np.pi / 8, bandwidth=None, no_data=np.nan,
                    mask=np.ma.nomask, mesh_type="unstructured", return_counts=False, mean=None, normalizer=None,
-                   trend=None, fit_normalizer=False, **std_bins):
+                   trend=None, fit_normalizer=False, geo_scale=RADIAN_SCALE, **std_bins):
     """Empirical variogram (``variogram.py:151-470``); see the module notes for what is not restated."""
-    if latlon:
-        raise NotImplementedError("vario_estimate: lat-lon data is not part of this build")
-    if mean is not None or normalizer is not None or trend is not None or fit_normalizer:
-        raise NotImplementedError("vario_estimate: mean / normalizer / trend removal is not part of this build")
+    if normalizer is not None or fit_normalizer:
+        raise NotImplementedError("vario_estimate: normalizers are not part of this build")
     if bin_edges is not None:
         bin_edges = np.atleast_1d(np.asarray(bin_edges, dtype=np.double))
         bin_centers = (bin_edges[:-1] + bin_edges[1:]) / 2.0
@@ -257,6 +274,8 @@ def vario_estimate(pos, field, bin_edges=None, sampling_size=None, sampling_seed
         pos = generate_grid(axes)
     else:
         pos, dim = _unstruct_pos_shape(pos, field.shape)
+    if latlon and dim != 2:
+        raise ValueError("Variogram: given field needs to be 2D for lat-lon.")
     pnt_cnt = len(pos[0])
     field = field.reshape((-1, pnt_cnt))
     if masked:
@@ -280,6 +299,8 @@ def vario_estimate(pos, field, bin_edges=None, sampling_size=None, sampling_seed
         direction = ang2dir(angles=angles, dtype=np.double, dim=dim)
         dir_no = direction.shape[0]
     if dir_no > 0:
+        if latlon:
+            raise ValueError("Directional variogram not allowed for lat-lon.")
         norms = np.linalg.norm(direction, axis=1)
         if np.any(np.isclose(norms, 0)):
             raise ValueError(f"Zero length directions: {direction}")
@@ -291,13 +312,22 @@ def vario_estimate(pos, field, bin_edges=None, sampling_size=None, sampling_seed
         field = field[:, idx]
         pos = pos[:, idx]
     if bin_edges is None:
-        bin_edges = standard_bins(pos, dim, latlon, **std_bins)
+        bin_edges = standard_bins(pos, dim, latlon, geo_scale=geo_scale, **std_bins)
         bin_centers = (bin_edges[:-1] + bin_edges[1:]) / 2.0
+    if latlon:
+        # the kernel takes radians (out of place: a caller's edge array is left as it was)
+        bin_edges = bin_edges / geo_scale
+    # trend, then mean (a normalizer would act between them: variogram.py:433-440,
+    # normalizer/tools.py:162-186)
+    for i in range(field.shape[0]):
+        field[i] -= eval_func(trend, pos, dim, broadcast=True)
+        field[i] -= eval_func(mean, pos, dim, broadcast=True)
     code = "c" if estimator.lower() == "cressie" else "m"
     if estimator.lower() not in ("matheron", "cressie"):
         raise ValueError(f"Unknown variogram estimator function: {estimator}")
     if dir_no == 0:
-        estimates, counts = unstructured(field, bin_edges, pos, estimator_type=code)
+        estimates, counts = unstructured(field, bin_edges, pos, estimator_type=code,
+                                         distance_type="h" if latlon else "e")
     else:
         estimates, counts = directional(field, bin_edges, pos, direction, angles_tol, bandwidth,
                                         separate_dirs=_separate_dirs_test(direction, angles_tol),

@@ -27,6 +27,8 @@
  *     / structured / ma_structured                gsx_vario_directional_f64,
  *     src/gstools/variogram/variogram.py:12-15,   gsx_vario_structured_f64,
  *     :44-113, call :438-466, :560-565            gsx_vario_workspace
+ *   unstructured(..., distance_type='h')          gsx_vario_unstructured_haversine_f64
+ *     (latlon=True, variogram.py:446)
  *   emcee stretch-move draws of                   gsx_stretch_draws (host only)
  *     RNG.sample_ln_pdf for many chains,
  *     src/gstools/random/rng.py:38-104
@@ -227,6 +229,17 @@ int gsx_vario_unstructured_f64(int64_t F, int64_t P, int64_t dim, int64_t B,
                                const double* pos, int estimator, double* gamma,
                                int64_t* counts, void* workspace,
                                int64_t workspace_bytes, void* stream);
+/* Great-circle variant of gsx_vario_unstructured_f64 (the backend's
+ * unstructured(..., distance_type='h'), src/gstools/variogram/variogram.py:446
+ * with latlon=True): latlon (2, P) = latitude, longitude in degrees; a pair's
+ * distance is its great-circle angle in radians (haversine), edges in
+ * radians.  Workspace: gsx_vario_workspace(0, P, 2, B, 1). */
+int gsx_vario_unstructured_haversine_f64(int64_t F, int64_t P, int64_t B,
+                                         const double* f, const double* edges,
+                                         const double* latlon, int estimator,
+                                         double* gamma, int64_t* counts,
+                                         void* workspace, int64_t workspace_bytes,
+                                         void* stream);
 int gsx_vario_directional_f64(int64_t F, int64_t P, int64_t dim, int64_t B,
                               int64_t D, const double* f, const double* edges,
                               const double* pos, const double* directions,

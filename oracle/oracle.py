@@ -48,7 +48,8 @@ def load_lib():
         lib.oracle_krige.argtypes = [i64, i64, _dp, _dp, _dp, _dp, _dp]
         lib.oracle_krige.restype = None
         _lp = ctypes.POINTER(ctypes.c_int64)
-        lib.oracle_vario_unstructured.argtypes = [i64, i64, i64, i64, _dp, _dp, _dp, ctypes.c_int, _dp, _lp]
+        lib.oracle_vario_unstructured.argtypes = [i64, i64, i64, i64, _dp, _dp, _dp, ctypes.c_int, ctypes.c_int, _dp,
+                                                  _lp]
         lib.oracle_vario_unstructured.restype = None
         lib.oracle_vario_directional.argtypes = [i64, i64, i64, i64, i64, _dp, _dp, _dp, _dp, ctypes.c_double,
                                                  ctypes.c_double, ctypes.c_int, ctypes.c_int, _dp, _lp]
@@ -125,8 +126,12 @@ def _lptr(a):
     return a.ctypes.data_as(ctypes.POINTER(ctypes.c_int64))
 
 
-def vario_unstructured_c(f, bin_edges, pos, estimator_type="m"):
-    """C restatement of the ``unstructured`` variogram kernel -> (gamma (B,), counts (B,))."""
+def vario_unstructured_c(f, bin_edges, pos, estimator_type="m", distance_type="e"):
+    """C restatement of the ``unstructured`` variogram kernel -> (gamma (B,), counts (B,)).
+
+    ``distance_type`` "h": ``pos`` is (lat, lon) in degrees and distances are
+    great-circle angles in radians (haversine).
+    """
     f = np.atleast_2d(_c(f))
     pos = _c(pos)
     if pos.ndim == 1:
@@ -137,8 +142,13 @@ def vario_unstructured_c(f, bin_edges, pos, estimator_type="m"):
         raise ValueError("oracle: pos / field mismatch")
     nb = max(len(e) - 1, 0)
     gamma, counts = np.empty(nb), np.empty(nb, dtype=np.int64)
+    if distance_type not in ("e", "h"):
+        raise ValueError(f"oracle: unknown distance_type {distance_type!r}")
+    if distance_type == "h" and pos.shape[0] != 2:
+        raise ValueError("oracle: haversine distances need (lat, lon) positions")
     load_lib().oracle_vario_unstructured(nf, npnt, pos.shape[0], nb, _ptr(f), _ptr(e), _ptr(pos),
-                                         int(estimator_type == "c"), _ptr(gamma), _lptr(counts))
+                                         int(estimator_type == "c"), int(distance_type == "h"), _ptr(gamma),
+                                         _lptr(counts))
     return gamma, counts
 
 

@@ -30,6 +30,10 @@
 #include <math.h>
 #include <stdint.h>
 
+#ifndef M_PI
+#define M_PI 3.14159265358979323846
+#endif
+
 static double est(int cressie, double d) { return cressie ? sqrt(fabs(d)) : d * d; }
 
 static void normalize(int cressie, int64_t n, double *sums, const int64_t *counts) {
@@ -51,17 +55,33 @@ static double dist_euclid(int64_t dim, int64_t P, const double *pos, int64_t j, 
     return sqrt(s);
 }
 
-/* f (F, P), pos (dim, P), edges (B + 1) -> gamma (B), counts (B) */
+/* Great-circle angle (radians) between points j and k of pos = (lat, lon) in
+ * degrees: the haversine form of the published backend (gstools-cython >= 1,
+ * < 2, variogram kernel with distance_type 'h', selected by
+ * src/gstools/variogram/variogram.py:446 for latlon=True):
+ *   a = sin^2(dlat / 2) + cos(lat_j) cos(lat_k) sin^2(dlon / 2),
+ *   d = 2 atan2(sqrt(a), sqrt(1 - a)),  angles in radians = degrees * (pi / 180). */
+static double dist_haversine(int64_t P, const double *pos, int64_t j, int64_t k) {
+    const double deg_2_rad = M_PI / 180.0;
+    const double diff_lat = (pos[k] - pos[j]) * deg_2_rad;
+    const double diff_lon = (pos[P + k] - pos[P + j]) * deg_2_rad;
+    const double sl = sin(diff_lat / 2.0), so = sin(diff_lon / 2.0);
+    const double arg = sl * sl + cos(pos[j] * deg_2_rad) * cos(pos[k] * deg_2_rad) * (so * so);
+    return 2.0 * atan2(sqrt(arg), sqrt(1.0 - arg));
+}
+
+/* f (F, P), pos (dim, P), edges (B + 1) -> gamma (B), counts (B); haversine != 0:
+ * pos is (lat, lon) in degrees and the distance the great-circle angle */
 void oracle_vario_unstructured(int64_t F, int64_t P, int64_t dim, int64_t B, const double *f,
-                               const double *edges, const double *pos, int cressie, double *gamma,
-                               int64_t *counts) {
+                               const double *edges, const double *pos, int cressie, int haversine,
+                               double *gamma, int64_t *counts) {
 #pragma omp parallel for schedule(dynamic)
     for (int64_t i = 0; i < B; ++i) {
         double s = 0.0;
         int64_t c = 0;
         for (int64_t j = 0; j < P - 1; ++j)
             for (int64_t k = j + 1; k < P; ++k) {
-                const double dist = dist_euclid(dim, P, pos, j, k);
+                const double dist = haversine ? dist_haversine(P, pos, j, k) : dist_euclid(dim, P, pos, j, k);
                 if (dist < edges[i] || dist >= edges[i + 1]) continue;
                 for (int64_t m = 0; m < F; ++m) {
                     const double a = f[m * P + k], b = f[m * P + j];

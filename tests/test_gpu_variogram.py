@@ -7,7 +7,8 @@ reference's rounding, so every pair lands in the same bin); estimates within
 Covered: 1-3 dimensions, stacked fields beyond one launch's field chunk,
 NaN values and positions, non-monotone edges, more (direction, bin) slots
 than one launch holds, directional tolerance / bandwidth / separated
-directions, Cressie, structured and masked lags.
+directions, Cressie, structured and masked lags, great-circle distances
+(lat-lon), mean / trend removal.
 """
 
 import sys
@@ -140,7 +141,30 @@ def test_structured_matches_oracle(n0, n1, masked, est):
     assert np.array_equal(c2.cpu().numpy(), c_ref)
 
 
-@pytest.mark.parametrize("scenario", ["known_values", "uncorrelated", "sampling", "assertions", "direction_assertions"])
+@pytest.mark.parametrize("nf,est", [(1, "m"), (3, "c"), (9, "m")])
+def test_haversine_matches_oracle(nf, est):
+    """Great-circle pair binning (``distance_type`` "h") against the oracle: points over the
+    whole sphere with the poles, the dateline, duplicates and a NaN latitude (NaN distances
+    count in every bin); more fields than one launch stages."""
+    _say(f"vario haversine F={nf} est={est}")
+    rs = np.random.RandomState(31 + nf)
+    p = 1500
+    lat, lon = rs.uniform(-90, 90, p), rs.uniform(-180, 180, p)
+    lat[:4], lon[:4] = [90.0, -90.0, 12.5, 12.5], [0.0, 45.0, 179.99, -179.99]
+    lat[4], lon[4] = lat[5], lon[5]  # a distance-0 pair
+    lat[7] = np.nan
+    pos = np.stack([lat, lon])
+    f = rs.normal(size=(nf, p))
+    f[0, 10:13] = np.nan
+    for edges in (np.linspace(0, np.pi, 31), np.array([0.0, 0.1, 0.15, 1.0, 2.5, 3.2]), np.array([1.0, 0.5, 2.0])):
+        g, c = v.unstructured(f, edges, pos, est, distance_type="h")
+        g_ref, c_ref = oracle.vario_unstructured_c(f, edges, pos, est, "h")
+        assert np.array_equal(c, c_ref)
+        _close(g, g_ref)
+
+
+@pytest.mark.parametrize("scenario", ["known_values", "uncorrelated", "sampling", "assertions", "direction_assertions",
+                                      "latlon_scales", "mean_and_trend"])
 def test_reference_scenarios(scenario):
     _say(f"vario scenario {scenario}")
     getattr(cases, scenario)(gs)
@@ -159,3 +183,4 @@ def test_reference_fields_from_srf():
     cases.stacked_and_no_data(gs, smooth1d)
     cases.direction_axis(gs, field)
     cases.direction_angle(gs, field)
+    cases.latlon_auto_binning(gs, field)

@@ -132,11 +132,61 @@ def direction_angle(gs, field3d):
 
 
 def direction_assertions(gs):
-    """``tests/test_variogram_unstructured.py:275-300`` (the lat-lon case is not restated)."""
+    """``tests/test_variogram_unstructured.py:275-300``."""
     pos = [[1, 2, 3], [1, 2, 3]]
     bins = [1, 2]
     fld = np.ma.array([1, 2, 3])
     for kw in ({"direction": [0, 0]}, {"direction": [[[3, 1]]]}, {"direction": [[3, 1, 2]]},
-               {"angles": [[[1]]]}, {"angles": [[1, 1]]}):
+               {"angles": [[[1]]]}, {"angles": [[1, 1]]}, {"direction": [1, 0], "latlon": True}):
         with pytest.raises(ValueError):
             gs.vario_estimate(pos, fld, bins, **kw)
+
+
+def latlon_auto_binning(gs, field3d):
+    """Lat-lon part of ``tests/test_variogram_unstructured.py:362-398`` and the 1D check of
+    ``:400-405``: Sturges bins (15 for 100 points) up to a third of the great-circle
+    diameter, increasing and positive."""
+    pos = (range(10), range(10), range(10))
+    bin_center, emp_vario = gs.vario_estimate(pos[:2], field3d[..., 0], mesh_type="structured", latlon=True)
+    assert len(bin_center) == 15
+    assert np.all(bin_center[1:] > bin_center[:-1]) and np.all(bin_center > 0)
+    assert np.all(np.isfinite(emp_vario))
+    with pytest.raises(ValueError):
+        gs.vario_estimate([[1, 2]], [1, 2], latlon=True)
+
+
+def latlon_scales(gs):
+    """``geo_scale``: bins given in km (or degrees) select the same pairs as the same bins
+    in radians; the returned centers stay in the caller's unit (``variogram.py:421-431``)."""
+    rs = np.random.RandomState(77)
+    lat, lon = rs.uniform(-60, 60, 300), rs.uniform(-170, 170, 300)
+    fld = rs.normal(size=300)
+    rad = np.linspace(0, 1.5, 12)
+    bc_r, g_r, c_r = gs.vario_estimate((lat, lon), fld, rad, latlon=True, return_counts=True)
+    for scale in (gs.KM_SCALE, gs.DEGREE_SCALE):
+        edges = rad * scale
+        kept = edges.copy()
+        bc, g, c = gs.vario_estimate((lat, lon), fld, edges, latlon=True, geo_scale=scale, return_counts=True)
+        assert np.array_equal(edges, kept)  # the caller's edges are not divided in place
+        assert np.allclose(bc, bc_r * scale)
+        assert np.array_equal(c, c_r) and np.allclose(g, g_r, rtol=1e-12)
+
+
+def mean_and_trend(gs):
+    """``mean`` / ``trend`` removal (``variogram.py:433-440``): a constant mean leaves the
+    variogram unchanged; a callable trend equals removing it from the field first."""
+    rs = np.random.RandomState(5)
+    x = rs.uniform(0, 50, size=(2, 400))
+    fld = rs.normal(size=400) + 0.3 * x[0] - 0.1 * x[1]
+    bins = np.linspace(0, 20, 9)
+    _, g0, c0 = gs.vario_estimate(x, fld, bins, return_counts=True)
+    _, g1, c1 = gs.vario_estimate(x, fld, bins, mean=3.5, return_counts=True)
+    assert np.array_equal(c0, c1) and np.allclose(g0, g1, rtol=1e-12)
+
+    def trend(a, b):
+        return 0.3 * a - 0.1 * b
+
+    _, gt = gs.vario_estimate(x, fld, bins, trend=trend, mean=[1.0, 2.0])
+    _, gm = gs.vario_estimate(x, fld - trend(*x), bins)
+    assert np.allclose(gt, gm, rtol=1e-12)
+    assert np.all(gt < g0)  # the linear trend inflates the raw variogram
