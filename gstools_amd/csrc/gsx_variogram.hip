@@ -100,6 +100,24 @@ __device__ double sq_threshold(double e) {
     return x;
 }
 
+// Q(tol) = the smallest double q in [0, 1) with acos(q) < tol (acos non-increasing):
+// for q in [0, 1), acos(q) < tol exactly when q >= Q(tol); 1.0 when no q in [0, 1)
+// qualifies (tol <= 0 or NaN), 0.0 when every one does.  Bisection over the bit
+// patterns of [0, 1] with the same acos the per-pair test would call.
+__device__ double acos_threshold(double tol) {
+    if (!(acos(1.0) < tol)) return 1.0;  // tol <= 0 or NaN: no q qualifies
+    if (acos(0.0) < tol) return 0.0;
+    long long lo = __double_as_longlong(0.0), hi = __double_as_longlong(1.0);  // acos(lo) >= tol > acos(hi)
+    while (hi - lo > 1) {
+        const long long mid = lo + (hi - lo) / 2;
+        if (acos(__longlong_as_double(mid)) < tol)
+            hi = mid;
+        else
+            lo = mid;
+    }
+    return __longlong_as_double(hi);
+}
+
 // Smallest-index form of the bin search for non-decreasing edges: g in [-1, B]
 // with e[g] <= x < e[g+1] (e[-1] = -inf, e[B+1] = +inf), starting from a guess
 // (uniform edges: the guess is exact but for rounding, so the loops run 0-1 times).
@@ -132,7 +150,10 @@ __global__ __launch_bounds__(kVT) void k_vario_pairs(VarioArgs a, const double* 
     // their two bounding thresholds at one address with no clamping
     double* s_thp = s_edges + (a.B + 1);
     double* s_thr = s_thp + 1;
-    double* s_dirs = s_thp + (a.B + 3);
+    // directional tests on thresholds: acos(q) < angles_tol <=> q >= s_tol[0] (q in
+    // [0, 1)), sqrt(bd) < bandwidth <=> bd < s_tol[1] (bd >= 0 or NaN)
+    double* s_tol = s_thp + (a.B + 3);
+    double* s_dirs = s_tol + 2;
     static_assert(!HAV || (DIM == 2 && !FAST), "haversine distances: 2D, general loop");
     constexpr int NPOS = HAV ? DIM + 1 : DIM;  // staged coordinate rows
     // staged tile: coordinates then field values, rows of kVT, then kVPad zeros;
@@ -146,6 +167,10 @@ __global__ __launch_bounds__(kVT) void k_vario_pairs(VarioArgs a, const double* 
         const double e = edges[i];
         s_edges[i] = e;
         s_thr[i] = sq_threshold(e);
+    }
+    if (a.directional && t == kVT - 1) {
+        s_tol[0] = acos_threshold(a.angles_tol);
+        s_tol[1] = a.bandwidth > 0.0 ? sq_threshold(a.bandwidth) : 0.0;
     }
     if (t == 0) {
         s_thp[0] = -__builtin_inf();
@@ -369,11 +394,11 @@ __global__ __launch_bounds__(kVT) void k_vario_pairs(VarioArgs a, const double* 
                                 const double r = dv[c] - sp * u[c];
                                 bd += r * r;
                             }
-                            in_band = sqrt(bd) < a.bandwidth;
+                            in_band = bd < s_tol[1];  // sqrt(bd) < bandwidth
                         }
                         if (dist > 0.0) {
                             const double q = fabs(sp) / dist;
-                            if (q < 1.0) in_angle = acos(q) < a.angles_tol;
+                            if (q < 1.0) in_angle = q >= s_tol[0];  // acos(q) < angles_tol
                         }
                         if (!(in_band && in_angle)) continue;
                     }
@@ -507,7 +532,7 @@ __global__ void k_vario_lag_collect(int64_t n0, int64_t tiles, const double* __r
 
 // LDS bytes of a pair launch with NH histogram slots (+ one scratch row)
 inline size_t vario_lds_bytes(int64_t B, int64_t D, int dim, int64_t F, int64_t NH) {
-    return sizeof(double) * (2 * (size_t)(B + 1) + 2 + (size_t)(D * dim) + (size_t)dim * kVT + (size_t)F * kVT + kVPad +
+    return sizeof(double) * (2 * (size_t)(B + 1) + 4 + (size_t)(D * dim) + (size_t)dim * kVT + (size_t)F * kVT + kVPad +
                              (size_t)(NH + 1) * kVT) +
            sizeof(unsigned) * (size_t)(NH + 1) * kVT;
 }
