@@ -139,6 +139,39 @@ def bench_mfma_grid(torch, steps, warmup):
     return res
 
 
+def bench_incompr_grid(torch, steps, warmup):
+    """One 128^3 incompressible Gaussian grid (a batch item of the mixed workload), N 4096:
+    the split-bf16 grid path (one matrix-core sum per component) against the FP32
+    incompressible sum on the materialised grid."""
+    import oracle
+    from gstools_amd import Gaussian, IncomprRandMeth, summate_incompr_device
+
+    log("bench: incompressible grid (128^3, N 4096)")
+    model = Gaussian(dim=3, var=1.0, len_scale=10.0)
+    gen = IncomprRandMeth(model, mode_no=4096, seed=20170519, mean_velocity=1.0)
+    dev = torch.device("cuda", 0)
+    ax = np.arange(128, dtype=np.double)
+    axes = [ax] * 3
+    n = 128**3
+    work = float(n) * 4096
+    grid = _grid_pos(torch, [torch.from_numpy(ax).to(dev)] * 3)
+    cov, z1, z2 = (torch.from_numpy(a).to(dev) for a in (gen._cov_sample, gen._z_1, gen._z_2))
+    scale = float(np.sqrt(model.var / 4096))
+    res = {"workload": "incompressible 3D Gaussian var 1 len 10, grid 128^3 spacing 1, mode_no 4096 (MCMC modes)",
+           "points": n, "modes": 4096, "components": 3}
+    wall_bf, _ = time_kernel(lambda: gen.structured(axes, precision="bf16x3"), steps, warmup, torch)
+    out_bf = gen.structured(axes, precision="bf16x3").cpu().numpy().astype(np.float64)
+    wall_32, _ = time_kernel(lambda: summate_incompr_device(cov, z1, z2, grid, "fp32"), steps, warmup, torch)
+    out_64 = summate_incompr_device(cov, z1, z2, grid, "fp64").cpu().numpy() * scale
+    out_bf[0] -= 1.0
+    rel = [oracle.rel_errors(out_bf[c].ravel(), out_64[c]) for c in range(3)]
+    res["bf16x3_structured"] = {"value": work / wall_bf, "ms_per_step": wall_bf * 1e3,
+                                "rel_max_vs_fp64": max(r[0] for r in rel), "rel_rms_vs_fp64": max(r[1] for r in rel)}
+    res["fp32_points"] = {"value": work / wall_32, "ms_per_step": wall_32 * 1e3}
+    res["speedup"] = wall_32 / wall_bf
+    return res
+
+
 def bench_fourier(torch, steps, warmup):
     """Periodic 2D Gaussian (var 1, len 200) on a 4096^2 grid, period 4096, mode_no 256^2 (Fourier method)."""
     import oracle
@@ -528,8 +561,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--extras", type=int, default=1, help="also run the other configurations")
-    ap.add_argument("--only", choices=["fp32", "fp64", "mfma", "fourier", "krige", "variogram", "adjoint", "batch",
-                                       "copy"],
+    ap.add_argument("--only", choices=["fp32", "fp64", "mfma", "incompr", "fourier", "krige", "variogram", "adjoint",
+                                       "batch", "copy"],
                     help="run one section only (for profiling) and print its JSON")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -551,8 +584,9 @@ def main():
     dev = torch.device("cuda", 0)
     k_ext = max(3, args.steps // 4)
     w_ext = max(1, args.warmup)
-    if args.only in ("mfma", "fourier", "krige", "variogram", "adjoint", "batch", "copy"):
+    if args.only in ("mfma", "incompr", "fourier", "krige", "variogram", "adjoint", "batch", "copy"):
         fn = {"mfma": lambda: bench_mfma_grid(torch, k_ext, w_ext),
+              "incompr": lambda: bench_incompr_grid(torch, k_ext, w_ext),
               "variogram": lambda: bench_variogram(torch, k_ext, w_ext),
               "krige": lambda: bench_krige(torch, k_ext, w_ext),
               "fourier": lambda: bench_fourier(torch, k_ext, w_ext),
@@ -638,7 +672,8 @@ def main():
     }
     if args.extras:
         extras = {}
-        sections = (("mfma_grid", bench_mfma_grid), ("fourier", bench_fourier), ("krige", bench_krige),
+        sections = (("mfma_grid", bench_mfma_grid), ("incompr_grid", bench_incompr_grid), ("fourier", bench_fourier),
+                    ("krige", bench_krige),
                     ("variogram", bench_variogram),
                     ("adjoint", bench_adjoint),
                     ("batch", lambda t, k, w: bench_batch(t, k, w, *batch_inputs)))

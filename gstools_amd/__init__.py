@@ -47,6 +47,7 @@ from .summate import (  # noqa: F401
     summate_grid_device,
     summate_incompr,
     summate_incompr_device,
+    summate_incompr_grid_device,
 )
 
 from .variogram import standard_bins, vario_estimate, vario_estimate_axis  # noqa: F401

@@ -64,6 +64,11 @@ EXPORTS = {
     "gsx_summate_grid_f32": (_i32, _GRID),
     "gsx_summate_grid_mfma_workspace": (_i64, [_i64, _i64, _i64p]),
     "gsx_summate_grid_mfma": (_i32, [_i64, _i64, _i64p, _vp, _vp, _vp, _vp, _i32, _f64, _vp, _vp, _i64, _vp]),
+    "gsx_summate_incompr_grid_f64": (_i32, _GRID),
+    "gsx_summate_incompr_grid_f32": (_i32, _GRID),
+    "gsx_summate_incompr_grid_mfma_workspace": (_i64, [_i64, _i64, _i64p]),
+    "gsx_summate_incompr_grid_mfma": (_i32, [_i64, _i64, _i64p, _vp, _vp, _vp, _vp, _i32, _f64, _vp, _vp, _i64,
+                                             _vp]),
     "gsx_summate_fourier_f64": (_i32, _FOUR),
     "gsx_summate_fourier_f32": (_i32, _FOUR),
     "gsx_summate_fourier_grid_f64": (_i32, _FOUR_GRID),

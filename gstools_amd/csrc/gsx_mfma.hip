@@ -85,8 +85,9 @@ inline MfmaGeom mfma_geom(int64_t d, int64_t N, const int64_t* shape) {
     return g;
 }
 
-inline int64_t mfma_ws_bytes(const MfmaGeom& g) {
-    return 2 * g.a_elems * 2 + (g.e1_elems + g.e2_elems) * 8;
+// A hi / lo (16-bit), E1 and one E2 table per output component (complex FP32)
+inline int64_t mfma_ws_bytes(const MfmaGeom& g, int64_t comps = 1) {
+    return 2 * g.a_elems * 2 + (g.e1_elems + comps * g.e2_elems) * 8;
 }
 
 // ---------------------------------------------------------------- splits ---
@@ -108,9 +109,12 @@ __device__ __forceinline__ void split2(float a, float b, uint32_t& hi, uint32_t&
 }
 
 // --------------------------------------------------------------- tables ----
-// One thread per table entry: A (n0p x Np), E1 (n1 x Np), E2 (n2 x Np).
+// One thread per table entry: A (n0p x Np), E1 (n1 x Np), E2 (comps x n2 x Np).
+// comps > 1 (incompressible fields): E2 of component c carries the projector
+// weight p_c(k_j) = delta_c0 - k_jc k_j0 / |k_j|^2 (generator.py:479-494) in
+// place of the spectrum factor.
 template <bool FP16>
-__global__ __launch_bounds__(256) void k_mfma_tables(MfmaGeom g, int64_t d, int64_t N,
+__global__ __launch_bounds__(256) void k_mfma_tables(MfmaGeom g, int64_t d, int64_t N, int64_t comps,
                                                      const double* __restrict__ ax0,
                                                      const double* __restrict__ ax1,
                                                      const double* __restrict__ ax2,
@@ -123,7 +127,7 @@ __global__ __launch_bounds__(256) void k_mfma_tables(MfmaGeom g, int64_t d, int6
                                                      float2* __restrict__ e1,
                                                      float2* __restrict__ e2) {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const int64_t na = g.n0p * g.Np, n1 = g.e1_elems, n2 = g.e2_elems;
+    const int64_t na = g.n0p * g.Np, n1 = g.e1_elems, n2 = comps * g.e2_elems;
     if (t < na) {
         const int64_t row = t / g.Np, j = t - row * g.Np;
         float re = 0.0f, im = 0.0f;
@@ -155,14 +159,20 @@ __global__ __launch_bounds__(256) void k_mfma_tables(MfmaGeom g, int64_t d, int6
         e1[u] = v;
     } else if (t < na + n1 + n2) {
         const int64_t u = t - na - n1;
-        const int64_t i2 = u / g.Np, j = u - i2 * g.Np;
+        const int64_t comp = u / g.e2_elems, uc = u - comp * g.e2_elems;
+        const int64_t i2 = uc / g.Np, j = uc - i2 * g.Np;
         float2 v = {0.0f, 0.0f};
         if (j < N) {
             double s, c;
             const double kz = cov[(d - 1) * N + j];
             const double th = kz * (d == 3 ? ax2[i2] : ax1[i2]);
             if (fabs(th) <= kFastPhaseMax) sincos_fast(th, &s, &c); else sincos(th, &s, &c);
-            const double f = fac ? fac[j] : 1.0;
+            double f = fac ? fac[j] : 1.0;
+            if (comps > 1) {
+                double k2 = 0.0;
+                for (int64_t e = 0; e < d; ++e) k2 += cov[e * N + j] * cov[e * N + j];
+                f = (comp == 0 ? 1.0 : 0.0) - cov[comp * N + j] * cov[j] / k2;
+            }
             const double a = f * z1[j], b = f * z2[j];
             // (a - i b)(c + i s)
             v = {(float)(a * c + b * s), (float)(a * s - b * c)};
@@ -194,7 +204,7 @@ __global__ __launch_bounds__(kMWaves * 64, 2) void k_mfma_grid(MfmaGeom g, doubl
                                                                const uint32_t* __restrict__ a_lo,
                                                                const float2* __restrict__ e1,
                                                                const float2* __restrict__ e2,
-                                                               float* __restrict__ out) {
+                                                               float* __restrict__ out, int64_t out_stride) {
     __shared__ __align__(16) uint16_t lds[2][2][kMBM * kMLdsRow];  // [buffer][hi/lo][row * stride]
 
     const int lane = threadIdx.x & 63;
@@ -202,8 +212,15 @@ __global__ __launch_bounds__(kMWaves * 64, 2) void k_mfma_grid(MfmaGeom g, doubl
     const int grp = lane >> 4;  // K group g
     const int l16 = lane & 15;
 
-    // XCD-aware tile mapping: dispatch places block b on XCD b % 8.
-    const int64_t b = blockIdx.x;
+    // Component (incompressible fields: one E2 table and output row per
+    // component), then the XCD-aware tile mapping: dispatch places block b on
+    // XCD b % 8 (the tile count per component is a multiple of 8 or the
+    // mapping below falls back to row-major).
+    const int64_t ntile = g.row_tiles * g.col_tiles;
+    const int64_t comp = (int64_t)blockIdx.x / ntile;
+    const int64_t b = (int64_t)blockIdx.x - comp * ntile;
+    e2 += comp * g.e2_elems;
+    out += comp * out_stride;
     int64_t rt, ct;
     {
         const int64_t xcd = b & 7, w = b >> 3;
@@ -372,7 +389,7 @@ __global__ __launch_bounds__(kMWaves * 64, 2) void k_mfma_grid3(MfmaGeom g, doub
                                                                 const uint32_t* __restrict__ a_lo,
                                                                 const float2* __restrict__ e1,
                                                                 const float2* __restrict__ e2,
-                                                                float* __restrict__ out) {
+                                                                float* __restrict__ out, int64_t out_stride) {
     __shared__ Stage3 st[2];
     const int tid = threadIdx.x;
     const int lane = tid & 63;
@@ -380,7 +397,11 @@ __global__ __launch_bounds__(kMWaves * 64, 2) void k_mfma_grid3(MfmaGeom g, doub
     const int grp = lane >> 4;
     const int l16 = lane & 15;
 
-    const int64_t b = blockIdx.x;
+    const int64_t ntile = g.row_tiles * g.col_tiles;  // per component
+    const int64_t comp = (int64_t)blockIdx.x / ntile;
+    const int64_t b = (int64_t)blockIdx.x - comp * ntile;
+    e2 += comp * g.e2_elems;
+    out += comp * out_stride;
     int64_t rt, ct;
     {
         const int64_t xcd = b & 7, w = b >> 3;
@@ -515,36 +536,38 @@ __global__ void k_fill_zero_f32(float* __restrict__ p, int64_t n) {
 }
 
 template <bool FP16>
-int mfma_launch(const MfmaGeom& g, int64_t d, int64_t N, const double* ax0, const double* ax1,
+int mfma_launch(const MfmaGeom& g, int64_t d, int64_t N, int64_t comps, const double* ax0, const double* ax1,
                 const double* ax2, const double* cov, const double* z1, const double* z2, const double* fac,
                 double alpha, float* out, void* ws, hipStream_t s) {
     uint32_t* a_hi = static_cast<uint32_t*>(ws);
     uint32_t* a_lo = a_hi + g.a_elems / 2;
     float2* e1 = reinterpret_cast<float2*>(a_lo + g.a_elems / 2);
     float2* e2 = e1 + g.e1_elems;
-    const int64_t entries = g.n0p * g.Np + g.e1_elems + g.e2_elems;
+    const int64_t entries = g.n0p * g.Np + g.e1_elems + comps * g.e2_elems;
     hipLaunchKernelGGL((k_mfma_tables<FP16>), dim3((unsigned)ceil_div(entries, 256)), dim3(256), 0, s, g, d,
-                       N, ax0, ax1, ax2, cov, z1, z2, fac, a_hi, a_lo, e1, e2);
+                       N, comps, ax0, ax1, ax2, cov, z1, z2, fac, a_hi, a_lo, e1, e2);
     int st = launch_status();
     if (st != GSX_OK) return st;
-    const int64_t blocks = g.row_tiles * g.col_tiles;
+    const int64_t blocks = comps * g.row_tiles * g.col_tiles;
+    const int64_t stride = g.n0 * g.ncols;  // output values per component
     if (g.g2 == 1) {
         hipLaunchKernelGGL((k_mfma_grid3<FP16>), dim3((unsigned)blocks), dim3(kMWaves * 64), 0, s, g, alpha,
-                           a_hi, a_lo, e1, e2, out);
+                           a_hi, a_lo, e1, e2, out, stride);
     } else {
         if (d == 2)
             hipLaunchKernelGGL((k_mfma_grid<FP16, 16, true>), dim3((unsigned)blocks), dim3(kMWaves * 64), 0, s, g,
-                               alpha, a_hi, a_lo, e1, e2, out);
+                               alpha, a_hi, a_lo, e1, e2, out, stride);
         else
             hipLaunchKernelGGL((k_mfma_grid<FP16, 16, false>), dim3((unsigned)blocks), dim3(kMWaves * 64), 0, s, g,
-                               alpha, a_hi, a_lo, e1, e2, out);
+                               alpha, a_hi, a_lo, e1, e2, out, stride);
     }
     return launch_status();
 }
 
+// comps: 1 (scalar sum) or d (incompressible: out holds d components of prod(shape) values)
 int mfma_entry(int64_t d, int64_t N, const int64_t* shape, const double* axes, const double* cov,
                const double* z1, const double* z2, const double* fac, int variant, double alpha, float* out,
-               void* workspace, int64_t workspace_bytes, void* stream) {
+               void* workspace, int64_t workspace_bytes, void* stream, int64_t comps = 1) {
     if (d == 1) return GSX_E_UNSUPPORTED;
     if ((d != 2 && d != 3) || N < 0 || !shape) return GSX_E_INVALID;
     if (variant != GSX_MFMA_BF16X3 && variant != GSX_MFMA_FP16X3) return GSX_E_INVALID;
@@ -557,18 +580,19 @@ int mfma_entry(int64_t d, int64_t N, const int64_t* shape, const double* axes, c
     if (!axes || !out || (N > 0 && (!cov || !z1 || !z2))) return GSX_E_INVALID;
     hipStream_t s = as_stream(stream);
     if (N == 0) {  // empty sum
-        hipLaunchKernelGGL(k_fill_zero_f32, dim3((unsigned)ceil_div(P, 256)), dim3(256), 0, s, out, P);
+        hipLaunchKernelGGL(k_fill_zero_f32, dim3((unsigned)ceil_div(comps * P, 256)), dim3(256), 0, s, out,
+                           comps * P);
         return launch_status();
     }
     const MfmaGeom g = mfma_geom(d, N, shape);
-    if (g.row_tiles * g.col_tiles > 0x7fffffffLL) return GSX_E_INVALID;
-    if (!workspace || workspace_bytes < mfma_ws_bytes(g)) return GSX_E_WORKSPACE;
+    if (comps * g.row_tiles * g.col_tiles > 0x7fffffffLL) return GSX_E_INVALID;
+    if (!workspace || workspace_bytes < mfma_ws_bytes(g, comps)) return GSX_E_WORKSPACE;
     const double* ax0 = axes;
     const double* ax1 = axes + shape[0];
     const double* ax2 = d == 3 ? ax1 + shape[1] : nullptr;
     if (variant == GSX_MFMA_FP16X3)
-        return mfma_launch<true>(g, d, N, ax0, ax1, ax2, cov, z1, z2, fac, alpha, out, workspace, s);
-    return mfma_launch<false>(g, d, N, ax0, ax1, ax2, cov, z1, z2, fac, alpha, out, workspace, s);
+        return mfma_launch<true>(g, d, N, comps, ax0, ax1, ax2, cov, z1, z2, fac, alpha, out, workspace, s);
+    return mfma_launch<false>(g, d, N, comps, ax0, ax1, ax2, cov, z1, z2, fac, alpha, out, workspace, s);
 }
 
 }  // namespace gsx
@@ -588,6 +612,22 @@ int gsx_summate_grid_mfma(int64_t d, int64_t N, const int64_t* shape, const doub
                           void* workspace, int64_t workspace_bytes, void* stream) {
     return gsx::mfma_entry(d, N, shape, axes, cov, z1, z2, nullptr, variant, alpha, out, workspace,
                            workspace_bytes, stream);
+}
+
+int64_t gsx_summate_incompr_grid_mfma_workspace(int64_t d, int64_t N, const int64_t* shape) {
+    if ((d != 2 && d != 3) || N < 0 || !shape) return GSX_E_INVALID;
+    for (int i = 0; i < d; ++i)
+        if (shape[i] < 0) return GSX_E_INVALID;
+    const gsx::MfmaGeom g = gsx::mfma_geom(d, N, shape);
+    return gsx::mfma_ws_bytes(g, d);
+}
+
+int gsx_summate_incompr_grid_mfma(int64_t d, int64_t N, const int64_t* shape, const double* axes, const double* cov,
+                                  const double* z1, const double* z2, int variant, double alpha, float* out,
+                                  void* workspace, int64_t workspace_bytes, void* stream) {
+    if (d == 1) return GSX_E_INVALID;  // incompressible fields are 2D or 3D
+    return gsx::mfma_entry(d, N, shape, axes, cov, z1, z2, nullptr, variant, alpha, out, workspace,
+                           workspace_bytes, stream, d);
 }
 
 int gsx_summate_fourier_grid_mfma(int64_t d, int64_t N, const int64_t* shape, const double* axes,

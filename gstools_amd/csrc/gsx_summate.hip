@@ -326,10 +326,10 @@ int forward_unstructured(int64_t d, int64_t N, int64_t P, const double* cov, con
     }
 }
 
-template <class OutT>
+template <class OutT, bool INCOMPR = false>
 int forward_grid(int64_t d, int64_t N, const int64_t* shape, const double* axes, const double* cov,
                  const double* z1, const double* z2, const double* fac, OutT* out, void* stream) {
-    if (d < 1 || d > 3 || N < 0 || !shape) return GSX_E_INVALID;
+    if (d < (INCOMPR ? 2 : 1) || d > 3 || N < 0 || !shape) return GSX_E_INVALID;
     int64_t P = 1;
     for (int i = 0; i < d; ++i) {
         if (shape[i] < 0) return GSX_E_INVALID;
@@ -344,10 +344,18 @@ int forward_grid(int64_t d, int64_t N, const int64_t* shape, const double* axes,
     const double* a2 = d > 2 ? a1 + shape[1] : nullptr;
     const uint32_t n1 = d > 1 ? (uint32_t)shape[1] : 1u;
     const uint32_t n2 = d > 2 ? (uint32_t)shape[2] : 1u;
-    switch (d) {
-        case 1: return launch_forward<1, false>(GridPoints<1>{a0, a1, a2, n1, n2}, N, P, cov, z1, z2, fac, out, s);
-        case 2: return launch_forward<2, false>(GridPoints<2>{a0, a1, a2, n1, n2}, N, P, cov, z1, z2, fac, out, s);
-        default: return launch_forward<3, false>(GridPoints<3>{a0, a1, a2, n1, n2}, N, P, cov, z1, z2, fac, out, s);
+    if constexpr (INCOMPR) {
+        if (d == 2) return launch_forward<2, true>(GridPoints<2>{a0, a1, a2, n1, n2}, N, P, cov, z1, z2, fac, out, s);
+        return launch_forward<3, true>(GridPoints<3>{a0, a1, a2, n1, n2}, N, P, cov, z1, z2, fac, out, s);
+    } else {
+        switch (d) {
+            case 1:
+                return launch_forward<1, false>(GridPoints<1>{a0, a1, a2, n1, n2}, N, P, cov, z1, z2, fac, out, s);
+            case 2:
+                return launch_forward<2, false>(GridPoints<2>{a0, a1, a2, n1, n2}, N, P, cov, z1, z2, fac, out, s);
+            default:
+                return launch_forward<3, false>(GridPoints<3>{a0, a1, a2, n1, n2}, N, P, cov, z1, z2, fac, out, s);
+        }
     }
 }
 
@@ -385,6 +393,18 @@ int gsx_summate_grid_f32(int64_t d, int64_t N, const int64_t* shape, const doubl
                          const double* cov, const double* z1, const double* z2, float* out,
                          void* stream) {
     return gsx::forward_grid(d, N, shape, axes, cov, z1, z2, nullptr, out, stream);
+}
+
+int gsx_summate_incompr_grid_f64(int64_t d, int64_t N, const int64_t* shape, const double* axes,
+                                 const double* cov, const double* z1, const double* z2, double* out,
+                                 void* stream) {
+    return gsx::forward_grid<double, true>(d, N, shape, axes, cov, z1, z2, nullptr, out, stream);
+}
+
+int gsx_summate_incompr_grid_f32(int64_t d, int64_t N, const int64_t* shape, const double* axes,
+                                 const double* cov, const double* z1, const double* z2, float* out,
+                                 void* stream) {
+    return gsx::forward_grid<float, true>(d, N, shape, axes, cov, z1, z2, nullptr, out, stream);
 }
 
 // Fourier method: the randomisation sum with per-mode spectrum factors.

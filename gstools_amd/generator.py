@@ -33,6 +33,7 @@ from .summate import (
     PRECISIONS,
     summate_device,
     summate_grid_device,
+    summate_incompr_grid_device,
     summate_incompr_device,
 )
 
@@ -323,6 +324,40 @@ class IncomprRandMeth(RandMeth):
                 nugget = torch.as_tensor(nugget, device=summed.device)
             return out + nugget
         return self.mean_u * e1 + scale * summed.cpu().numpy() + nugget
+
+    def structured(self, axes, precision=None, add_nugget=True):
+        """Vector field (d, *shape) on the 'ij' meshgrid of ``axes`` without materialising it.
+
+        ``summate_incompr`` (``generator.py:479-494``) on the grid: the direct
+        sums ("fp64" / "fp32") or, on the matrix cores ("bf16x3" / "fp16x3"),
+        one GEMM per component with the projector weight
+        ``p_c(k_j) = delta_c0 - k_jc k_j0 / |k_j|^2`` per mode; then
+        ``mean_u e1 + mean_u sqrt(var / N) * sum`` as in ``__call__``.  A k = 0
+        mode makes every value NaN, as the reference's 0 / 0 does.
+        """
+        precision = precision or self.precision
+        if precision not in GRID_PRECISIONS:
+            raise ValueError(f"precision must be one of {GRID_PRECISIONS}")
+        d = self.model.dim
+        shape = tuple(int(np.size(a)) for a in axes)
+        if len(shape) != d:
+            raise ValueError(f"expected {d} axes, got {len(shape)}")
+        mfma = precision in ("bf16x3", "fp16x3")
+        dtype = torch.float32 if mfma else torch.float64
+        if self.zero_var:
+            out = torch.zeros((d,) + shape, dtype=dtype, device="cuda")
+        else:
+            cov, z1, z2 = self._device_modes()
+            scale = float(self.mean_u * np.sqrt(self.model.var / self._mode_no))
+            if mfma:
+                out = summate_incompr_grid_device(cov, z1, z2, axes, precision, alpha=scale).reshape((d,) + shape)
+            else:
+                out = summate_incompr_grid_device(cov, z1, z2, axes, precision).to(torch.float64)
+                out = (out * scale).reshape((d,) + shape)
+        out[0] += self.mean_u
+        if add_nugget and self.model.nugget > 0:
+            out = out + torch.as_tensor(self.get_nugget((d,) + shape), device=out.device, dtype=out.dtype)
+        return out
 
     def _create_unit_vector(self, broadcast_shape, axis=0):
         shape = np.ones(len(broadcast_shape), dtype=int)

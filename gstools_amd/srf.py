@@ -50,7 +50,7 @@ class SRF:
                 raise ValueError("SRF: position tuple does not match the model dimension")
             shape = tuple(a.size for a in axes)
             iso_free = np.allclose(self.model.anis, 1.0) and np.allclose(self.model.angles, 0.0)
-            if iso_free and self.value_type == "scalar":
+            if iso_free and hasattr(self.generator, "structured"):
                 field = self.generator.structured(axes, precision=precision).cpu().numpy()
                 return self._post(np.asarray(field, dtype=np.double))
             pos = generate_grid(axes)

@@ -204,6 +204,53 @@ def summate_grid_device(cov_samples, z_1, z_2, axes, precision="fp64", alpha=1.0
     return out
 
 
+def summate_incompr_grid_device(cov_samples, z_1, z_2, axes, precision="fp64", alpha=1.0, out=None):
+    """Incompressible sum on the 'ij' meshgrid of ``axes`` -> tensor (d, prod(shape)).
+
+    "fp64" / "fp32": the direct sum, one phase per (point, mode) for all d
+    components ("fp64" gives the phases of the materialised grid bit for bit;
+    the result is unscaled and ``alpha`` must be 1).  "bf16x3" / "fp16x3": one
+    matrix-core GEMM per component in one launch, with the projector weight
+    per mode, multiplied by ``alpha``.
+    """
+    dev = _device()
+    cov, z1, z2 = _modes(cov_samples, z_1, z_2, dev)
+    d, n = cov.shape
+    if d < 2:
+        raise ValueError("Only 2D and 3D incompressible fields can be generated.")
+    axes = [_as_f64(a, "axes", dev).reshape(-1) for a in axes]
+    if len(axes) != d:
+        raise ValueError(f"expected {d} axes, got {len(axes)}")
+    shape = [int(a.numel()) for a in axes]
+    npts = int(np.prod(shape))
+    flat = torch.cat(axes)
+    shape_c = (ctypes.c_int64 * 3)(*(shape + [1] * (3 - d)))
+    if precision not in GRID_PRECISIONS:
+        raise ValueError(f"precision must be one of {GRID_PRECISIONS}")
+    dtype = torch.float64 if precision == "fp64" else torch.float32
+    if out is None:
+        out = torch.empty((d, npts), dtype=dtype, device=dev)
+    elif out.dtype != dtype or tuple(out.shape) != (d, npts) or not out.is_contiguous():
+        raise ValueError("out: wrong dtype, shape or layout")
+    lib = _native.lib()
+    if precision in ("fp64", "fp32"):
+        if alpha != 1.0:
+            raise ValueError("alpha applies to the matrix-core path only")
+        fn = lib.gsx_summate_incompr_grid_f64 if precision == "fp64" else lib.gsx_summate_incompr_grid_f32
+        _native.check(fn(d, n, shape_c, _ptr(flat), _ptr(cov), _ptr(z1), _ptr(z2), _ptr(out), _stream()),
+                      "summate_incompr_grid")
+        return out
+    variant = _native.MFMA_BF16X3 if precision == "bf16x3" else _native.MFMA_FP16X3
+    ws_bytes = int(lib.gsx_summate_incompr_grid_mfma_workspace(d, n, shape_c))
+    if ws_bytes < 0:
+        _native.check(int(ws_bytes), "summate_incompr_grid_mfma")
+    ws = torch.empty(max(ws_bytes, 1), dtype=torch.uint8, device=dev)
+    _native.check(lib.gsx_summate_incompr_grid_mfma(d, n, shape_c, _ptr(flat), _ptr(cov), _ptr(z1), _ptr(z2),
+                                                    variant, float(alpha), _ptr(out), _ptr(ws), ws_bytes,
+                                                    _stream()), "summate_incompr_grid_mfma")
+    return out
+
+
 def summate_fourier_device(spectrum_factor, modes, z_1, z_2, pos, precision="fp64", out=None):
     """Fourier-method sum on the GPU -> torch tensor (P,).
 

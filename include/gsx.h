@@ -12,6 +12,8 @@
  *   summate on a meshgrid from Field.pre_pos     gsx_summate_grid_f64 / _f32,
  *     src/gstools/field/base.py:254-297 ->         gsx_summate_grid_mfma
  *     tools/geometric.py:340-356
+ *   summate_incompr on a meshgrid                gsx_summate_incompr_grid_f64 / _f32,
+ *     src/gstools/field/generator.py:529-567      gsx_summate_incompr_grid_mfma
  *   summate_fourier(spectrum_factor, modes,       gsx_summate_fourier_f64 / _f32,
  *     z_1, z_2, pos)                              gsx_summate_fourier_grid_f64 / _f32,
  *     src/gstools/field/generator.py:23, :67-75,  gsx_summate_fourier_grid_mfma
@@ -126,6 +128,31 @@ int gsx_summate_grid_mfma(int64_t d, int64_t N, const int64_t* shape,
                           const double* z1, const double* z2, int variant,
                           double alpha, float* out, void* workspace,
                           int64_t workspace_bytes, void* stream);
+
+/* Incompressible sum on a structured grid (summate_incompr at the meshgrid of
+ * the axes, generator.py:479-494, :529-567 with generate_grid positions):
+ * out (d, prod(shape)) in C order, d = 2 or 3.  The direct forms compute each
+ * phase once for all d components, as gsx_summate_incompr_f64 / _f32 on the
+ * materialised grid (same phases).  The matrix-core form runs one GEMM per
+ * component in one launch: component c is the scalar sum with the projector
+ * weight p_c(k_j) = delta_c0 - k_jc k_j0 / |k_j|^2 per mode, scaled by alpha;
+ * workspace from gsx_summate_incompr_grid_mfma_workspace(). */
+int gsx_summate_incompr_grid_f64(int64_t d, int64_t N, const int64_t* shape,
+                                 const double* axes, const double* cov,
+                                 const double* z1, const double* z2,
+                                 double* out, void* stream);
+int gsx_summate_incompr_grid_f32(int64_t d, int64_t N, const int64_t* shape,
+                                 const double* axes, const double* cov,
+                                 const double* z1, const double* z2,
+                                 float* out, void* stream);
+int64_t gsx_summate_incompr_grid_mfma_workspace(int64_t d, int64_t N,
+                                                const int64_t* shape);
+int gsx_summate_incompr_grid_mfma(int64_t d, int64_t N, const int64_t* shape,
+                                  const double* axes, const double* cov,
+                                  const double* z1, const double* z2,
+                                  int variant, double alpha, float* out,
+                                  void* workspace, int64_t workspace_bytes,
+                                  void* stream);
 
 /* Fourier method: out = sum_j f_j (z1_j cos(k_j.x) + z2_j sin(k_j.x)) with the
  * per-mode spectrum factor f = spectrum_factor (N,) (Fourier.reset_seed,

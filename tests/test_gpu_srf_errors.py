@@ -73,6 +73,36 @@ def test_srf_incompressible_known_values():
     assert field[0, 1, 0] == pytest.approx(1.06180674, abs=5e-8)
 
 
+@pytest.mark.parametrize("dim,shape", [(2, (37, 29)), (3, (24, 20, 18))])
+def test_incompressible_structured_all_precisions(dim, shape):
+    """``IncomprRandMeth.structured``: each component is a grid sum with the per-mode
+    projector weight; FP64 within 1e-12 of the incompressible sum on the materialised
+    grid (the oracle), FP32 within 1e-6, the matrix-core splits within 1e-5 (rms)."""
+    _say(f"incompr structured dim={dim}")
+    from gstools_amd import IncomprRandMeth
+
+    model = Gaussian(dim=dim, var=1.5, len_scale=4.0)
+    axes = [np.linspace(0.0, 30.0, n) for n in shape]
+    gen = IncomprRandMeth(model, mode_no=384, seed=11, mean_velocity=0.7)
+    scale = 0.7 * np.sqrt(1.5 / 384)
+    ref = scale * oracle.summate_incompr_c(gen._cov_sample, gen._z_1, gen._z_2, generate_grid(axes))
+    ref[0] += 0.7
+    for precision, bar in (("fp64", 1e-12), ("fp32", 1e-6), ("bf16x3", 1e-5), ("fp16x3", 1e-5)):
+        out = gen.structured(axes, precision=precision).cpu().numpy().astype(np.float64)
+        assert out.shape == (dim,) + shape
+        for c in range(dim):
+            err = oracle.rel_errors(out[c].ravel() - (0.7 if c == 0 else 0.0),
+                                    ref[c] - (0.7 if c == 0 else 0.0))
+            assert (err[0] if precision in ("fp64", "fp32") else err[1]) <= bar, (precision, c, err)
+    pts = gen(generate_grid(axes))
+    # the direct FP64 grid form has the phases of the materialised grid: same bits
+    assert np.array_equal(gen.structured(axes, precision="fp64").cpu().numpy().reshape(dim, -1), pts)
+    # the C ABI's incompressible grid entries reject 1D grids
+    lib = _native.lib()
+    shp = (ctypes.c_int64 * 3)(5, 1, 1)
+    assert lib.gsx_summate_incompr_grid_mfma_workspace(1, 4, shp) < 0
+
+
 def test_zero_variance_and_nugget():
     model = Gaussian(dim=2, var=0.0, len_scale=3.0)
     rm = RandMeth(model, mode_no=10, seed=1)
