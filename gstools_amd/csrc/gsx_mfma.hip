@@ -85,9 +85,38 @@ inline MfmaGeom mfma_geom(int64_t d, int64_t N, const int64_t* shape) {
     return g;
 }
 
+// Blocks resident at once (256 CUs x 2 blocks at 2 waves per SIMD): grids with
+// fewer tiles split the K steps over floor(512 / blocks) blocks (at least 8 steps
+// = 128 modes each), so the launch stays one round with no tail, and add the
+// partial sums afterwards.
+constexpr int64_t kMFillBlocks = 512;
+
+// (>= 1 also with no K steps or no tiles: those launch nothing)
+inline int64_t mfma_steps_per_split(const MfmaGeom& g, int64_t comps) {
+    const int64_t blocks = comps * g.row_tiles * g.col_tiles;
+    const int64_t steps = g.Np / kMKM;
+    if (steps < 1 || blocks < 1) return 1;
+    int64_t k = blocks >= kMFillBlocks ? 1 : kMFillBlocks / blocks;
+    if (k > steps / 8) k = steps / 8;
+    if (k < 1) k = 1;
+    return ceil_div(steps, k);
+}
+
+inline int64_t mfma_ksplit(const MfmaGeom& g, int64_t comps) {
+    const int64_t steps = g.Np / kMKM;
+    return steps < 1 ? 1 : ceil_div(steps, mfma_steps_per_split(g, comps));
+}
+
 // A hi / lo (16-bit), E1 and one E2 table per output component (complex FP32)
-inline int64_t mfma_ws_bytes(const MfmaGeom& g, int64_t comps = 1) {
+inline int64_t mfma_table_bytes(const MfmaGeom& g, int64_t comps) {
     return 2 * g.a_elems * 2 + (g.e1_elems + comps * g.e2_elems) * 8;
+}
+
+// tables, then (K split) the FP32 partial sums, 256-byte aligned
+inline int64_t mfma_ws_bytes(const MfmaGeom& g, int64_t comps = 1) {
+    const int64_t ks = mfma_ksplit(g, comps);
+    const int64_t tables = ceil_div(mfma_table_bytes(g, comps), (int64_t)256) * 256;
+    return ks > 1 ? tables + ks * comps * g.n0 * g.ncols * 4 : mfma_table_bytes(g, comps);
 }
 
 // ---------------------------------------------------------------- splits ---
@@ -198,13 +227,16 @@ __device__ __forceinline__ float2 cmul(float2 a, float2 b) {
 }
 
 // E1ONE: the grid has a one-point middle axis (2D grids), E1 == 1, so B is E2.
-template <bool FP16, int G2, bool E1ONE>
+// SPLIT: the K steps are split over blocks (small grids), partial sums written.
+template <bool FP16, int G2, bool E1ONE, bool SPLIT>
 __global__ __launch_bounds__(kMWaves * 64, 2) void k_mfma_grid(MfmaGeom g, double alpha,
                                                                const uint32_t* __restrict__ a_hi,
                                                                const uint32_t* __restrict__ a_lo,
                                                                const float2* __restrict__ e1,
                                                                const float2* __restrict__ e2,
-                                                               float* __restrict__ out, int64_t out_stride) {
+                                                               float* __restrict__ out, int64_t out_stride,
+                                                               int64_t comps, int64_t steps_per,
+                                                               float* __restrict__ partial) {
     __shared__ __align__(16) uint16_t lds[2][2][kMBM * kMLdsRow];  // [buffer][hi/lo][row * stride]
 
     const int lane = threadIdx.x & 63;
@@ -212,15 +244,17 @@ __global__ __launch_bounds__(kMWaves * 64, 2) void k_mfma_grid(MfmaGeom g, doubl
     const int grp = lane >> 4;  // K group g
     const int l16 = lane & 15;
 
-    // Component (incompressible fields: one E2 table and output row per
-    // component), then the XCD-aware tile mapping: dispatch places block b on
-    // XCD b % 8 (the tile count per component is a multiple of 8 or the
-    // mapping below falls back to row-major).
+    // Block index = (K split, component, tile), tile fastest.  Component:
+    // incompressible fields have one E2 table and output row per component.
+    // K split (small grids): the split runs steps [split * steps_per, +steps_per)
+    // and writes its unscaled partial sum, added in a fixed order afterwards.
+    // Then the XCD-aware tile mapping: dispatch places block b on XCD b % 8.
     const int64_t ntile = g.row_tiles * g.col_tiles;
-    const int64_t comp = (int64_t)blockIdx.x / ntile;
-    const int64_t b = (int64_t)blockIdx.x - comp * ntile;
+    const int64_t rest = (int64_t)blockIdx.x / ntile;
+    const int64_t b = (int64_t)blockIdx.x - rest * ntile;
+    const int64_t comp = rest % comps, split = SPLIT ? rest / comps : 0;
     e2 += comp * g.e2_elems;
-    out += comp * out_stride;
+    out = SPLIT ? partial + rest * out_stride : out + comp * out_stride;
     int64_t rt, ct;
     {
         const int64_t xcd = b & 7, w = b >> 3;
@@ -286,13 +320,15 @@ __global__ __launch_bounds__(kMWaves * 64, 2) void k_mfma_grid(MfmaGeom g, doubl
     };
 
     const int64_t steps = g.Np / kMKM;
-    load_a(0);
+    const int64_t s0 = SPLIT ? split * steps_per : 0;
+    const int64_t s1 = SPLIT ? (s0 + steps_per < steps ? s0 + steps_per : steps) : steps;
+    load_a(s0);
     store_a(0);
     __syncthreads();
 
-    for (int64_t s = 0; s < steps; ++s) {
-        const int buf = (int)(s & 1);
-        if (s + 1 < steps) load_a(s + 1);
+    for (int64_t s = s0; s < s1; ++s) {
+        const int buf = (int)((s - s0) & 1);
+        if (s + 1 < s1) load_a(s + 1);
 
         // B fragments for this K step, built in registers.
         u32x4 bh[kMCF], bl[kMCF];
@@ -348,11 +384,11 @@ __global__ __launch_bounds__(kMWaves * 64, 2) void k_mfma_grid(MfmaGeom g, doubl
                 acc[rf][cf] = mfma<FP16>(al, bh[cf], acc[rf][cf]);
             }
         }
-        if (s + 1 < steps) store_a(buf ^ 1);
+        if (s + 1 < s1) store_a(buf ^ 1);
         __syncthreads();
     }
 
-    const float fa = (float)alpha;
+    const float fa = SPLIT ? 1.0f : (float)alpha;  // partial sums are scaled when added
 #pragma unroll
     for (int rf = 0; rf < kMRF; ++rf) {
 #pragma unroll
@@ -383,13 +419,15 @@ struct alignas(16) Stage3 {
     float e2[16 * 32];
 };
 
-template <bool FP16>
+template <bool FP16, bool SPLIT>
 __global__ __launch_bounds__(kMWaves * 64, 2) void k_mfma_grid3(MfmaGeom g, double alpha,
                                                                 const uint32_t* __restrict__ a_hi,
                                                                 const uint32_t* __restrict__ a_lo,
                                                                 const float2* __restrict__ e1,
                                                                 const float2* __restrict__ e2,
-                                                                float* __restrict__ out, int64_t out_stride) {
+                                                                float* __restrict__ out, int64_t out_stride,
+                                                                int64_t comps, int64_t steps_per,
+                                                                float* __restrict__ partial) {
     __shared__ Stage3 st[2];
     const int tid = threadIdx.x;
     const int lane = tid & 63;
@@ -397,11 +435,13 @@ __global__ __launch_bounds__(kMWaves * 64, 2) void k_mfma_grid3(MfmaGeom g, doub
     const int grp = lane >> 4;
     const int l16 = lane & 15;
 
-    const int64_t ntile = g.row_tiles * g.col_tiles;  // per component
-    const int64_t comp = (int64_t)blockIdx.x / ntile;
-    const int64_t b = (int64_t)blockIdx.x - comp * ntile;
+    // block index = (K split, component, tile), as in k_mfma_grid
+    const int64_t ntile = g.row_tiles * g.col_tiles;
+    const int64_t rest = (int64_t)blockIdx.x / ntile;
+    const int64_t b = (int64_t)blockIdx.x - rest * ntile;
+    const int64_t comp = rest % comps, split = SPLIT ? rest / comps : 0;
     e2 += comp * g.e2_elems;
-    out += comp * out_stride;
+    out = SPLIT ? partial + rest * out_stride : out + comp * out_stride;
     int64_t rt, ct;
     {
         const int64_t xcd = b & 7, w = b >> 3;
@@ -457,13 +497,15 @@ __global__ __launch_bounds__(kMWaves * 64, 2) void k_mfma_grid3(MfmaGeom g, doub
         for (int cf = 0; cf < kMCF; ++cf) acc[rf][cf] = f32x4{0.f, 0.f, 0.f, 0.f};
 
     const int64_t steps = g.Np / kMKM;
-    load(0);
+    const int64_t s0 = SPLIT ? split * steps_per : 0;
+    const int64_t s1 = SPLIT ? (s0 + steps_per < steps ? s0 + steps_per : steps) : steps;
+    load(s0);
     store(0);
     __syncthreads();
     const int zsw = l16 & 7;
-    for (int64_t s = 0; s < steps; ++s) {
-        const int buf = (int)(s & 1);
-        if (s + 1 < steps) load(s + 1);
+    for (int64_t s = s0; s < s1; ++s) {
+        const int buf = (int)((s - s0) & 1);
+        if (s + 1 < s1) load(s + 1);
 
         // B fragments: E2 of this lane's i2 (row l16) times E1 of each fragment's i1
         float2 zf[4];
@@ -508,11 +550,11 @@ __global__ __launch_bounds__(kMWaves * 64, 2) void k_mfma_grid3(MfmaGeom g, doub
                 acc[rf][cf] = mfma<FP16>(al, bh[cf], acc[rf][cf]);
             }
         }
-        if (s + 1 < steps) store(buf ^ 1);
+        if (s + 1 < s1) store(buf ^ 1);
         __syncthreads();
     }
 
-    const float fa = (float)alpha;
+    const float fa = SPLIT ? 1.0f : (float)alpha;  // partial sums are scaled when added
     const int64_t i2 = i2b + l16;
 #pragma unroll
     for (int cf = 0; cf < kMCF; ++cf) {
@@ -528,6 +570,16 @@ __global__ __launch_bounds__(kMWaves * 64, 2) void k_mfma_grid3(MfmaGeom g, doub
             }
         }
     }
+}
+
+// out[i] = alpha * sum_k partial[k * n + i], k in order (K-split partial sums)
+__global__ void k_mfma_ksum(const float* __restrict__ partial, int64_t ksplit, int64_t n, float alpha,
+                            float* __restrict__ out) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float acc = 0.0f;
+    for (int64_t k = 0; k < ksplit; ++k) acc += partial[k * n + i];
+    out[i] = alpha * acc;
 }
 
 __global__ void k_fill_zero_f32(float* __restrict__ p, int64_t n) {
@@ -548,19 +600,42 @@ int mfma_launch(const MfmaGeom& g, int64_t d, int64_t N, int64_t comps, const do
                        N, comps, ax0, ax1, ax2, cov, z1, z2, fac, a_hi, a_lo, e1, e2);
     int st = launch_status();
     if (st != GSX_OK) return st;
-    const int64_t blocks = comps * g.row_tiles * g.col_tiles;
+    const int64_t steps_per = mfma_steps_per_split(g, comps);
+    const int64_t ksplit = mfma_ksplit(g, comps);
+    const int64_t blocks = ksplit * comps * g.row_tiles * g.col_tiles;
     const int64_t stride = g.n0 * g.ncols;  // output values per component
+    float* partial = nullptr;
+    if (ksplit > 1)
+        partial = reinterpret_cast<float*>(static_cast<char*>(ws) +
+                                           ceil_div(mfma_table_bytes(g, comps), (int64_t)256) * 256);
+    const dim3 grid((unsigned)blocks), block(kMWaves * 64);
     if (g.g2 == 1) {
-        hipLaunchKernelGGL((k_mfma_grid3<FP16>), dim3((unsigned)blocks), dim3(kMWaves * 64), 0, s, g, alpha,
-                           a_hi, a_lo, e1, e2, out, stride);
-    } else {
-        if (d == 2)
-            hipLaunchKernelGGL((k_mfma_grid<FP16, 16, true>), dim3((unsigned)blocks), dim3(kMWaves * 64), 0, s, g,
-                               alpha, a_hi, a_lo, e1, e2, out, stride);
+        if (ksplit > 1)
+            hipLaunchKernelGGL((k_mfma_grid3<FP16, true>), grid, block, 0, s, g, alpha, a_hi, a_lo, e1, e2, out,
+                               stride, comps, steps_per, partial);
         else
-            hipLaunchKernelGGL((k_mfma_grid<FP16, 16, false>), dim3((unsigned)blocks), dim3(kMWaves * 64), 0, s, g,
-                               alpha, a_hi, a_lo, e1, e2, out, stride);
+            hipLaunchKernelGGL((k_mfma_grid3<FP16, false>), grid, block, 0, s, g, alpha, a_hi, a_lo, e1, e2, out,
+                               stride, comps, steps_per, partial);
+    } else if (d == 2) {
+        if (ksplit > 1)
+            hipLaunchKernelGGL((k_mfma_grid<FP16, 16, true, true>), grid, block, 0, s, g, alpha, a_hi, a_lo, e1, e2,
+                               out, stride, comps, steps_per, partial);
+        else
+            hipLaunchKernelGGL((k_mfma_grid<FP16, 16, true, false>), grid, block, 0, s, g, alpha, a_hi, a_lo, e1, e2,
+                               out, stride, comps, steps_per, partial);
+    } else {
+        if (ksplit > 1)
+            hipLaunchKernelGGL((k_mfma_grid<FP16, 16, false, true>), grid, block, 0, s, g, alpha, a_hi, a_lo, e1, e2,
+                               out, stride, comps, steps_per, partial);
+        else
+            hipLaunchKernelGGL((k_mfma_grid<FP16, 16, false, false>), grid, block, 0, s, g, alpha, a_hi, a_lo, e1,
+                               e2, out, stride, comps, steps_per, partial);
     }
+    st = launch_status();
+    if (st != GSX_OK || ksplit == 1) return st;
+    const int64_t n = comps * stride;
+    hipLaunchKernelGGL(k_mfma_ksum, dim3((unsigned)ceil_div(n, (int64_t)256)), dim3(256), 0, s, partial, ksplit, n,
+                       (float)alpha, out);
     return launch_status();
 }
 
@@ -585,7 +660,9 @@ int mfma_entry(int64_t d, int64_t N, const int64_t* shape, const double* axes, c
         return launch_status();
     }
     const MfmaGeom g = mfma_geom(d, N, shape);
-    if (comps * g.row_tiles * g.col_tiles > 0x7fffffffLL) return GSX_E_INVALID;
+    if (mfma_ksplit(g, comps) * comps * g.row_tiles * g.col_tiles > 0x7fffffffLL ||
+        ceil_div(comps * P, (int64_t)256) > 0x7fffffffLL)
+        return GSX_E_INVALID;
     if (!workspace || workspace_bytes < mfma_ws_bytes(g, comps)) return GSX_E_WORKSPACE;
     const double* ax0 = axes;
     const double* ax1 = axes + shape[0];

@@ -105,3 +105,19 @@ def test_device_trig_matches_mpmath(devmath):
     assert np.max(np.abs(cs - ref_cs)) < 1e-15
     assert np.max(np.abs(s - ref_s)) < 3e-16
     assert np.max(np.abs(c - ref_c)) < 3e-16
+
+
+@pytest.mark.parametrize("fn", ["gsx_summate_grid_mfma_workspace", "gsx_summate_incompr_grid_mfma_workspace"])
+def test_mfma_workspace_queries(fn):
+    """Workspace sizes are host arithmetic: no modes, empty axes, small grids (whose K
+    steps are split over blocks, adding a partial-sum buffer) and large grids."""
+    q = getattr(_native.lib(), fn)
+    shp = lambda *s: (ctypes.c_int64 * 3)(*(list(s) + [1] * (3 - len(s))))
+    assert q(3, 0, shp(8, 8, 8)) >= 0  # N = 0: no K steps
+    assert q(3, 64, shp(0, 8, 8)) >= 0 and q(2, 64, shp(16, 0)) >= 0  # empty axes
+    small = q(3, 4096, shp(40, 48, 48))
+    tables = 2 * (128 * 4096) * 2 * 2 + (48 * 4096 + (3 if "incompr" in fn else 1) * 48 * 4096) * 8
+    assert small > tables  # the partial sums come after the tables
+    big = q(3, 4096, shp(512, 512, 512))
+    assert big == 2 * (512 * 4096) * 2 * 2 + (512 * 4096 + (3 if "incompr" in fn else 1) * 512 * 4096) * 8
+    assert q(3, -1, shp(8, 8, 8)) < 0 and q(1, 8, shp(8)) < 0
