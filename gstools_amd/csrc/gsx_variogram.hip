@@ -133,7 +133,10 @@ __device__ __forceinline__ int bin_from_guess(const double* e, int B, double x, 
 // would be if-converted into computing both estimators for every pair).
 // HAV: (lat, lon) points in degrees, great-circle distances (general loop, DIM 2,
 // a third staged row holds cos(lat)).
-template <int DIM, bool FAST, bool CRESSIE, bool HAV = false>
+// DIRF: directional estimate of one field with non-decreasing edges: waves whose
+// points and staged tile are all finite run a straight-line loop over four pairs
+// per step and the directions in turn (others take the general loop).
+template <int DIM, bool FAST, bool CRESSIE, bool HAV = false, bool DIRF = false>
 __global__ __launch_bounds__(kVT) void k_vario_pairs(VarioArgs a, const double* __restrict__ f,
                                                      const double* __restrict__ edges,
                                                      const double* __restrict__ pos,
@@ -155,6 +158,7 @@ __global__ __launch_bounds__(kVT) void k_vario_pairs(VarioArgs a, const double* 
     double* s_tol = s_thp + (a.B + 3);
     double* s_dirs = s_tol + 2;
     static_assert(!HAV || (DIM == 2 && !FAST), "haversine distances: 2D, general loop");
+    static_assert(!DIRF || (!FAST && !HAV), "directional straight-line loop: its own variant");
     constexpr int NPOS = HAV ? DIM + 1 : DIM;  // staged coordinate rows
     // staged tile: coordinates then field values, rows of kVT, then kVPad zeros;
     // a read up to 3 entries past a row lands in the next row or in the pad
@@ -347,6 +351,99 @@ __global__ __launch_bounds__(kVT) void k_vario_pairs(VarioArgs a, const double* 
                         run(std::integral_constant<bool, false>{}, std::integral_constant<bool, true>{});
                     else
                         run(std::integral_constant<bool, false>{}, std::integral_constant<bool, false>{});
+                }
+                continue;
+            }
+        }
+        if constexpr (DIRF) {
+            const double fj0 = fj[0];
+            const bool clean = !tile_nan && !__any((xj_nan || !__builtin_isfinite(fj0) || a.F != 1) ? 1 : 0);
+            if (mono && clean) {
+                // Finite points: no NaN distance or value.  Per pair: the bin from
+                // the squared distance and the edges' thresholds (exact, as in the
+                // omnidirectional loop), dist = sqrt(d2) for the angle test; per
+                // direction the band / angle tests on their thresholds, then one
+                // histogram slot per pair updated as in the omnidirectional loop
+                // (scratch row NH, carried values, writes in pair order).  With
+                // separate directions the first matching direction takes the pair.
+                const int kend = (int)kn;
+                const bool band = a.bandwidth > 0.0;
+                const double qtol = s_tol[0], btol = s_tol[1];
+                for (int k4 = kstart; k4 < kend; k4 += 4) {
+                    double dv[4][DIM], dist[4], val[4];
+                    int g[4];
+                    bool taken[4];
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) {
+                        const int kk = k4 + u;  // past kend: staged values or the pad
+                        double d2 = 0.0;
+#pragma unroll
+                        for (int c = 0; c < DIM; ++c) {
+                            dv[u][c] = s_pos[c * kVT + kk] - xj[c];
+                            d2 += dv[u][c] * dv[u][c];
+                        }
+                        const double x = s_f[kk] - fj0;
+                        val[u] = CRESSIE ? sqrt(fabs(x)) : x * x;
+                        int gu;
+                        if (uniform) {
+                            float q = __builtin_fmaf(__builtin_amdgcn_sqrtf((float)d2), inv_wf, qoff);
+                            q = __builtin_amdgcn_fmed3f(q, -1.0f, (float)Bi);
+                            const int g0 = (int)q;
+                            gu = g0 - (s_thp[g0 + 1] > d2 ? 1 : 0) + (s_thp[g0 + 2] <= d2 ? 1 : 0);
+                        } else {
+                            gu = (int)upper_edge(s_thr, a.B, d2);
+                        }
+                        g[u] = (kk < kend && (unsigned)gu < (unsigned)Bi) ? gu : -1;
+                        dist[u] = sqrt(d2);
+                        taken[u] = false;
+                    }
+                    for (int64_t d = d_first; d <= d_last; ++d) {
+                        const double* ud = s_dirs + d * DIM;
+                        int slot[4];
+#pragma unroll
+                        for (int u = 0; u < 4; ++u) {
+                            double sp = 0.0;
+#pragma unroll
+                            for (int c = 0; c < DIM; ++c) sp += dv[u][c] * ud[c];
+                            bool in = g[u] >= 0 && !taken[u];
+                            if (band) {
+                                double bd = 0.0;
+#pragma unroll
+                                for (int c = 0; c < DIM; ++c) {
+                                    const double r = dv[u][c] - sp * ud[c];
+                                    bd += r * r;
+                                }
+                                in = in && bd < btol;  // sqrt(bd) < bandwidth
+                            }
+                            if (dist[u] > 0.0) {
+                                const double q = fabs(sp) / dist[u];
+                                in = in && (!(q < 1.0) || q >= qtol);  // acos(q) < angles_tol
+                            }
+                            if (a.separate) taken[u] = taken[u] || in;
+                            const unsigned sl = (unsigned)(d * a.B + g[u] - a.h0);
+                            slot[u] = (in && sl < (unsigned)NHi) ? (int)sl : NHi;
+                        }
+                        double hs[4];
+                        unsigned hc[4];
+#pragma unroll
+                        for (int u = 0; u < 4; ++u) {
+                            hs[u] = h_sum[slot[u] * kVT + t];
+                            hc[u] = h_cnt[slot[u] * kVT + t];
+                        }
+#pragma unroll
+                        for (int u = 1; u < 4; ++u)
+#pragma unroll
+                            for (int w2 = 0; w2 < u; ++w2) {
+                                const bool m = slot[u] == slot[w2];
+                                hs[u] = m ? hs[w2] + val[w2] : hs[u];
+                                hc[u] = m ? hc[w2] + 1u : hc[u];
+                            }
+#pragma unroll
+                        for (int u = 0; u < 4; ++u) {
+                            h_sum[slot[u] * kVT + t] = hs[u] + val[u];
+                            h_cnt[slot[u] * kVT + t] = hc[u] + 1u;
+                        }
+                    }
                 }
                 continue;
             }
@@ -558,16 +655,16 @@ inline int64_t vario_slots_per_launch(size_t limit, int64_t B, int64_t D, int di
     return (int64_t)((limit - fixed) / ((sizeof(double) + sizeof(unsigned)) * kVT));
 }
 
-template <int DIM, bool FAST, bool CRESSIE, bool HAV = false>
+template <int DIM, bool FAST, bool CRESSIE, bool HAV = false, bool DIRF = false>
 int launch_pairs_t(const VarioArgs& a, size_t lds, int64_t blocks, hipStream_t s, const double* f,
                    const double* edges, const double* pos, const double* dirs, double* part_sum,
                    unsigned long long* part_cnt) {
     if (lds > 65536 &&
-        hipFuncSetAttribute(reinterpret_cast<const void*>(&k_vario_pairs<DIM, FAST, CRESSIE, HAV>),
+        hipFuncSetAttribute(reinterpret_cast<const void*>(&k_vario_pairs<DIM, FAST, CRESSIE, HAV, DIRF>),
                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) != hipSuccess)
         return GSX_E_LAUNCH;
-    hipLaunchKernelGGL((k_vario_pairs<DIM, FAST, CRESSIE, HAV>), dim3((unsigned)blocks), dim3(kVT), lds, s, a, f,
-                       edges, pos, dirs, part_sum, part_cnt);
+    hipLaunchKernelGGL((k_vario_pairs<DIM, FAST, CRESSIE, HAV, DIRF>), dim3((unsigned)blocks), dim3(kVT), lds, s, a,
+                       f, edges, pos, dirs, part_sum, part_cnt);
     return launch_status();
 }
 
@@ -579,6 +676,13 @@ int launch_pairs(const VarioArgs& a, size_t lds, int64_t blocks, hipStream_t s, 
         if (a.cressie)
             return launch_pairs_t<DIM, true, true>(a, lds, blocks, s, f, edges, pos, dirs, part_sum, part_cnt);
         return launch_pairs_t<DIM, true, false>(a, lds, blocks, s, f, edges, pos, dirs, part_sum, part_cnt);
+    }
+    if (a.directional && a.F == 1) {
+        if (a.cressie)
+            return launch_pairs_t<DIM, false, true, false, true>(a, lds, blocks, s, f, edges, pos, dirs, part_sum,
+                                                                 part_cnt);
+        return launch_pairs_t<DIM, false, false, false, true>(a, lds, blocks, s, f, edges, pos, dirs, part_sum,
+                                                              part_cnt);
     }
     return launch_pairs_t<DIM, false, false>(a, lds, blocks, s, f, edges, pos, dirs, part_sum, part_cnt);
 }

@@ -113,6 +113,26 @@ def test_directional_matches_oracle(separate, bandwidth):
     _close(g, g_ref)
 
 
+@pytest.mark.parametrize("separate,bandwidth", [(False, -1.0), (True, 1.2), (False, 0.7)])
+def test_directional_one_field_matches_oracle(separate, bandwidth):
+    """One field (the straight-line directional loop): uniform and non-uniform edges,
+    with NaN values in some waves (those take the general loop), duplicate points."""
+    _say(f"vario directional one field separate={separate} bandwidth={bandwidth}")
+    rs = np.random.RandomState(17)
+    pos = rs.uniform(0, 10, size=(3, 2000))
+    pos[:, 5] = pos[:, 6]  # a distance-0 pair
+    f = rs.normal(size=2000)
+    f[1500:1503] = np.nan
+    dirs = rs.normal(size=(3, 3))
+    dirs /= np.linalg.norm(dirs, axis=1)[:, None]
+    for edges in (np.linspace(0, 9, 13), np.array([0.0, 0.5, 0.6, 2.0, 4.5, 8.0])):
+        for est in ("m", "c"):
+            g, c = v.directional(f, edges, pos, dirs, np.pi / 7, bandwidth, separate, est)
+            g_ref, c_ref = oracle.vario_directional_c(f, edges, pos, dirs, np.pi / 7, bandwidth, separate, est)
+            assert np.array_equal(c, c_ref)
+            _close(g, g_ref)
+
+
 def test_directional_slot_chunks():
     """(direction, bin) slots beyond one launch's LDS: launches over slot ranges."""
     _say("vario directional many slots")
