@@ -34,7 +34,7 @@
 namespace gsx {
 
 constexpr int kVT = 256;        // points per tile (threads per block)
-constexpr int kVPad = 4;        // zero doubles after the staged field rows
+constexpr int kVPad = 8;        // zero doubles after the staged field rows
 constexpr int kVFields = 8;     // fields per launch
 constexpr int kVMaxDim = 3;
 constexpr int kVBlocks = 2048;  // persistent blocks
@@ -161,7 +161,7 @@ __global__ __launch_bounds__(kVT) void k_vario_pairs(VarioArgs a, const double* 
     static_assert(!DIRF || (!FAST && !HAV), "directional straight-line loop: its own variant");
     constexpr int NPOS = HAV ? DIM + 1 : DIM;  // staged coordinate rows
     // staged tile: coordinates then field values, rows of kVT, then kVPad zeros;
-    // a read up to 3 entries past a row lands in the next row or in the pad
+    // a read up to 7 entries past a row lands in the next row or in the pad
     double* s_pos = s_dirs + a.D * DIM;                     // [NPOS][kVT]
     double* s_f = s_pos + NPOS * kVT;                       // [F][kVT]
     double* h_sum = s_f + a.F * kVT + kVPad;                // [NH + 1][kVT], row NH scratch
@@ -266,15 +266,18 @@ __global__ __launch_bounds__(kVT) void k_vario_pairs(VarioArgs a, const double* 
                 auto run = [&](auto uni, auto clean) {
                     constexpr bool UNI = decltype(uni)::value;
                     constexpr bool CLEAN = decltype(clean)::value;
-                    for (int k4 = kstart; k4 < kend; k4 += 4) {
-                        int slot[4];
-                        double val[4];
+                    // eight pairs per iteration: their bins are found together (the
+                    // threshold reads overlap), then two updates of four slots each,
+                    // in order (the second group's reads see the first group's writes)
+                    for (int k8 = kstart; k8 < kend; k8 += 8) {
+                        int slot[8];
+                        double val[8];
 #pragma unroll
-                        for (int u = 0; u < 4; ++u) {
-                            // past kend (at most 3 entries) the reads see staged values,
+                        for (int u = 0; u < 8; ++u) {
+                            // past kend (at most 7 entries) the reads see staged values,
                             // zeros or the pad, all finite when CLEAN; such pairs go to
                             // the scratch row
-                            const int kk = k4 + u;
+                            const int kk = k8 + u;
                             const bool live = kk < kend;
                             double d2 = 0.0;
 #pragma unroll
@@ -318,25 +321,28 @@ __global__ __launch_bounds__(kVT) void k_vario_pairs(VarioArgs a, const double* 
                             slot[u] = (ok && sl < (unsigned)NHi) ? (int)sl : NHi;
                             val[u] = v;
                         }
-                        double hs[4];
-                        unsigned hc[4];
 #pragma unroll
-                        for (int u = 0; u < 4; ++u) {
-                            hs[u] = h_sum[slot[u] * kVT + t];
-                            hc[u] = h_cnt[slot[u] * kVT + t];
-                        }
+                        for (int grp = 0; grp < 8; grp += 4) {
+                            double hs[4];
+                            unsigned hc[4];
 #pragma unroll
-                        for (int u = 1; u < 4; ++u)
-#pragma unroll
-                            for (int w2 = 0; w2 < u; ++w2) {
-                                const bool m = slot[u] == slot[w2];
-                                hs[u] = m ? hs[w2] + val[w2] : hs[u];
-                                hc[u] = m ? hc[w2] + 1u : hc[u];
+                            for (int u = 0; u < 4; ++u) {
+                                hs[u] = h_sum[slot[grp + u] * kVT + t];
+                                hc[u] = h_cnt[slot[grp + u] * kVT + t];
                             }
 #pragma unroll
-                        for (int u = 0; u < 4; ++u) {
-                            h_sum[slot[u] * kVT + t] = hs[u] + val[u];
-                            h_cnt[slot[u] * kVT + t] = hc[u] + 1u;
+                            for (int u = 1; u < 4; ++u)
+#pragma unroll
+                                for (int w2 = 0; w2 < u; ++w2) {
+                                    const bool m = slot[grp + u] == slot[grp + w2];
+                                    hs[u] = m ? hs[w2] + val[grp + w2] : hs[u];
+                                    hc[u] = m ? hc[w2] + 1u : hc[u];
+                                }
+#pragma unroll
+                            for (int u = 0; u < 4; ++u) {
+                                h_sum[slot[grp + u] * kVT + t] = hs[u] + val[grp + u];
+                                h_cnt[slot[grp + u] * kVT + t] = hc[u] + 1u;
+                            }
                         }
                     }
                 };
