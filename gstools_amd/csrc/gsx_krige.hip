@@ -9,13 +9,14 @@
 // error into the kriging variance max(sill - error, 0).
 //
 // Variance form: W = M V is an FP64 GEMM (2 K^2 P flops) with an epilogue
-// that never stores W: every block owns a 64-row x 128-point tile of W,
+// that never stores W: every block owns a 128-row x 128-point tile of W,
 // accumulates it on the matrix cores (v_mfma_f64_16x16x4, 4 waves in a 2 x 2
-// arrangement, each 32 rows x 64 points), and reduces V o W and cond o W over
-// its 64 rows into per-row-tile partial sums.  A second kernel adds the
-// partials of the row tiles in a fixed order (bitwise reproducible).  Operand
-// tiles (64 x 16 of M, 16 x 128 of V) are staged through LDS with the next
-// stage prefetched into registers.
+// arrangement, each 64 rows x 64 points, 16 MFMA tiles in 227 registers at
+// 2 waves per SIMD), and reduces V o W and cond o W over its 128 rows into
+// per-row-tile partial sums.  A second kernel adds the partials of the row
+// tiles in a fixed order (bitwise reproducible).  Operand tiles (128 x 16 of
+// M, 16 x 128 of V) are staged through LDS with the next stage prefetched into
+// registers.
 //
 // Field-only form: field = (M^T cond) . v_k, a weight vector (K) followed by
 // an HBM-bound pass over V (no GEMM).
@@ -38,7 +39,8 @@ namespace gsx {
 
 typedef __attribute__((ext_vector_type(4))) double f64x4;
 
-constexpr int kKrRows = 64;   // W rows per block
+constexpr int kKrRows = 128;  // W rows per block
+constexpr int kKrRF = kKrRows / 32;  // 16-row MFMA tiles per wave (two row waves)
 constexpr int kKrPts = 128;   // points per block
 constexpr int kKrK = 16;      // reduction depth per LDS stage
 constexpr int kKrApad = kKrK + 1;
@@ -46,7 +48,7 @@ constexpr int kKrApad = kKrK + 1;
 // (rows 144 doubles apart) fall in disjoint LDS banks
 constexpr int kKrBpad = kKrPts + 16;
 
-__global__ __launch_bounds__(256) void k_krige_gemm(int64_t K, int64_t P, int64_t row_tiles, int64_t col_tiles,
+__global__ __launch_bounds__(256, 2) void k_krige_gemm(int64_t K, int64_t P, int64_t row_tiles, int64_t col_tiles,
                                                     const double* __restrict__ M,
                                                     const double* __restrict__ V,
                                                     const double* __restrict__ cond,
@@ -76,11 +78,12 @@ __global__ __launch_bounds__(256) void k_krige_gemm(int64_t K, int64_t P, int64_
     const int64_t i0 = rt_idx * kKrRows;
     const int64_t p0 = ct_idx * kKrPts;
 
-    // staging assignment: A 64 x kKrK (kKrK / 4 consecutive per thread); B kKrK x 128
+    // staging assignment: A kKrRows x kKrK (kAU consecutive per thread); B kKrK x 128
     // (kKrK / 2 per thread: point t % 128 of rows t / 128 + 2u, so a wave's loads
     // and LDS stores are unit-stride and the stores conflict-free)
-    constexpr int kAU = kKrK / 4, kBU = kKrK / 2;
-    const int ar = t >> 2, ac = (t & 3) * kAU;
+    constexpr int kAT = 256 / kKrRows;  // threads per A row
+    constexpr int kAU = kKrK / kAT, kBU = kKrK / 2;
+    const int ar = t / kAT, ac = (t % kAT) * kAU;
     const int br = t >> 7, bc = t & 127;
     double pa[kAU], pb[kBU];
     auto load = [&](int64_t j0) {
@@ -104,9 +107,9 @@ __global__ __launch_bounds__(256) void k_krige_gemm(int64_t K, int64_t P, int64_
         for (int u = 0; u < kBU; ++u) Bs[(br + 2 * u) * kKrBpad + bc] = pb[u];
     };
 
-    f64x4 acc[2][4];
+    f64x4 acc[kKrRF][4];
 #pragma unroll
-    for (int rt = 0; rt < 2; ++rt)
+    for (int rt = 0; rt < kKrRF; ++rt)
 #pragma unroll
         for (int ct = 0; ct < 4; ++ct) acc[rt][ct] = f64x4{0.0, 0.0, 0.0, 0.0};
 
@@ -118,30 +121,30 @@ __global__ __launch_bounds__(256) void k_krige_gemm(int64_t K, int64_t P, int64_
         if (j0 + kKrK < K) load(j0 + kKrK);
 #pragma unroll
         for (int kk = 0; kk < kKrK; kk += 4) {
-            double a[2], b[4];
+            double a[kKrRF], b[4];
 #pragma unroll
-            for (int rt = 0; rt < 2; ++rt) a[rt] = As[(wr * 32 + rt * 16 + l16) * kKrApad + kk + g];
+            for (int rt = 0; rt < kKrRF; ++rt) a[rt] = As[(wr * (kKrRows / 2) + rt * 16 + l16) * kKrApad + kk + g];
 #pragma unroll
             for (int ct = 0; ct < 4; ++ct) b[ct] = Bs[(kk + g) * kKrBpad + wc * 64 + ct * 16 + l16];
 #pragma unroll
-            for (int rt = 0; rt < 2; ++rt)
+            for (int rt = 0; rt < kKrRF; ++rt)
 #pragma unroll
                 for (int ct = 0; ct < 4; ++ct)
                     acc[rt][ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[rt], b[ct], acc[rt][ct], 0, 0, 0);
         }
     }
 
-    // epilogue: per column, sum over this wave's 32 rows of cond o W and V o W
+    // epilogue: per column, sum over this wave's kKrRows / 2 rows of cond o W and V o W
     double sf[4], se[4];
 #pragma unroll
     for (int ct = 0; ct < 4; ++ct) {
         const int64_t p = p0 + wc * 64 + ct * 16 + l16;
         double f = 0.0, e = 0.0;
 #pragma unroll
-        for (int rt = 0; rt < 2; ++rt)
+        for (int rt = 0; rt < kKrRF; ++rt)
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                const int64_t i = i0 + wr * 32 + rt * 16 + g + 4 * r;
+                const int64_t i = i0 + wr * (kKrRows / 2) + rt * 16 + g + 4 * r;
                 if (i < K && p < P) {
                     const double w = acc[rt][ct][r];
                     f += cond[i] * w;
