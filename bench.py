@@ -509,6 +509,26 @@ def bench_batch(torch, steps, warmup, specs, modes, sampling_info=None):
         ref = oracle.summate_incompr_c(cov, z1, z2, p) if it.incompr else oracle.summate_c(cov, z1, z2, p)[None]
         errs.append(oracle.rel_errors(batched[r][:, idx].cpu().numpy().astype(np.float64), ref)[0])
     plan.close()
+    # the same batch with the grid items (2D and incompressible 3D) on the matrix cores
+    from gstools_amd.batch import MixedBatch
+    mb = MixedBatch(items, "bf16x3", *best)
+    mixed_wall, _ = time_kernel(lambda: mb.launch(graph=True), steps, warmup, torch)
+    mixed_errs = []
+    for r in (0, 130, 250):
+        cov, z1, z2 = modes[r]
+        it = items[r]
+        idx = np.arange(min(2000, it.P))
+        if it.axes is not None:
+            ax = [a.cpu().numpy() for a in torch.split(it.axes, list(it.shape[:it.d]))]
+            p = np.asarray(np.meshgrid(*ax, indexing="ij")).reshape(it.d, -1)[:, idx]
+        else:
+            p = it.pos.cpu().numpy()[:, idx]
+        ref = oracle.summate_incompr_c(cov, z1, z2, p) if it.incompr else oracle.summate_c(cov, z1, z2, p)[None]
+        got = it.out[:, idx].cpu().numpy().astype(np.float64)
+        mixed_errs.append(max(oracle.rel_errors(got[c], ref[c])[1] for c in range(got.shape[0])))
+    mixed = {"precision": "bf16x3 grids (128 2D + 32 incompressible 3D), fp32 point sets, one HIP graph",
+             "graph_ms": mixed_wall * 1e3, "value": work / mixed_wall, "speedup_vs_fp32_graph": graph_wall / mixed_wall,
+             "rel_rms_vs_oracle_spot": mixed_errs}
     return {
         "workload": "256 realizations: 128 2D Gaussian grids (side log-U[64,2048]), 96 3D Exponential point "
                     "sets (log-U[1e4,1e6]), 32 incompressible 3D Gaussian 128^3 grids; mode_no in "
@@ -525,6 +545,7 @@ def bench_batch(torch, steps, warmup, specs, modes, sampling_info=None):
         "speedup_vs_per_realization": per_wall / graph_wall,
         "batched_equals_single_launches": bool(same),
         "rel_max_vs_oracle_spot": errs,
+        "mixed_matrix_cores": mixed,
         "mode_sampling": sampling_info,
     }
 

@@ -18,7 +18,7 @@ import torch
 
 from . import _native
 
-__all__ = ["BatchItem", "BatchPlan", "autotune", "TILE_VARIANTS"]
+__all__ = ["BatchItem", "BatchPlan", "MixedBatch", "autotune", "TILE_VARIANTS"]
 
 TILE_VARIANTS = [(512, 128), (512, 256), (1024, 128), (1024, 256), (2048, 128), (2048, 256)]
 
@@ -100,6 +100,65 @@ class BatchPlan:
             self.close()
         except Exception:
             pass
+
+
+class MixedBatch:
+    """Grid items (d = 2, 3; scalar or incompressible) on the matrix cores, the other
+    items in one persistent FP32 launch; optionally captured once and replayed as a
+    HIP graph.
+
+    Grid items run ``gsx_summate_grid_mfma`` / ``gsx_summate_incompr_grid_mfma`` in
+    split bf16 ("bf16x3") or fp16 ("fp16x3") with one shared workspace (the launches
+    are ordered on one stream); outputs are the unscaled FP32 sums, as in
+    :class:`BatchPlan`.  The graph holds kernel launches only (no memset nodes).
+    """
+
+    def __init__(self, items, precision="bf16x3", tile_points=0, mode_chunk=0):
+        if precision not in ("bf16x3", "fp16x3"):
+            raise ValueError("precision must be 'bf16x3' or 'fp16x3'")
+        self.items = list(items)
+        self.variant = _native.MFMA_BF16X3 if precision == "bf16x3" else _native.MFMA_FP16X3
+        self.grid_items = [it for it in self.items if it.axes is not None and it.d >= 2]
+        rest = [it for it in self.items if not (it.axes is not None and it.d >= 2)]
+        self.plan = BatchPlan(rest, tile_points, mode_chunk) if rest else None
+        lib = _native.lib()
+        self._calls = []
+        need = 1
+        for it in self.grid_items:
+            shape = (ctypes.c_int64 * 3)(*it.shape)
+            q = lib.gsx_summate_incompr_grid_mfma_workspace if it.incompr else lib.gsx_summate_grid_mfma_workspace
+            nbytes = int(q(it.d, it.N, shape))
+            if nbytes < 0:
+                _native.check(nbytes, "mixed batch workspace")
+            need = max(need, nbytes)
+            self._calls.append((it, shape))
+        self.ws = torch.empty(need, dtype=torch.uint8, device=self.items[0].out.device if self.items else "cuda")
+        self.graph = None
+
+    def _run(self):
+        lib = _native.lib()
+        s = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+        ws, nws = ctypes.c_void_p(self.ws.data_ptr()), self.ws.numel()
+        for it, shape in self._calls:
+            fn = lib.gsx_summate_incompr_grid_mfma if it.incompr else lib.gsx_summate_grid_mfma
+            _native.check(fn(it.d, it.N, shape, it.axes.data_ptr(), it.cov.data_ptr(), it.z1.data_ptr(),
+                             it.z2.data_ptr(), self.variant, 1.0, it.out.data_ptr(), ws, nws, s),
+                          "mixed batch grid item")
+        if self.plan is not None:
+            self.plan.launch(graph=False)
+
+    def launch(self, graph=False):
+        if not graph:
+            self._run()
+            return
+        if self.graph is None:
+            self._run()  # eager warm-up before the capture
+            torch.cuda.synchronize()
+            g = torch.cuda.CUDAGraph()
+            with torch.cuda.graph(g):
+                self._run()
+            self.graph = g
+        self.graph.replay()
 
 
 def _cache_path():

@@ -15,7 +15,7 @@ import torch
 
 import oracle
 from gstools_amd import summate_device, summate_grid_device, summate_incompr_device
-from gstools_amd.batch import TILE_VARIANTS, BatchItem, BatchPlan, autotune
+from gstools_amd.batch import TILE_VARIANTS, BatchItem, BatchPlan, MixedBatch, autotune
 from gstools_amd.geometry import generate_grid
 
 pytestmark = pytest.mark.gpu
@@ -89,6 +89,37 @@ def test_batch_graph_replays():
         for it, e in zip(items, eager):
             assert torch.equal(it.out, e), f"replay {rep} differs"
     plan.close()
+
+
+@pytest.mark.parametrize("precision", ["bf16x3", "fp16x3"])
+def test_mixed_batch_grids_on_matrix_cores(precision):
+    """MixedBatch: the 2D / 3D grid items (scalar and incompressible) on the matrix
+    cores within 1e-5 (rms) of FP64, the others bitwise equal to their single FP32
+    launches; graph replays (with eager work between) equal the eager run."""
+    _say(f"mixed batch {precision}")
+    items, refs = _mixed_items(seed=2)
+    mb = MixedBatch(items, precision)
+    assert len(mb.grid_items) == 2
+    mb.launch()
+    torch.cuda.synchronize()
+    eager = [it.out.clone() for it in items]
+    for it, (cov, z1, z2, p, axes, incompr) in zip(items, refs):
+        got = it.out.cpu().numpy().astype(np.float64)
+        ref = oracle.summate_incompr_c(cov, z1, z2, p) if incompr else oracle.summate_c(cov, z1, z2, p)[None]
+        if it in mb.grid_items:
+            for c in range(got.shape[0]):
+                assert oracle.rel_errors(got[c], ref[c])[1] <= 1e-5, (precision, c)
+        else:
+            assert np.array_equal(it.out.cpu().numpy(), _single(cov, z1, z2, p, axes, incompr))
+    for rep in range(3):
+        for it in items:
+            it.out.fill_(float("nan"))
+        _ = summate_device(refs[1][0], refs[1][1], refs[1][2], refs[1][3], "fp32")
+        torch.cuda.synchronize()
+        mb.launch(graph=True)
+        torch.cuda.synchronize()
+        for it, e in zip(items, eager):
+            assert torch.equal(it.out, e), f"replay {rep} differs"
 
 
 def test_batch_empty_and_invalid():
