@@ -8,7 +8,7 @@ import ctypes
 import os
 import threading
 
-__all__ = ["lib", "check", "GsxError", "LIB_PATH", "EXPORTS", "MFMA_BF16X3", "MFMA_FP16X3"]
+__all__ = ["lib", "check", "GsxError", "LIB_PATH", "EXPORTS", "MFMA_BF16X3", "MFMA_FP16X3", "MFMA_UNIFORM_LAST"]
 
 LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libgsx.so")
 
@@ -20,6 +20,7 @@ GSX_E_WORKSPACE = -4
 GSX_E_UNSUPPORTED = -5
 MFMA_BF16X3 = 0
 MFMA_FP16X3 = 1
+MFMA_UNIFORM_LAST = 16  # flag: the last axis of a 2D grid is uniformly spaced
 
 _vp = ctypes.c_void_p
 _i64 = ctypes.c_int64

@@ -17,6 +17,7 @@ import numpy as np
 import torch
 
 from . import _native
+from .summate import uniform_axis
 
 __all__ = ["BatchItem", "BatchPlan", "MixedBatch", "autotune", "TILE_VARIANTS"]
 
@@ -131,7 +132,10 @@ class MixedBatch:
             if nbytes < 0:
                 _native.check(nbytes, "mixed batch workspace")
             need = max(need, nbytes)
-            self._calls.append((it, shape))
+            variant = self.variant
+            if it.d == 2 and not it.incompr and uniform_axis(it.axes[it.shape[0]:it.shape[0] + it.shape[1]]):
+                variant |= _native.MFMA_UNIFORM_LAST  # factorised last axis
+            self._calls.append((it, shape, variant))
         self.ws = torch.empty(need, dtype=torch.uint8, device=self.items[0].out.device if self.items else "cuda")
         self.graph = None
 
@@ -139,10 +143,10 @@ class MixedBatch:
         lib = _native.lib()
         s = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
         ws, nws = ctypes.c_void_p(self.ws.data_ptr()), self.ws.numel()
-        for it, shape in self._calls:
+        for it, shape, variant in self._calls:
             fn = lib.gsx_summate_incompr_grid_mfma if it.incompr else lib.gsx_summate_grid_mfma
             _native.check(fn(it.d, it.N, shape, it.axes.data_ptr(), it.cov.data_ptr(), it.z1.data_ptr(),
-                             it.z2.data_ptr(), self.variant, 1.0, it.out.data_ptr(), ws, nws, s),
+                             it.z2.data_ptr(), variant, 1.0, it.out.data_ptr(), ws, nws, s),
                           "mixed batch grid item")
         if self.plan is not None:
             self.plan.launch(graph=False)

@@ -51,12 +51,12 @@ constexpr int kMWaves = 4;
 constexpr int kMCF = 4;              // 16-column fragments per wave
 constexpr int kMRF = kMBM / 16;      // 16-row fragments per wave (all block rows)
 constexpr int kMKM = 16;             // modes per K step
-constexpr int kMLdsRow = 2 * kMKM + 8;  // halfwords per LDS row (16 B pad)
 
 struct MfmaGeom {
     int64_t n0, n1, n2;   // grid (n1 = 1 for 2D grids)
     int64_t n0p, Np;      // padded rows / modes
     int64_t ncols;        // n1 * n2
+    int64_t out_cols;     // output columns per row: n1 * n2, or the split axis' length
     int g2;               // i2 fragments per column group (1 or 16)
     int64_t ti1, ti2;     // column tile extent along i1 / i2
     int64_t t1, t2;       // column tiles along i1 / i2
@@ -72,6 +72,7 @@ inline MfmaGeom mfma_geom(int64_t d, int64_t N, const int64_t* shape) {
     g.n0p = ceil_div(g.n0, kMBM) * kMBM;
     g.Np = ceil_div(N, kMKM) * kMKM;
     g.ncols = g.n1 * g.n2;
+    g.out_cols = g.ncols;
     g.g2 = (g.n1 >= 16 && g.n2 >= 16) ? 1 : 16;
     g.ti1 = 16 / g.g2;
     g.ti2 = 16 * (int64_t)g.g2;
@@ -116,7 +117,28 @@ inline int64_t mfma_table_bytes(const MfmaGeom& g, int64_t comps) {
 inline int64_t mfma_ws_bytes(const MfmaGeom& g, int64_t comps = 1) {
     const int64_t ks = mfma_ksplit(g, comps);
     const int64_t tables = ceil_div(mfma_table_bytes(g, comps), (int64_t)256) * 256;
-    return ks > 1 ? tables + ks * comps * g.n0 * g.ncols * 4 : mfma_table_bytes(g, comps);
+    return ks > 1 ? tables + ks * comps * g.n0 * g.out_cols * 4 : mfma_table_bytes(g, comps);
+}
+
+// A uniformly spaced last axis of a 2D grid, y_j = y_0 + j h, factorises:
+// e^{i k y_(16a+b)} = e^{i k y_(16a)} e^{i k (y_b - y_0)}.  The 2D sum is then
+// the 3D sum over virtual axes (y_(16a): ceil(n/16) values, y_b - y_0: 16
+// values) with the mode's k_1 on both, run by k_mfma_grid3: two small phase
+// tables replace the n x N table of the axis, and the GEMM's B operand is
+// built in registers.  Output column 16a + b of the virtual grid is column j
+// of the real one (columns >= n are not written).
+constexpr int64_t kSplitMin = 256;  // n / 16 >= 16: the 3D kernel's column tile
+
+inline MfmaGeom mfma_split_geom(int64_t N, const int64_t* shape) {
+    const int64_t vshape[3] = {shape[0], ceil_div(shape[1], (int64_t)16), 16};
+    MfmaGeom g = mfma_geom(3, N, vshape);
+    g.out_cols = shape[1];
+    return g;
+}
+
+// split workspace: tables and partial sums, then the two virtual axes
+inline int64_t mfma_split_ws_bytes(const MfmaGeom& gv) {
+    return ceil_div(mfma_ws_bytes(gv), (int64_t)256) * 256 + (gv.n1 + 16) * 8;
 }
 
 // ---------------------------------------------------------------- splits ---
@@ -142,8 +164,10 @@ __device__ __forceinline__ void split2(float a, float b, uint32_t& hi, uint32_t&
 // comps > 1 (incompressible fields): E2 of component c carries the projector
 // weight p_c(k_j) = delta_c0 - k_jc k_j0 / |k_j|^2 (generator.py:479-494) in
 // place of the spectrum factor.
+// kz_row: the row of cov holding E2's wave-vector component (d - 1, or 1 for a split axis).
 template <bool FP16>
 __global__ __launch_bounds__(256) void k_mfma_tables(MfmaGeom g, int64_t d, int64_t N, int64_t comps,
+                                                     int64_t kz_row,
                                                      const double* __restrict__ ax0,
                                                      const double* __restrict__ ax1,
                                                      const double* __restrict__ ax2,
@@ -193,7 +217,7 @@ __global__ __launch_bounds__(256) void k_mfma_tables(MfmaGeom g, int64_t d, int6
         float2 v = {0.0f, 0.0f};
         if (j < N) {
             double s, c;
-            const double kz = cov[(d - 1) * N + j];
+            const double kz = cov[kz_row * N + j];
             const double th = kz * (d == 3 ? ax2[i2] : ax1[i2]);
             if (fabs(th) <= kFastPhaseMax) sincos_fast(th, &s, &c); else sincos(th, &s, &c);
             double f = fac ? fac[j] : 1.0;
@@ -237,7 +261,9 @@ __global__ __launch_bounds__(kMWaves * 64, 2) void k_mfma_grid(MfmaGeom g, doubl
                                                                float* __restrict__ out, int64_t out_stride,
                                                                int64_t comps, int64_t steps_per,
                                                                float* __restrict__ partial) {
-    __shared__ __align__(16) uint16_t lds[2][2][kMBM * kMLdsRow];  // [buffer][hi/lo][row * stride]
+    // [buffer][hi/lo][row * 16 + chunk * 4]: 64-byte A rows, 16-byte chunk c of row r
+    // stored at c ^ ((r >> 1) & 3) (the 8 lanes of a 16-byte LDS access phase hit 8 bank groups)
+    __shared__ __align__(16) uint32_t lds[2][2][kMBM * 16];
 
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
@@ -295,40 +321,40 @@ __global__ __launch_bounds__(kMWaves * 64, 2) void k_mfma_grid(MfmaGeom g, doubl
         for (int cf = 0; cf < kMCF; ++cf) acc[rf][cf] = f32x4{0.f, 0.f, 0.f, 0.f};
 
     // A staging: 128 rows x 64 B per K step for hi and for lo = 512 chunks of
-    // 16 B each; thread t moves chunks t and t + 256 of both tables.
+    // 16 B each, straight from global memory into LDS (global_load_lds_dwordx4):
+    // lane l of an instruction writes the wave's LDS base + 16 l, so the lane
+    // filling slot q (row q >> 2) fetches the chunk the swizzle puts there.
     const int64_t row0 = rt * kMBM;
-    const int64_t a_stride = g.Np;  // uint32 (element pairs) per A row
-    u32x4 pre[2][2];
-    auto load_a = [&](int64_t step) {
+    const int wv = __builtin_amdgcn_readfirstlane(wave);
+    int64_t aoff[2];
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        const int q = h * 256 + wv * 64 + lane;
+        const int r = q >> 2;
+        aoff[h] = (row0 + r) * g.Np + (((q & 3) ^ ((r >> 1) & 3)) * 4);
+    }
+    auto load_a = [&](int64_t step, int buf) {
 #pragma unroll
         for (int h = 0; h < 2; ++h) {
-            const int c = threadIdx.x + h * 256;
-            const int r = c >> 2, part = c & 3;
-            const int64_t off = (row0 + r) * a_stride + step * kMKM + part * 4;
-            pre[h][0] = *reinterpret_cast<const u32x4*>(a_hi + off);
-            pre[h][1] = *reinterpret_cast<const u32x4*>(a_lo + off);
-        }
-    };
-    auto store_a = [&](int buf) {
-#pragma unroll
-        for (int h = 0; h < 2; ++h) {
-            const int c = threadIdx.x + h * 256;
-            const int r = c >> 2, part = c & 3;
-            *reinterpret_cast<u32x4*>(&lds[buf][0][r * kMLdsRow + part * 8]) = pre[h][0];
-            *reinterpret_cast<u32x4*>(&lds[buf][1][r * kMLdsRow + part * 8]) = pre[h][1];
+            const int slot = (h * 256 + wv * 64) * 4;
+            __builtin_amdgcn_global_load_lds((const void*)(a_hi + aoff[h] + step * kMKM),
+                                             (__attribute__((address_space(3))) void*)&lds[buf][0][slot], 16, 0, 0);
+            __builtin_amdgcn_global_load_lds((const void*)(a_lo + aoff[h] + step * kMKM),
+                                             (__attribute__((address_space(3))) void*)&lds[buf][1][slot], 16, 0, 0);
         }
     };
 
     const int64_t steps = g.Np / kMKM;
     const int64_t s0 = SPLIT ? split * steps_per : 0;
     const int64_t s1 = SPLIT ? (s0 + steps_per < steps ? s0 + steps_per : steps) : steps;
-    load_a(s0);
-    store_a(0);
+    load_a(s0, 0);
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
     __syncthreads();
 
+    const int asw = (l16 >> 1) & 3;
     for (int64_t s = s0; s < s1; ++s) {
         const int buf = (int)((s - s0) & 1);
-        if (s + 1 < s1) load_a(s + 1);
+        if (s + 1 < s1) load_a(s + 1, buf ^ 1);
 
         // B fragments for this K step, built in registers.
         u32x4 bh[kMCF], bl[kMCF];
@@ -374,9 +400,9 @@ __global__ __launch_bounds__(kMWaves * 64, 2) void k_mfma_grid(MfmaGeom g, doubl
 
 #pragma unroll
         for (int rf = 0; rf < kMRF; ++rf) {
-            const int r = rf * 16 + l16;
-            const u32x4 ah = *reinterpret_cast<const u32x4*>(&lds[buf][0][r * kMLdsRow + grp * 8]);
-            const u32x4 al = *reinterpret_cast<const u32x4*>(&lds[buf][1][r * kMLdsRow + grp * 8]);
+            const int pos = (rf * 16 + l16) * 16 + ((grp ^ asw) * 4);
+            const u32x4 ah = *reinterpret_cast<const u32x4*>(&lds[buf][0][pos]);
+            const u32x4 al = *reinterpret_cast<const u32x4*>(&lds[buf][1][pos]);
 #pragma unroll
             for (int cf = 0; cf < kMCF; ++cf) {
                 acc[rf][cf] = mfma<FP16>(ah, bh[cf], acc[rf][cf]);
@@ -384,7 +410,8 @@ __global__ __launch_bounds__(kMWaves * 64, 2) void k_mfma_grid(MfmaGeom g, doubl
                 acc[rf][cf] = mfma<FP16>(al, bh[cf], acc[rf][cf]);
             }
         }
-        if (s + 1 < s1) store_a(buf ^ 1);
+        // the next step's LDS loads have landed (the barrier's own wait covers LDS, not them)
+        asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
         __syncthreads();
     }
 
@@ -397,7 +424,7 @@ __global__ __launch_bounds__(kMWaves * 64, 2) void k_mfma_grid(MfmaGeom g, doubl
             if (row < g.n0) {
 #pragma unroll
                 for (int cf = 0; cf < kMCF; ++cf)
-                    if (cok[cf]) out[row * g.ncols + col[cf]] = fa * acc[rf][cf][i];
+                    if (cok[cf]) out[row * g.out_cols + col[cf]] = fa * acc[rf][cf][i];
             }
         }
     }
@@ -406,8 +433,9 @@ __global__ __launch_bounds__(kMWaves * 64, 2) void k_mfma_grid(MfmaGeom g, doubl
 // 3D grids with n1, n2 >= 16: block tile = 128 rows x (16 i1 x 16 i2) columns.
 // Each K step stages, through double-buffered LDS: the A tile (hi and lo,
 // 128 rows x 16 element pairs) and the E1 / E2 slices of the block's 16 i1
-// and 16 i2 values (16 modes each), all loaded one step ahead into
-// registers by the whole block.  Layouts are XOR-swizzled so that the 8
+// and 16 i2 values (16 modes each), loaded one step ahead by the whole block
+// straight from global memory into LDS (global_load_lds_dwordx4: no staging
+// registers, no LDS store instructions).  Layouts are XOR-swizzled so that the 8
 // lanes of one 16-byte LDS access phase (32 banks x 4 B = 8 bank groups of
 // 16 B) hit 8 distinct groups, for the reads (8 rows) and the stores:
 //   A row r (64 B = 4 chunks, two rows per 128 B):  chunk c stored at c ^ ((r >> 1) & 3)
@@ -458,36 +486,36 @@ __global__ __launch_bounds__(kMWaves * 64, 2) void k_mfma_grid3(MfmaGeom g, doub
     const int64_t tile1 = ct / g.t2, tile2 = ct - tile1 * g.t2;
     const int64_t i1b = tile1 * 16, i2b = tile2 * 16, row0 = rt * kMBM;
 
-    // cooperative loads: A chunks tid and tid + 256 (row c >> 2, chunk c & 3);
-    // E chunk: threads 0..127 -> E1, 128..255 -> E2 (row (tid & 127) >> 3, chunk tid & 7)
-    const int erow = (tid & 127) >> 3, ech = tid & 7;
+    // each K step, each thread fills two 16-byte LDS slots of the A tile (hi and lo: slots
+    // tid and tid + 256, row q >> 2) and one of E: threads 0..127 -> E1, 128..255 -> E2
+    // (row (tid & 127) >> 3)
+    const int erow = (tid & 127) >> 3;
     const bool is_e1 = tid < 128;
     const int64_t eidx = is_e1 ? (i1b + erow < g.n1 ? i1b + erow : g.n1 - 1)
                                : (i2b + erow < g.n2 ? i2b + erow : g.n2 - 1);
-    const float2* ebase = (is_e1 ? e1 : e2) + eidx * g.Np + ech * 2;
-    u32x4 pa[2][2];
-    float4 pe;
-    auto load = [&](int64_t step) {
+    // global -> LDS loads: lane l of an instruction writes 16 B at the wave's LDS base + 16 l,
+    // so each lane fetches the global chunk that the swizzled layout puts at its LDS slot
+    const int wv = __builtin_amdgcn_readfirstlane(wave);
+    int64_t aoff[2];
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        const int q = h * 256 + wv * 64 + lane;
+        const int r = q >> 2;
+        aoff[h] = (row0 + r) * g.Np + (((q & 3) ^ ((r >> 1) & 3)) * 4);
+    }
+    const float2* ebase = (is_e1 ? e1 : e2) + eidx * g.Np + (((tid & 7) ^ (erow & 7)) * 2);
+    auto load = [&](int64_t step, int buf) {
 #pragma unroll
         for (int h = 0; h < 2; ++h) {
-            const int c = tid + h * 256;
-            const int64_t off = (row0 + (c >> 2)) * g.Np + step * kMKM + (c & 3) * 4;
-            pa[h][0] = *reinterpret_cast<const u32x4*>(a_hi + off);
-            pa[h][1] = *reinterpret_cast<const u32x4*>(a_lo + off);
+            const int slot = (h * 256 + wv * 64) * 4;
+            __builtin_amdgcn_global_load_lds((const void*)(a_hi + aoff[h] + step * kMKM),
+                                             (__attribute__((address_space(3))) void*)&st[buf].ah[slot], 16, 0, 0);
+            __builtin_amdgcn_global_load_lds((const void*)(a_lo + aoff[h] + step * kMKM),
+                                             (__attribute__((address_space(3))) void*)&st[buf].al[slot], 16, 0, 0);
         }
-        pe = *reinterpret_cast<const float4*>(ebase + step * kMKM);
-    };
-    auto store = [&](int buf) {
-#pragma unroll
-        for (int h = 0; h < 2; ++h) {
-            const int c = tid + h * 256;
-            const int r = c >> 2;
-            const int pos = r * 16 + (((c & 3) ^ ((r >> 1) & 3)) * 4);
-            *reinterpret_cast<u32x4*>(&st[buf].ah[pos]) = pa[h][0];
-            *reinterpret_cast<u32x4*>(&st[buf].al[pos]) = pa[h][1];
-        }
-        float* es = is_e1 ? st[buf].e1 : st[buf].e2;
-        *reinterpret_cast<float4*>(es + erow * 32 + ((ech ^ (erow & 7)) * 4)) = pe;
+        float* es = (wv < 2 ? st[buf].e1 : st[buf].e2) + (wv & 1) * 256;
+        __builtin_amdgcn_global_load_lds((const void*)(ebase + step * kMKM),
+                                         (__attribute__((address_space(3))) void*)es, 16, 0, 0);
     };
 
     f32x4 acc[kMRF][kMCF];
@@ -499,14 +527,13 @@ __global__ __launch_bounds__(kMWaves * 64, 2) void k_mfma_grid3(MfmaGeom g, doub
     const int64_t steps = g.Np / kMKM;
     const int64_t s0 = SPLIT ? split * steps_per : 0;
     const int64_t s1 = SPLIT ? (s0 + steps_per < steps ? s0 + steps_per : steps) : steps;
-    load(s0);
-    store(0);
+    load(s0, 0);
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
     __syncthreads();
     const int zsw = l16 & 7;
     for (int64_t s = s0; s < s1; ++s) {
         const int buf = (int)((s - s0) & 1);
-        if (s + 1 < s1) load(s + 1);
-
+        if (s + 1 < s1) load(s + 1, buf ^ 1);
         // B fragments: E2 of this lane's i2 (row l16) times E1 of each fragment's i1
         float2 zf[4];
         {
@@ -550,7 +577,8 @@ __global__ __launch_bounds__(kMWaves * 64, 2) void k_mfma_grid3(MfmaGeom g, doub
                 acc[rf][cf] = mfma<FP16>(al, bh[cf], acc[rf][cf]);
             }
         }
-        if (s + 1 < s1) store(buf ^ 1);
+        // the next step's LDS loads have landed (the barrier's own wait covers LDS, not them)
+        asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
         __syncthreads();
     }
 
@@ -561,12 +589,13 @@ __global__ __launch_bounds__(kMWaves * 64, 2) void k_mfma_grid3(MfmaGeom g, doub
         const int64_t i1 = i1b + wave * kMCF + cf;
         if (i1 >= g.n1 || i2 >= g.n2) continue;
         const int64_t col = i1 * g.n2 + i2;
+        if (col >= g.out_cols) continue;  // past the end of a split axis
 #pragma unroll
         for (int rf = 0; rf < kMRF; ++rf) {
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
                 const int64_t row = row0 + rf * 16 + grp * 4 + i;
-                if (row < g.n0) out[row * g.ncols + col] = fa * acc[rf][cf][i];
+                if (row < g.n0) out[row * g.out_cols + col] = fa * acc[rf][cf][i];
             }
         }
     }
@@ -587,23 +616,55 @@ __global__ void k_fill_zero_f32(float* __restrict__ p, int64_t n) {
     if (i < n) p[i] = 0.0f;
 }
 
+// Virtual axes of a split last axis (one block): vy[a] = y[16a], vz[b] = y[b] - y[0].
+// Checks y[j] == y[16 floor(j/16)] + (y[j mod 16] - y[0]) to 4 ulp of max|y| for every
+// j (the same FP64 operations as the host's check); any miss (or a NaN) makes every vy
+// NaN, so the whole output is NaN rather than a wrong field.
+__global__ __launch_bounds__(256) void k_split_axis(const double* __restrict__ y, int64_t n,
+                                                    double* __restrict__ vy, double* __restrict__ vz) {
+    __shared__ double s_max[256];
+    __shared__ int s_bad;
+    const int t = threadIdx.x;
+    double m = 0.0;
+    for (int64_t j = t; j < n; j += 256) m = fmax(m, fabs(y[j]));
+    s_max[t] = m;
+    if (t == 0) s_bad = 0;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (t < w) s_max[t] = fmax(s_max[t], s_max[t + w]);
+        __syncthreads();
+    }
+    const double tol = 0x1p-50 * s_max[0];
+    const double y0 = y[0];
+    bool bad = false;
+    for (int64_t j = t; j < n; j += 256) {
+        const double r = y[j & ~(int64_t)15] + (y[j & 15] - y0);
+        if (!(fabs(y[j] - r) <= tol)) bad = true;
+    }
+    if (bad) s_bad = 1;
+    __syncthreads();
+    const double poison = s_bad ? __builtin_nan("") : 0.0;
+    for (int64_t a = t; a < (n + 15) / 16; a += 256) vy[a] = y[16 * a] + poison;
+    if (t < 16) vz[t] = y[t] - y0;
+}
+
 template <bool FP16>
-int mfma_launch(const MfmaGeom& g, int64_t d, int64_t N, int64_t comps, const double* ax0, const double* ax1,
-                const double* ax2, const double* cov, const double* z1, const double* z2, const double* fac,
-                double alpha, float* out, void* ws, hipStream_t s) {
+int mfma_launch(const MfmaGeom& g, int64_t d, int64_t N, int64_t comps, int64_t kz_row, const double* ax0,
+                const double* ax1, const double* ax2, const double* cov, const double* z1, const double* z2,
+                const double* fac, double alpha, float* out, void* ws, hipStream_t s) {
     uint32_t* a_hi = static_cast<uint32_t*>(ws);
     uint32_t* a_lo = a_hi + g.a_elems / 2;
     float2* e1 = reinterpret_cast<float2*>(a_lo + g.a_elems / 2);
     float2* e2 = e1 + g.e1_elems;
     const int64_t entries = g.n0p * g.Np + g.e1_elems + comps * g.e2_elems;
     hipLaunchKernelGGL((k_mfma_tables<FP16>), dim3((unsigned)ceil_div(entries, 256)), dim3(256), 0, s, g, d,
-                       N, comps, ax0, ax1, ax2, cov, z1, z2, fac, a_hi, a_lo, e1, e2);
+                       N, comps, kz_row, ax0, ax1, ax2, cov, z1, z2, fac, a_hi, a_lo, e1, e2);
     int st = launch_status();
     if (st != GSX_OK) return st;
     const int64_t steps_per = mfma_steps_per_split(g, comps);
     const int64_t ksplit = mfma_ksplit(g, comps);
     const int64_t blocks = ksplit * comps * g.row_tiles * g.col_tiles;
-    const int64_t stride = g.n0 * g.ncols;  // output values per component
+    const int64_t stride = g.n0 * g.out_cols;  // output values per component
     float* partial = nullptr;
     if (ksplit > 1)
         partial = reinterpret_cast<float*>(static_cast<char*>(ws) +
@@ -645,6 +706,8 @@ int mfma_entry(int64_t d, int64_t N, const int64_t* shape, const double* axes, c
                void* workspace, int64_t workspace_bytes, void* stream, int64_t comps = 1) {
     if (d == 1) return GSX_E_UNSUPPORTED;
     if ((d != 2 && d != 3) || N < 0 || !shape) return GSX_E_INVALID;
+    const bool uniform_last = (variant & GSX_MFMA_UNIFORM_LAST) != 0;
+    variant &= ~GSX_MFMA_UNIFORM_LAST;
     if (variant != GSX_MFMA_BF16X3 && variant != GSX_MFMA_FP16X3) return GSX_E_INVALID;
     int64_t P = 1;
     for (int i = 0; i < d; ++i) {
@@ -659,17 +722,33 @@ int mfma_entry(int64_t d, int64_t N, const int64_t* shape, const double* axes, c
                            comps * P);
         return launch_status();
     }
-    const MfmaGeom g = mfma_geom(d, N, shape);
+    const bool split = uniform_last && d == 2 && comps == 1 && shape[1] >= kSplitMin;
+    const MfmaGeom g = split ? mfma_split_geom(N, shape) : mfma_geom(d, N, shape);
     if (mfma_ksplit(g, comps) * comps * g.row_tiles * g.col_tiles > 0x7fffffffLL ||
         ceil_div(comps * P, (int64_t)256) > 0x7fffffffLL)
         return GSX_E_INVALID;
-    if (!workspace || workspace_bytes < mfma_ws_bytes(g, comps)) return GSX_E_WORKSPACE;
+    if (!workspace || workspace_bytes < (split ? mfma_split_ws_bytes(g) : mfma_ws_bytes(g, comps)))
+        return GSX_E_WORKSPACE;
     const double* ax0 = axes;
     const double* ax1 = axes + shape[0];
     const double* ax2 = d == 3 ? ax1 + shape[1] : nullptr;
+    int64_t dk = d, kz_row = d - 1;
+    if (split) {
+        double* vy = reinterpret_cast<double*>(static_cast<char*>(workspace) +
+                                               ceil_div(mfma_ws_bytes(g), (int64_t)256) * 256);
+        double* vz = vy + g.n1;
+        hipLaunchKernelGGL(k_split_axis, dim3(1), dim3(256), 0, s, ax1, shape[1], vy, vz);
+        const int st = launch_status();
+        if (st != GSX_OK) return st;
+        ax1 = vy;
+        ax2 = vz;
+        dk = 3;      // tables of the 3D form
+        kz_row = 1;  // both virtual axes carry the mode's k_1
+    }
     if (variant == GSX_MFMA_FP16X3)
-        return mfma_launch<true>(g, d, N, comps, ax0, ax1, ax2, cov, z1, z2, fac, alpha, out, workspace, s);
-    return mfma_launch<false>(g, d, N, comps, ax0, ax1, ax2, cov, z1, z2, fac, alpha, out, workspace, s);
+        return mfma_launch<true>(g, dk, N, comps, kz_row, ax0, ax1, ax2, cov, z1, z2, fac, alpha, out, workspace,
+                                 s);
+    return mfma_launch<false>(g, dk, N, comps, kz_row, ax0, ax1, ax2, cov, z1, z2, fac, alpha, out, workspace, s);
 }
 
 }  // namespace gsx
@@ -681,7 +760,11 @@ int64_t gsx_summate_grid_mfma_workspace(int64_t d, int64_t N, const int64_t* sha
     for (int i = 0; i < d; ++i)
         if (shape[i] < 0) return GSX_E_INVALID;
     const gsx::MfmaGeom g = gsx::mfma_geom(d, N, shape);
-    return gsx::mfma_ws_bytes(g);
+    const int64_t ws = gsx::mfma_ws_bytes(g);
+    if (d != 2 || shape[1] < gsx::kSplitMin) return ws;
+    // enough for GSX_MFMA_UNIFORM_LAST as well
+    const int64_t ws_split = gsx::mfma_split_ws_bytes(gsx::mfma_split_geom(N, shape));
+    return ws > ws_split ? ws : ws_split;
 }
 
 int gsx_summate_grid_mfma(int64_t d, int64_t N, const int64_t* shape, const double* axes, const double* cov,

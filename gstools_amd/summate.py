@@ -75,6 +75,29 @@ def _modes(cov_samples, z_1, z_2, dev):
     return cov, z1, z2
 
 
+# points on a 2D grid's last axis from which the matrix-core path factorises a
+# uniformly spaced axis (n / 16 >= 16 virtual points: one column tile)
+SPLIT_MIN = 256
+
+
+def uniform_axis(axis):
+    """True when the 1-D ``axis`` is uniformly spaced in the sense the matrix-core
+    path's ``MFMA_UNIFORM_LAST`` flag requires: ``y[j] == y[16 (j // 16)] + (y[j % 16]
+    - y[0])`` to 4 ulp of ``max|y|`` for every j (the same FP64 operations as the
+    library's device check), with at least ``SPLIT_MIN`` points.  NaN -> False."""
+    if isinstance(axis, torch.Tensor):
+        y = axis.detach().to("cpu", torch.float64).numpy().reshape(-1)
+    else:
+        y = np.asarray(axis, dtype=np.float64).reshape(-1)
+    n = y.size
+    if n < SPLIT_MIN:
+        return False
+    j = np.arange(n)
+    r = y[j & ~15] + (y[j & 15] - y[0])
+    tol = 2.0 ** -50 * np.max(np.abs(y))
+    return bool(np.all(np.abs(y - r) <= tol))
+
+
 def _factor(spectrum_factor, n, dev):
     if spectrum_factor is None:
         return None
@@ -161,6 +184,7 @@ def summate_grid_device(cov_samples, z_1, z_2, axes, precision="fp64", alpha=1.0
     cov, z1, z2 = _modes(cov_samples, z_1, z_2, dev)
     d, n = cov.shape
     fac = _factor(spectrum_factor, n, dev)
+    uniform_last = len(axes) == 2 and precision in ("bf16x3", "fp16x3") and uniform_axis(axes[1])
     axes = [_as_f64(a, "axes", dev).reshape(-1) for a in axes]
     if len(axes) != d:
         raise ValueError(f"expected {d} axes, got {len(axes)}")
@@ -189,6 +213,8 @@ def summate_grid_device(cov_samples, z_1, z_2, axes, precision="fp64", alpha=1.0
                       "summate_grid")
         return out
     variant = _native.MFMA_BF16X3 if precision == "bf16x3" else _native.MFMA_FP16X3
+    if uniform_last:  # factorised last axis (no n x N phase table)
+        variant |= _native.MFMA_UNIFORM_LAST
     ws_bytes = int(lib.gsx_summate_grid_mfma_workspace(d, n, shape_c))
     if ws_bytes < 0:
         _native.check(int(ws_bytes), "summate_grid_mfma")

@@ -74,6 +74,13 @@ extern "C" {
 /* MFMA split-precision variants for gsx_summate_grid_mfma. */
 #define GSX_MFMA_BF16X3 0 /* bf16 hi/lo split, hi*hi + hi*lo + lo*hi */
 #define GSX_MFMA_FP16X3 1 /* fp16 hi/lo split, hi*hi + hi*lo + lo*hi */
+/* Flag or-ed into the variant: the caller states that the last axis of a 2D
+ * grid is uniformly spaced (y_j = y_0 + j h: y_j == y_(16a) + (y_b - y_0),
+ * j = 16a + b, to 4 ulp of max|y|).  With at least 256 points on it the
+ * library factorises that axis into two short ones (no n x N phase table;
+ * faster).  Checked on the device: a non-uniform axis gives an all-NaN
+ * output.  Ignored for 3D grids, incompressible sums and shorter axes. */
+#define GSX_MFMA_UNIFORM_LAST 16
 
 const char* gsx_status_string(int status);
 int gsx_version(void);

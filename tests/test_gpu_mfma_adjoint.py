@@ -4,6 +4,10 @@ Matrix-core path (split bf16 / fp16 products, FP32 accumulation): compared
 with the oracle's FP64 sum on the materialised grid; rel_max and rel_rms are
 printed and bounded (fp16x3 <= 1e-5 for both, bf16x3 rel_rms <= 1e-5).
 
+A uniformly spaced last axis of a 2D grid (``MFMA_UNIFORM_LAST``) is summed in its
+factorised form, to the same bars; the flag's device check turns a non-uniform axis
+into an all-NaN output.
+
 Adjoint: FP64 kernel against the oracle's C adjoint (<= 1e-10 relative to the
 gradient's scale), against central finite differences of the GPU forward in
 FP64 (h = 1e-4), and against torch float64 autograd of the same formula; the
@@ -11,6 +15,7 @@ FP32 kernel within 1e-5 of the FP64 one.  The autograd wrapper's len_scale
 gradient is checked against finite differences.
 """
 
+import ctypes
 import sys
 
 import numpy as np
@@ -18,9 +23,11 @@ import pytest
 import torch
 
 import oracle
-from gstools_amd import Gaussian, Matern, RandMeth, summate_adjoint_device, summate_device, summate_grid_device
+from gstools_amd import (Gaussian, Matern, RandMeth, _native, summate_adjoint_device, summate_device,
+                         summate_grid_device)
 from gstools_amd.autograd import RandMethSum, randmeth_field
 from gstools_amd.geometry import generate_grid
+from gstools_amd.summate import uniform_axis
 
 pytestmark = pytest.mark.gpu
 
@@ -67,6 +74,73 @@ def test_mfma_alpha_and_empty_modes():
     cov0, z10, z20 = _modes(3, 0, 5)
     z = summate_grid_device(cov0, z10, z20, axes, "bf16x3").cpu().numpy()
     assert np.array_equal(z, np.zeros(20 * 18 * 17, dtype=np.float32))
+
+
+def _grid_mfma_raw(cov, z1, z2, axes, variant):
+    """gsx_summate_grid_mfma through the C ABI with an explicit variant word."""
+    lib = _native.lib()
+    dev = torch.device("cuda")
+    t = lambda a: torch.from_numpy(np.ascontiguousarray(a, dtype=np.float64)).to(dev)
+    d, n = cov.shape
+    shape = [len(a) for a in axes]
+    shape_c = (ctypes.c_int64 * 3)(*(shape + [1] * (3 - d)))
+    ws_bytes = int(lib.gsx_summate_grid_mfma_workspace(d, n, shape_c))
+    ws = torch.empty(max(ws_bytes, 1), dtype=torch.uint8, device=dev)
+    out = torch.empty(int(np.prod(shape)), dtype=torch.float32, device=dev)
+    flat, tc, t1, t2 = t(np.concatenate(axes)), t(cov), t(z1), t(z2)
+    p = lambda x: ctypes.c_void_p(x.data_ptr())
+    st = lib.gsx_summate_grid_mfma(d, n, shape_c, p(flat), p(tc), p(t1), p(t2), variant, 1.0, p(out), p(ws),
+                                   ws_bytes, None)
+    assert st == 0, st
+    return out.cpu().numpy().astype(np.float64)
+
+
+@pytest.mark.parametrize("shape,n", [
+    ((20, 256), 100),    # one row tile, 16 virtual rows: K split over blocks
+    ((37, 1000), 700),   # 1000 = 62 * 16 + 8: the last virtual row is half outside
+    ((130, 4100), 64),   # two row tiles, 257 virtual rows
+])
+def test_mfma_uniform_last_axis(shape, n):
+    """A uniformly spaced last axis of a 2D grid (>= 256 points) is factorised into
+    y_(16a) and y_b - y_0 and summed by the 3D matrix-core kernel: within the same bars
+    of the oracle's FP64 sum as the plain 2D form, and close to that form."""
+    _say(f"mfma uniform last axis shape={shape} N={n}")
+    cov, z1, z2 = _modes(2, n, 91 + n, kscale=0.6)
+    axes = [np.linspace(-4.0, 9.0, shape[0]), 2.5 + 0.37 * np.arange(shape[1])]
+    assert uniform_axis(axes[1])
+    ref = oracle.summate_c(cov, z1, z2, generate_grid(axes))
+    for precision, variant in (("fp16x3", _native.MFMA_FP16X3), ("bf16x3", _native.MFMA_BF16X3)):
+        got = summate_grid_device(cov, z1, z2, axes, precision).cpu().numpy().astype(np.float64)
+        split = _grid_mfma_raw(cov, z1, z2, axes, variant | _native.MFMA_UNIFORM_LAST)
+        plain = _grid_mfma_raw(cov, z1, z2, axes, variant)
+        assert np.array_equal(got, split)  # the host sets the flag for this axis
+        for name, v in (("split", split), ("plain", plain)):
+            rel_max, rel_rms = oracle.rel_errors(v, ref)
+            _say(f"  {precision} {name} rel_max={rel_max:.3e} rel_rms={rel_rms:.3e}")
+            if precision == "fp16x3":
+                assert rel_max <= 1e-5 and rel_rms <= 1e-5
+            else:
+                assert rel_rms <= 1e-5 and rel_max <= 5e-5
+        assert not np.array_equal(split, plain)  # the two forms round differently
+
+
+def test_mfma_uniform_flag_checked_and_ignored():
+    """The flag on a non-uniform axis gives an all-NaN output (the device check); on a 3D
+    grid or an axis shorter than 256 points it is ignored (same bits as without it)."""
+    _say("mfma uniform flag: non-uniform axis / ignored cases")
+    cov, z1, z2 = _modes(2, 200, 3, kscale=0.6)
+    y = np.arange(300.0) * 0.5
+    y[123] += 1e-9
+    assert not uniform_axis(y)
+    out = _grid_mfma_raw(cov, z1, z2, [np.arange(40.0), y], _native.MFMA_BF16X3 | _native.MFMA_UNIFORM_LAST)
+    assert np.isnan(out).all()
+    short = [np.arange(40.0), np.arange(255.0)]
+    a = _grid_mfma_raw(cov, z1, z2, short, _native.MFMA_BF16X3 | _native.MFMA_UNIFORM_LAST)
+    assert np.array_equal(a, _grid_mfma_raw(cov, z1, z2, short, _native.MFMA_BF16X3))
+    cov3, z13, z23 = _modes(3, 100, 4, kscale=0.6)
+    ax3 = [np.arange(20.0), np.arange(17.0), np.arange(300.0)]
+    a = _grid_mfma_raw(cov3, z13, z23, ax3, _native.MFMA_FP16X3 | _native.MFMA_UNIFORM_LAST)
+    assert np.array_equal(a, _grid_mfma_raw(cov3, z13, z23, ax3, _native.MFMA_FP16X3))
 
 
 def test_mfma_matern_config_sample():

@@ -121,3 +121,27 @@ def test_mfma_workspace_queries(fn):
     big = q(3, 4096, shp(512, 512, 512))
     assert big == 2 * (512 * 4096) * 2 * 2 + (512 * 4096 + (3 if "incompr" in fn else 1) * 512 * 4096) * 8
     assert q(3, -1, shp(8, 8, 8)) < 0 and q(1, 8, shp(8)) < 0
+
+
+def test_uniform_axis_check():
+    """The host side of ``MFMA_UNIFORM_LAST``: uniform spacing to 4 ulp of max|y| with at
+    least 256 points; a perturbed point, a NaN, a short axis or a geometric axis fail."""
+    from gstools_amd.summate import SPLIT_MIN, uniform_axis
+
+    assert SPLIT_MIN == 256
+    assert uniform_axis(np.arange(4096.0))
+    assert uniform_axis(np.linspace(-5.0, 5.0, 257))
+    assert uniform_axis(2.5 + 0.37 * np.arange(4100))
+    assert uniform_axis(1e6 + 0.1 * np.arange(1000))
+    assert not uniform_axis(np.arange(255.0))
+    y = np.arange(300.0) * 0.5
+    y[123] += 1e-9
+    assert not uniform_axis(y)
+    y[123] = np.nan
+    assert not uniform_axis(y)
+    assert not uniform_axis(np.geomspace(1.0, 2.0, 400))
+    # the 2D workspace query also covers the factorised form (256 points and more)
+    q = _native.lib().gsx_summate_grid_mfma_workspace
+    shp = lambda *s: (ctypes.c_int64 * 3)(*(list(s) + [1] * (3 - len(s))))
+    assert q(2, 100, shp(20, 256)) >= q(3, 100, shp(20, 16, 16))
+    assert q(2, 4096, shp(4096, 4096)) >= q(3, 4096, shp(4096, 256, 16))
