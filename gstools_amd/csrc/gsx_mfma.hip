@@ -57,9 +57,8 @@ struct MfmaGeom {
     int64_t n0p, Np;      // padded rows / modes
     int64_t ncols;        // n1 * n2
     int64_t out_cols;     // output columns per row: n1 * n2, or the split axis' length
-    int g2;               // i2 fragments per column group (1 or 16)
-    int64_t ti1, ti2;     // column tile extent along i1 / i2
-    int64_t t1, t2;       // column tiles along i1 / i2
+    int g2;               // 1: k_mfma_grid3 (16 i1 x 16 i2 column tiles); 16: k_mfma_grid
+    int64_t t1, t2;       // column tiles along i1 / i2 (k_mfma_grid: 1 x tiles of 256 flat columns)
     int64_t row_tiles, col_tiles;
     int64_t a_elems, e1_elems, e2_elems;
 };
@@ -74,10 +73,8 @@ inline MfmaGeom mfma_geom(int64_t d, int64_t N, const int64_t* shape) {
     g.ncols = g.n1 * g.n2;
     g.out_cols = g.ncols;
     g.g2 = (g.n1 >= 16 && g.n2 >= 16) ? 1 : 16;
-    g.ti1 = 16 / g.g2;
-    g.ti2 = 16 * (int64_t)g.g2;
-    g.t1 = ceil_div(g.n1, g.ti1);
-    g.t2 = ceil_div(g.n2, g.ti2);
+    g.t1 = g.g2 == 1 ? ceil_div(g.n1, (int64_t)16) : 1;
+    g.t2 = g.g2 == 1 ? ceil_div(g.n2, (int64_t)16) : ceil_div(g.ncols, (int64_t)256);
     g.row_tiles = g.n0p / kMBM;
     g.col_tiles = g.t1 * g.t2;
     g.a_elems = g.n0p * 2 * g.Np;  // 16-bit elements, hi and lo tables each
@@ -252,7 +249,7 @@ __device__ __forceinline__ float2 cmul(float2 a, float2 b) {
 
 // E1ONE: the grid has a one-point middle axis (2D grids), E1 == 1, so B is E2.
 // SPLIT: the K steps are split over blocks (small grids), partial sums written.
-template <bool FP16, int G2, bool E1ONE, bool SPLIT>
+template <bool FP16, bool E1ONE, bool SPLIT>
 __global__ __launch_bounds__(kMWaves * 64, 2) void k_mfma_grid(MfmaGeom g, double alpha,
                                                                const uint32_t* __restrict__ a_hi,
                                                                const uint32_t* __restrict__ a_lo,
@@ -294,24 +291,22 @@ __global__ __launch_bounds__(kMWaves * 64, 2) void k_mfma_grid(MfmaGeom g, doubl
             ct = b / R;
         }
     }
-    const int64_t tile1 = ct / g.t2, tile2 = ct - tile1 * g.t2;
-
-    // Column of each fragment; clamped copies index the tables.
+    // Columns are flat (c = i1 * n2 + i2, 256 per tile: 16 per fragment), so a
+    // short last axis (thin 3D slabs) wastes no lanes; the
+    // clamped (i1, i2) of each fragment column index the tables.
     int64_t col[kMCF];
     bool cok[kMCF];
     const float2* e1p[kMCF];
     const float2* e2p[kMCF];
 #pragma unroll
     for (int cf = 0; cf < kMCF; ++cf) {
-        const int f = wave * kMCF + cf;
-        const int64_t i1 = tile1 * (16 / G2) + f / G2;
-        const int64_t i2 = tile2 * (16 * G2) + 16 * (f % G2) + l16;
-        cok[cf] = i1 < g.n1 && i2 < g.n2;
-        col[cf] = i1 * g.n2 + i2;
-        const int64_t c1 = i1 < g.n1 ? i1 : g.n1 - 1;
-        const int64_t c2 = i2 < g.n2 ? i2 : g.n2 - 1;
-        e1p[cf] = e1 + c1 * g.Np + 4 * grp;
-        e2p[cf] = e2 + c2 * g.Np + 4 * grp;
+        const int64_t c = ct * 256 + 16 * (wave * kMCF + cf) + l16;
+        cok[cf] = c < g.ncols;
+        col[cf] = c;
+        const int64_t cc = cok[cf] ? c : g.ncols - 1;
+        const int64_t i1 = cc / g.n2, i2 = cc - i1 * g.n2;
+        e1p[cf] = e1 + i1 * g.Np + 4 * grp;
+        e2p[cf] = e2 + i2 * g.Np + 4 * grp;
     }
 
     f32x4 acc[kMRF][kMCF];
@@ -360,15 +355,6 @@ __global__ __launch_bounds__(kMWaves * 64, 2) void k_mfma_grid(MfmaGeom g, doubl
         u32x4 bh[kMCF], bl[kMCF];
         const int64_t moff = s * kMKM;
         float2 zf[4];
-        if constexpr (G2 == 1) {
-            // all fragments of the wave share i2: one E2 load
-            const float4* p = reinterpret_cast<const float4*>(e2p[0] + moff);
-            const float4 q0 = p[0], q1 = p[1];
-            zf[0] = {q0.x, q0.y};
-            zf[1] = {q0.z, q0.w};
-            zf[2] = {q1.x, q1.y};
-            zf[3] = {q1.z, q1.w};
-        }
 #pragma unroll
         for (int cf = 0; cf < kMCF; ++cf) {
             float2 yf[4];
@@ -380,7 +366,7 @@ __global__ __launch_bounds__(kMWaves * 64, 2) void k_mfma_grid(MfmaGeom g, doubl
                 yf[2] = {q1.x, q1.y};
                 yf[3] = {q1.z, q1.w};
             }
-            if constexpr (G2 != 1) {
+            {
                 const float4* p = reinterpret_cast<const float4*>(e2p[cf] + moff);
                 const float4 q0 = p[0], q1 = p[1];
                 zf[0] = {q0.x, q0.y};
@@ -679,17 +665,17 @@ int mfma_launch(const MfmaGeom& g, int64_t d, int64_t N, int64_t comps, int64_t 
                                stride, comps, steps_per, partial);
     } else if (d == 2) {
         if (ksplit > 1)
-            hipLaunchKernelGGL((k_mfma_grid<FP16, 16, true, true>), grid, block, 0, s, g, alpha, a_hi, a_lo, e1, e2,
+            hipLaunchKernelGGL((k_mfma_grid<FP16, true, true>), grid, block, 0, s, g, alpha, a_hi, a_lo, e1, e2,
                                out, stride, comps, steps_per, partial);
         else
-            hipLaunchKernelGGL((k_mfma_grid<FP16, 16, true, false>), grid, block, 0, s, g, alpha, a_hi, a_lo, e1, e2,
+            hipLaunchKernelGGL((k_mfma_grid<FP16, true, false>), grid, block, 0, s, g, alpha, a_hi, a_lo, e1, e2,
                                out, stride, comps, steps_per, partial);
     } else {
         if (ksplit > 1)
-            hipLaunchKernelGGL((k_mfma_grid<FP16, 16, false, true>), grid, block, 0, s, g, alpha, a_hi, a_lo, e1, e2,
+            hipLaunchKernelGGL((k_mfma_grid<FP16, false, true>), grid, block, 0, s, g, alpha, a_hi, a_lo, e1, e2,
                                out, stride, comps, steps_per, partial);
         else
-            hipLaunchKernelGGL((k_mfma_grid<FP16, 16, false, false>), grid, block, 0, s, g, alpha, a_hi, a_lo, e1,
+            hipLaunchKernelGGL((k_mfma_grid<FP16, false, false>), grid, block, 0, s, g, alpha, a_hi, a_lo, e1,
                                e2, out, stride, comps, steps_per, partial);
     }
     st = launch_status();

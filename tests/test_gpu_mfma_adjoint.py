@@ -47,6 +47,9 @@ def _modes(d, n, seed, kscale=1.0):
     ((130, 17, 33), 64),     # two row tiles, column tiles with clamped i1
     ((64, 300), 500),        # 2D grid (one-point middle axis)
     ((5, 16, 16), 16),       # a single K step
+    ((40, 8, 300), 200),     # 3D, short middle axis: flat columns (generic kernel)
+    ((64, 300, 7), 256),     # 3D, short last axis: flat columns across i1 rows
+    ((33, 5, 5), 64),        # both column axes short: one ragged column tile
 ])
 @pytest.mark.parametrize("precision", ["fp16x3", "bf16x3"])
 def test_mfma_grid_matches_fp64(shape, n, precision):
