@@ -56,7 +56,8 @@ struct MfmaGeom {
     int64_t n0, n1, n2;   // grid (n1 = 1 for 2D grids)
     int64_t n0p, Np;      // padded rows / modes
     int64_t ncols;        // n1 * n2
-    int64_t out_cols;     // output columns per row: n1 * n2, or the split axis' length
+    int64_t out_cols;     // valid flat columns i1 * n2 + i2: n1 * n2, or the split axis' length
+    int64_t rs, c1s;      // output offset of (row, i1, i2) = row * rs + i1 * c1s + i2
     int g2;               // 1: k_mfma_grid3 (16 i1 x 16 i2 column tiles); 16: k_mfma_grid
     int64_t t1, t2;       // column tiles along i1 / i2 (k_mfma_grid: 1 x tiles of 256 flat columns)
     int64_t row_tiles, col_tiles;
@@ -72,6 +73,8 @@ inline MfmaGeom mfma_geom(int64_t d, int64_t N, const int64_t* shape) {
     g.Np = ceil_div(N, kMKM) * kMKM;
     g.ncols = g.n1 * g.n2;
     g.out_cols = g.ncols;
+    g.rs = g.ncols;
+    g.c1s = g.n2;
     g.g2 = (g.n1 >= 16 && g.n2 >= 16) ? 1 : 16;
     g.t1 = g.g2 == 1 ? ceil_div(g.n1, (int64_t)16) : 1;
     g.t2 = g.g2 == 1 ? ceil_div(g.n2, (int64_t)16) : ceil_div(g.ncols, (int64_t)256);
@@ -130,12 +133,52 @@ inline MfmaGeom mfma_split_geom(int64_t N, const int64_t* shape) {
     const int64_t vshape[3] = {shape[0], ceil_div(shape[1], (int64_t)16), 16};
     MfmaGeom g = mfma_geom(3, N, vshape);
     g.out_cols = shape[1];
+    g.rs = shape[1];
     return g;
 }
 
 // split workspace: tables and partial sums, then the two virtual axes
 inline int64_t mfma_split_ws_bytes(const MfmaGeom& gv) {
     return ceil_div(mfma_ws_bytes(gv), (int64_t)256) * 256 + (gv.n1 + 16) * 8;
+}
+
+// The GEMM's row axis is axis 0, in 128-row tiles.  A 3D grid with a short
+// axis 0 (n0 = 32 fills a quarter of each tile) runs with axes 0 and 1
+// swapped when that needs fewer tiles: rows = axis 1, column axes (axis 0,
+// axis 2), written through the output strides (rs = n2, c1s = n1 n2) in the
+// caller's C order; axis 2 stays the fastest, so stores stay coalesced.
+// Tile cost: a k_mfma_grid tile (per-lane gathers from the E tables) takes
+// about 2.2 k_mfma_grid3 tiles (512x512x8 against 512^3 per mode-point).
+struct MfmaPlan {
+    MfmaGeom g;
+    bool split;    // 2D grid, uniform last axis factorised (k_split_axis)
+    bool swapped;  // 3D grid run with axes 0 and 1 swapped
+};
+
+inline double mfma_tile_cost(const MfmaGeom& g) {
+    return (double)g.row_tiles * (double)g.col_tiles * (g.g2 == 1 ? 1.0 : 2.2);
+}
+
+inline MfmaPlan mfma_plan(int64_t d, int64_t N, const int64_t* shape, int64_t comps, bool uniform_last) {
+    MfmaPlan p;
+    p.split = uniform_last && d == 2 && comps == 1 && shape[1] >= kSplitMin;
+    p.swapped = false;
+    p.g = p.split ? mfma_split_geom(N, shape) : mfma_geom(d, N, shape);
+    if (d == 3 && shape[0] < shape[1]) {
+        const int64_t sw[3] = {shape[1], shape[0], shape[2]};
+        MfmaGeom gs = mfma_geom(3, N, sw);
+        if (mfma_tile_cost(gs) < 0.8 * mfma_tile_cost(p.g)) {
+            gs.rs = shape[2];
+            gs.c1s = shape[1] * shape[2];
+            p.g = gs;
+            p.swapped = true;
+        }
+    }
+    return p;
+}
+
+inline int64_t mfma_plan_ws_bytes(const MfmaPlan& p, int64_t comps) {
+    return p.split ? mfma_split_ws_bytes(p.g) : mfma_ws_bytes(p.g, comps);
 }
 
 // ---------------------------------------------------------------- splits ---
@@ -161,10 +204,12 @@ __device__ __forceinline__ void split2(float a, float b, uint32_t& hi, uint32_t&
 // comps > 1 (incompressible fields): E2 of component c carries the projector
 // weight p_c(k_j) = delta_c0 - k_jc k_j0 / |k_j|^2 (generator.py:479-494) in
 // place of the spectrum factor.
-// kz_row: the row of cov holding E2's wave-vector component (d - 1, or 1 for a split axis).
+// ka_row, k1_row, kz_row: the rows of cov holding the wave-vector components of A, E1
+// and E2 (0, 1, d - 1; 1, 1 for a split axis's two virtual axes; 1, 0, 2 with axes 0
+// and 1 swapped).  The projector weight always uses cov's own rows.
 template <bool FP16>
 __global__ __launch_bounds__(256) void k_mfma_tables(MfmaGeom g, int64_t d, int64_t N, int64_t comps,
-                                                     int64_t kz_row,
+                                                     int64_t ka_row, int64_t k1_row, int64_t kz_row,
                                                      const double* __restrict__ ax0,
                                                      const double* __restrict__ ax1,
                                                      const double* __restrict__ ax2,
@@ -183,7 +228,7 @@ __global__ __launch_bounds__(256) void k_mfma_tables(MfmaGeom g, int64_t d, int6
         float re = 0.0f, im = 0.0f;
         if (row < g.n0 && j < N) {
             double s, c;
-            const double th = cov[j] * ax0[row];
+            const double th = cov[ka_row * N + j] * ax0[row];
             if (fabs(th) <= kFastPhaseMax) sincos_fast(th, &s, &c); else sincos(th, &s, &c);
             re = (float)c;
             im = (float)(-s);
@@ -199,7 +244,7 @@ __global__ __launch_bounds__(256) void k_mfma_tables(MfmaGeom g, int64_t d, int6
         if (j < N) {
             if (d == 3) {
                 double s, c;
-                const double th = cov[N + j] * ax1[i1];
+                const double th = cov[k1_row * N + j] * ax1[i1];
                 if (fabs(th) <= kFastPhaseMax) sincos_fast(th, &s, &c); else sincos(th, &s, &c);
                 v = {(float)c, (float)s};
             } else {
@@ -294,16 +339,12 @@ __global__ __launch_bounds__(kMWaves * 64, 2) void k_mfma_grid(MfmaGeom g, doubl
     // Columns are flat (c = i1 * n2 + i2, 256 per tile: 16 per fragment), so a
     // short last axis (thin 3D slabs) wastes no lanes; the
     // clamped (i1, i2) of each fragment column index the tables.
-    int64_t col[kMCF];
-    bool cok[kMCF];
     const float2* e1p[kMCF];
     const float2* e2p[kMCF];
 #pragma unroll
     for (int cf = 0; cf < kMCF; ++cf) {
         const int64_t c = ct * 256 + 16 * (wave * kMCF + cf) + l16;
-        cok[cf] = c < g.ncols;
-        col[cf] = c;
-        const int64_t cc = cok[cf] ? c : g.ncols - 1;
+        const int64_t cc = c < g.ncols ? c : g.ncols - 1;
         const int64_t i1 = cc / g.n2, i2 = cc - i1 * g.n2;
         e1p[cf] = e1 + i1 * g.Np + 4 * grp;
         e2p[cf] = e2 + i2 * g.Np + 4 * grp;
@@ -402,6 +443,16 @@ __global__ __launch_bounds__(kMWaves * 64, 2) void k_mfma_grid(MfmaGeom g, doubl
     }
 
     const float fa = SPLIT ? 1.0f : (float)alpha;  // partial sums are scaled when added
+    // output offsets i1 * c1s + i2, computed after the loop (not held in registers through it)
+    int64_t col[kMCF];
+    bool cok[kMCF];
+#pragma unroll
+    for (int cf = 0; cf < kMCF; ++cf) {
+        const int64_t c = ct * 256 + 16 * (wave * kMCF + cf) + l16;
+        cok[cf] = c < g.ncols;
+        const int64_t i1 = c / g.n2;
+        col[cf] = i1 * g.c1s + (c - i1 * g.n2);
+    }
 #pragma unroll
     for (int rf = 0; rf < kMRF; ++rf) {
 #pragma unroll
@@ -410,7 +461,7 @@ __global__ __launch_bounds__(kMWaves * 64, 2) void k_mfma_grid(MfmaGeom g, doubl
             if (row < g.n0) {
 #pragma unroll
                 for (int cf = 0; cf < kMCF; ++cf)
-                    if (cok[cf]) out[row * g.out_cols + col[cf]] = fa * acc[rf][cf][i];
+                    if (cok[cf]) out[row * g.rs + col[cf]] = fa * acc[rf][cf][i];
             }
         }
     }
@@ -574,14 +625,14 @@ __global__ __launch_bounds__(kMWaves * 64, 2) void k_mfma_grid3(MfmaGeom g, doub
     for (int cf = 0; cf < kMCF; ++cf) {
         const int64_t i1 = i1b + wave * kMCF + cf;
         if (i1 >= g.n1 || i2 >= g.n2) continue;
-        const int64_t col = i1 * g.n2 + i2;
-        if (col >= g.out_cols) continue;  // past the end of a split axis
+        if (i1 * g.n2 + i2 >= g.out_cols) continue;  // past the end of a split axis
+        const int64_t col = i1 * g.c1s + i2;
 #pragma unroll
         for (int rf = 0; rf < kMRF; ++rf) {
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
                 const int64_t row = row0 + rf * 16 + grp * 4 + i;
-                if (row < g.n0) out[row * g.out_cols + col] = fa * acc[rf][cf][i];
+                if (row < g.n0) out[row * g.rs + col] = fa * acc[rf][cf][i];
             }
         }
     }
@@ -635,7 +686,8 @@ __global__ __launch_bounds__(256) void k_split_axis(const double* __restrict__ y
 }
 
 template <bool FP16>
-int mfma_launch(const MfmaGeom& g, int64_t d, int64_t N, int64_t comps, int64_t kz_row, const double* ax0,
+int mfma_launch(const MfmaGeom& g, int64_t d, int64_t N, int64_t comps, int64_t ka_row, int64_t k1_row,
+                int64_t kz_row, const double* ax0,
                 const double* ax1, const double* ax2, const double* cov, const double* z1, const double* z2,
                 const double* fac, double alpha, float* out, void* ws, hipStream_t s) {
     uint32_t* a_hi = static_cast<uint32_t*>(ws);
@@ -644,7 +696,7 @@ int mfma_launch(const MfmaGeom& g, int64_t d, int64_t N, int64_t comps, int64_t 
     float2* e2 = e1 + g.e1_elems;
     const int64_t entries = g.n0p * g.Np + g.e1_elems + comps * g.e2_elems;
     hipLaunchKernelGGL((k_mfma_tables<FP16>), dim3((unsigned)ceil_div(entries, 256)), dim3(256), 0, s, g, d,
-                       N, comps, kz_row, ax0, ax1, ax2, cov, z1, z2, fac, a_hi, a_lo, e1, e2);
+                       N, comps, ka_row, k1_row, kz_row, ax0, ax1, ax2, cov, z1, z2, fac, a_hi, a_lo, e1, e2);
     int st = launch_status();
     if (st != GSX_OK) return st;
     const int64_t steps_per = mfma_steps_per_split(g, comps);
@@ -708,17 +760,23 @@ int mfma_entry(int64_t d, int64_t N, const int64_t* shape, const double* axes, c
                            comps * P);
         return launch_status();
     }
-    const bool split = uniform_last && d == 2 && comps == 1 && shape[1] >= kSplitMin;
-    const MfmaGeom g = split ? mfma_split_geom(N, shape) : mfma_geom(d, N, shape);
+    const MfmaPlan plan = mfma_plan(d, N, shape, comps, uniform_last);
+    const MfmaGeom& g = plan.g;
+    const bool split = plan.split;
     if (mfma_ksplit(g, comps) * comps * g.row_tiles * g.col_tiles > 0x7fffffffLL ||
         ceil_div(comps * P, (int64_t)256) > 0x7fffffffLL)
         return GSX_E_INVALID;
-    if (!workspace || workspace_bytes < (split ? mfma_split_ws_bytes(g) : mfma_ws_bytes(g, comps)))
-        return GSX_E_WORKSPACE;
+    if (!workspace || workspace_bytes < mfma_plan_ws_bytes(plan, comps)) return GSX_E_WORKSPACE;
     const double* ax0 = axes;
     const double* ax1 = axes + shape[0];
     const double* ax2 = d == 3 ? ax1 + shape[1] : nullptr;
-    int64_t dk = d, kz_row = d - 1;
+    int64_t dk = d, ka_row = 0, k1_row = 1, kz_row = d - 1;
+    if (plan.swapped) {
+        ax0 = axes + shape[0];
+        ax1 = axes;
+        ka_row = 1;
+        k1_row = 0;
+    }
     if (split) {
         double* vy = reinterpret_cast<double*>(static_cast<char*>(workspace) +
                                                ceil_div(mfma_ws_bytes(g), (int64_t)256) * 256);
@@ -732,9 +790,10 @@ int mfma_entry(int64_t d, int64_t N, const int64_t* shape, const double* axes, c
         kz_row = 1;  // both virtual axes carry the mode's k_1
     }
     if (variant == GSX_MFMA_FP16X3)
-        return mfma_launch<true>(g, dk, N, comps, kz_row, ax0, ax1, ax2, cov, z1, z2, fac, alpha, out, workspace,
-                                 s);
-    return mfma_launch<false>(g, dk, N, comps, kz_row, ax0, ax1, ax2, cov, z1, z2, fac, alpha, out, workspace, s);
+        return mfma_launch<true>(g, dk, N, comps, ka_row, k1_row, kz_row, ax0, ax1, ax2, cov, z1, z2, fac, alpha,
+                                 out, workspace, s);
+    return mfma_launch<false>(g, dk, N, comps, ka_row, k1_row, kz_row, ax0, ax1, ax2, cov, z1, z2, fac, alpha,
+                              out, workspace, s);
 }
 
 }  // namespace gsx
@@ -745,11 +804,9 @@ int64_t gsx_summate_grid_mfma_workspace(int64_t d, int64_t N, const int64_t* sha
     if ((d != 2 && d != 3) || N < 0 || !shape) return GSX_E_INVALID;
     for (int i = 0; i < d; ++i)
         if (shape[i] < 0) return GSX_E_INVALID;
-    const gsx::MfmaGeom g = gsx::mfma_geom(d, N, shape);
-    const int64_t ws = gsx::mfma_ws_bytes(g);
-    if (d != 2 || shape[1] < gsx::kSplitMin) return ws;
+    const int64_t ws = gsx::mfma_plan_ws_bytes(gsx::mfma_plan(d, N, shape, 1, false), 1);
     // enough for GSX_MFMA_UNIFORM_LAST as well
-    const int64_t ws_split = gsx::mfma_split_ws_bytes(gsx::mfma_split_geom(N, shape));
+    const int64_t ws_split = gsx::mfma_plan_ws_bytes(gsx::mfma_plan(d, N, shape, 1, true), 1);
     return ws > ws_split ? ws : ws_split;
 }
 
@@ -764,8 +821,7 @@ int64_t gsx_summate_incompr_grid_mfma_workspace(int64_t d, int64_t N, const int6
     if ((d != 2 && d != 3) || N < 0 || !shape) return GSX_E_INVALID;
     for (int i = 0; i < d; ++i)
         if (shape[i] < 0) return GSX_E_INVALID;
-    const gsx::MfmaGeom g = gsx::mfma_geom(d, N, shape);
-    return gsx::mfma_ws_bytes(g, d);
+    return gsx::mfma_plan_ws_bytes(gsx::mfma_plan(d, N, shape, d, false), d);
 }
 
 int gsx_summate_incompr_grid_mfma(int64_t d, int64_t N, const int64_t* shape, const double* axes, const double* cov,

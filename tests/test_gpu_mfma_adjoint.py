@@ -50,6 +50,8 @@ def _modes(d, n, seed, kscale=1.0):
     ((40, 8, 300), 200),     # 3D, short middle axis: flat columns (generic kernel)
     ((64, 300, 7), 256),     # 3D, short last axis: flat columns across i1 rows
     ((33, 5, 5), 64),        # both column axes short: one ragged column tile
+    ((20, 300, 40), 200),    # short axis 0: axes 0 and 1 swapped (rows = axis 1)
+    ((5, 200, 17), 100),     # swapped, and the swapped column axes need the flat kernel
 ])
 @pytest.mark.parametrize("precision", ["fp16x3", "bf16x3"])
 def test_mfma_grid_matches_fp64(shape, n, precision):

@@ -73,7 +73,7 @@ def test_srf_incompressible_known_values():
     assert field[0, 1, 0] == pytest.approx(1.06180674, abs=5e-8)
 
 
-@pytest.mark.parametrize("dim,shape", [(2, (37, 29)), (3, (24, 20, 18)), (3, (30, 9, 5))])
+@pytest.mark.parametrize("dim,shape", [(2, (37, 29)), (3, (24, 20, 18)), (3, (30, 9, 5)), (3, (6, 140, 20))])
 def test_incompressible_structured_all_precisions(dim, shape):
     """``IncomprRandMeth.structured``: each component is a grid sum with the per-mode
     projector weight; FP64 within 1e-12 of the incompressible sum on the materialised
