@@ -145,3 +145,29 @@ def test_uniform_axis_check():
     shp = lambda *s: (ctypes.c_int64 * 3)(*(list(s) + [1] * (3 - len(s))))
     assert q(2, 100, shp(20, 256)) >= q(3, 100, shp(20, 16, 16))
     assert q(2, 4096, shp(4096, 4096)) >= q(3, 4096, shp(4096, 256, 16))
+
+
+def test_mfma_plan_through_workspace_sizes():
+    """The matrix-core grid plan is host arithmetic, visible in the workspace size:
+    a 3D grid with a short axis 0 runs with axes 0 and 1 swapped (A table of the
+    long axis, and here a K split that needs partial sums), a cubic grid keeps its
+    order, and a thin slab keeps axis 0 as rows."""
+    q = _native.lib().gsx_summate_grid_mfma_workspace
+    shp = lambda *s: (ctypes.c_int64 * 3)(*(list(s) + [1] * (3 - len(s))))
+    a256 = lambda b: -(-b // 256) * 256
+
+    def tables(rows, n1, n2, N=4096):
+        rows_p = -(-rows // 128) * 128
+        return 2 * (rows_p * 2 * N) * 2 + (n1 * N + n2 * N) * 8
+
+    # (32, 512, 512): swapped -> rows 512, column axes (32, 512): 4 x 64 = 256 blocks,
+    # K split in 2 -> tables + 2 partial sums of P floats
+    P = 32 * 512 * 512
+    assert q(3, 4096, shp(32, 512, 512)) == a256(tables(512, 32, 512)) + 2 * P * 4
+    # unswapped it would be rows 32 (one 128-row tile) x 1024 column tiles: no K split
+    assert q(3, 4096, shp(32, 512, 512)) != tables(32, 512, 512)
+    # (512, 512, 512) and a thin slab (512, 512, 8) keep axis 0 as rows
+    assert q(3, 4096, shp(512, 512, 512)) == tables(512, 512, 512)
+    slab = q(3, 4096, shp(512, 512, 8))
+    ks = 512 // (4 * 16)  # 4 row tiles x 16 flat column tiles of 256 -> 64 blocks
+    assert slab == a256(tables(512, 512, 8)) + ks * 512 * 512 * 8 * 4
