@@ -175,7 +175,7 @@ def bench_incompr_grid(torch, steps, warmup):
 def bench_fourier(torch, steps, warmup):
     """Periodic 2D Gaussian (var 1, len 200) on a 4096^2 grid, period 4096, mode_no 256^2 (Fourier method)."""
     import oracle
-    from gstools_amd import Fourier, Gaussian, summate_fourier_grid_device
+    from gstools_amd import Fourier, Gaussian, summate_fourier_grid_device, summate_fourier_tensor_device
 
     log("bench: fourier grid (4096^2, 256^2 modes)")
     side, m = 4096, 256
@@ -218,6 +218,20 @@ def bench_fourier(torch, steps, warmup):
     wall64, _ = time_kernel(f64, 2, 1, torch)
     res["fp64_direct"] = {"value": 128 * side * float(nm) / wall64, "measured_on": "128 of 4096 x-rows"}
     res["speedup_bf16x3_vs_fp64"] = res["bf16x3"]["value"] / res["fp64_direct"]["value"]
+    # the modes are a tensor grid: one FP64 GEMM contraction per axis (same sum, FP64)
+    kax = [torch.from_numpy(k).to(dev) for k in gen._kaxes]
+    out64 = torch.empty(n, dtype=torch.float64, device=dev)
+    ften = lambda: summate_fourier_tensor_device(fac, kax, z1, z2, axes, out=out64)
+    wallt, kert = time_kernel(ften, steps, warmup, torch)
+    got = out64.reshape(side, side)[rows].cpu().numpy()
+    # GEMM flops: first stage (2 side x 2 m) . (2 m x m), last stage (side x 2 m) . (2 m x side)
+    flops = 8.0 * side * m * m + 4.0 * side * side * m
+    res["tensor_fp64"] = {"value": work / wallt, "ms_per_step": wallt * 1e3,
+                          "rel_max_vs_fp64_direct": oracle.rel_errors(got, ref)[0],
+                          "rel_max_vs_oracle": oracle.rel_errors(got[:, cols].ravel(), ora)[0],
+                          "gemm_flops": flops, "fp64_tflops": flops / kert / 1e12,
+                          "note": "per-axis contraction (modes on a tensor grid), FP64 matrix cores"}
+    res["speedup_tensor_vs_bf16x3"] = res["tensor_fp64"]["value"] / res["bf16x3"]["value"]
     return res
 
 

@@ -44,6 +44,7 @@ from .summate import (  # noqa: F401
     summate_fourier,
     summate_fourier_device,
     summate_fourier_grid_device,
+    summate_fourier_tensor_device,
     summate_grid_device,
     summate_incompr,
     summate_incompr_device,

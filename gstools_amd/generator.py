@@ -32,6 +32,7 @@ from .summate import (
     GRID_PRECISIONS,
     PRECISIONS,
     summate_device,
+    summate_fourier_tensor_device,
     summate_grid_device,
     summate_incompr_grid_device,
     summate_incompr_device,
@@ -426,18 +427,31 @@ class Fourier(Generator):
             nugget = torch.as_tensor(nugget, device=field.device)
         return field + nugget
 
-    def structured(self, axes, precision=None, add_nugget=True):
+    def structured(self, axes, precision=None, add_nugget=True, method="tensor"):
         """Field on the 'ij' meshgrid of ``axes`` without materialising it.
 
-        As :meth:`RandMeth.structured`: ``"bf16x3"`` / ``"fp16x3"`` run on the
-        matrix cores (d = 2 or 3); returns a torch tensor shaped like the grid.
+        ``method="tensor"`` (default): the modes are a tensor grid, so the sum
+        factorises into one FP64 GEMM contraction per axis on the matrix cores
+        (``summate_fourier_tensor_device``); FP64 whatever ``precision`` says,
+        which only picks the output dtype (float64 for "fp64" / "fp32", float32
+        for "bf16x3" / "fp16x3").  ``method="direct"``: the per-point sum as in
+        :meth:`RandMeth.structured` (``"bf16x3"`` / ``"fp16x3"`` on the matrix
+        cores).  Returns a torch tensor shaped like the grid.
         """
         precision = precision or self.precision
         if precision not in GRID_PRECISIONS:
             raise ValueError(f"precision must be one of {GRID_PRECISIONS}")
+        if method not in ("tensor", "direct"):
+            raise ValueError("method must be 'tensor' or 'direct'")
         shape = tuple(int(np.size(a)) for a in axes)
         if self.zero_var:
             out = torch.zeros(shape, dtype=torch.float64, device="cuda")
+        elif method == "tensor":
+            fac, _, z1, z2 = self._device_modes()
+            out = summate_fourier_tensor_device(fac, self._kaxes, z1, z2, axes)
+            if precision in ("bf16x3", "fp16x3"):
+                out = out.to(torch.float32)
+            out = out.reshape(shape)
         else:
             fac, modes, z1, z2 = self._device_modes()
             out = summate_grid_device(modes, z1, z2, axes, precision, spectrum_factor=fac)
@@ -531,6 +545,7 @@ class Fourier(Generator):
         axes = [np.arange(-mode_no[d] / 2.0 * self._delta_k[d], mode_no[d] / 2.0 * self._delta_k[d],
                           self._delta_k[d]) for d in range(dim)]
         self._modes = np.asarray(np.meshgrid(*axes, indexing="ij"), dtype=np.double).reshape((dim, -1))
+        self._kaxes = [np.asarray(m, dtype=np.double) for m in axes]
         self._mode_no = [len(m) for m in axes]
 
     @property

@@ -25,7 +25,7 @@ from . import _native
 __all__ = [
     "summate", "summate_incompr", "summate_device", "summate_incompr_device",
     "summate_grid_device", "summate_adjoint_device", "summate_fourier", "summate_fourier_device",
-    "summate_fourier_grid_device", "PRECISIONS", "GRID_PRECISIONS",
+    "summate_fourier_grid_device", "summate_fourier_tensor_device", "PRECISIONS", "GRID_PRECISIONS",
 ]
 
 PRECISIONS = ("fp64", "fp32")
@@ -294,6 +294,47 @@ def summate_fourier_grid_device(spectrum_factor, modes, z_1, z_2, axes, precisio
     if spectrum_factor is None:
         raise ValueError("spectrum_factor is required")
     return summate_grid_device(modes, z_1, z_2, axes, precision, alpha, out, spectrum_factor=spectrum_factor)
+
+
+def summate_fourier_tensor_device(spectrum_factor, kaxes, z_1, z_2, axes, out=None):
+    """Fourier-method sum on the 'ij' meshgrid of ``axes`` for modes on a tensor grid:
+    ``kaxes`` holds one wavenumber array per axis and the modes are their 'ij' meshgrid
+    (``spectrum_factor``, ``z_1``, ``z_2`` over all of them, C order), as the Fourier
+    generator builds them (``generator.py:838-862``).  One FP64 GEMM contraction per
+    axis on the matrix cores (``gsx_summate_fourier_tensor_f64``); returns the flat
+    float64 sum in C order."""
+    dev = _device()
+    kax = [_as_f64(k, "kaxes", dev).reshape(-1) for k in kaxes]
+    axs = [_as_f64(a, "axes", dev).reshape(-1) for a in axes]
+    d = len(axs)
+    if not 1 <= d <= 3 or len(kax) != d:
+        raise ValueError("one wavenumber axis per grid axis, 1 to 3 axes")
+    mode_no = [int(k.numel()) for k in kax]
+    n = int(np.prod(mode_no))
+    fac = _as_f64(spectrum_factor, "spectrum_factor", dev).reshape(-1)
+    z1 = _as_f64(z_1, "z_1", dev).reshape(-1)
+    z2 = _as_f64(z_2, "z_2", dev).reshape(-1)
+    if fac.numel() != n or z1.numel() != n or z2.numel() != n:
+        raise ValueError(f"spectrum_factor, z_1, z_2 must have prod(mode_no) = {n} entries")
+    shape = [int(a.numel()) for a in axs]
+    npts = int(np.prod(shape))
+    if out is None:
+        out = torch.empty(npts, dtype=torch.float64, device=dev)
+    elif out.dtype != torch.float64 or out.numel() != npts or not out.is_contiguous():
+        raise ValueError("out: wrong dtype, size or layout")
+    m_c = (ctypes.c_int64 * 3)(*(mode_no + [1] * (3 - d)))
+    s_c = (ctypes.c_int64 * 3)(*(shape + [1] * (3 - d)))
+    lib = _native.lib()
+    ws_bytes = int(lib.gsx_summate_fourier_tensor_workspace(d, m_c, s_c))
+    if ws_bytes < 0:
+        _native.check(ws_bytes, "summate_fourier_tensor")
+    ws = torch.empty(max(ws_bytes, 1), dtype=torch.uint8, device=dev)
+    flat_k = torch.cat(kax) if n else torch.zeros(1, dtype=torch.float64, device=dev)
+    flat_x = torch.cat(axs)
+    _native.check(lib.gsx_summate_fourier_tensor_f64(d, m_c, _ptr(flat_k), s_c, _ptr(flat_x), _ptr(fac), _ptr(z1),
+                                                      _ptr(z2), _ptr(out), _ptr(ws), ws_bytes, _stream()),
+                  "summate_fourier_tensor")
+    return out
 
 
 def summate_adjoint_device(cov_samples, z_1, z_2, pos, g, precision="fp64"):

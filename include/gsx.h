@@ -16,8 +16,8 @@
  *     src/gstools/field/generator.py:529-567      gsx_summate_incompr_grid_mfma
  *   summate_fourier(spectrum_factor, modes,       gsx_summate_fourier_f64 / _f32,
  *     z_1, z_2, pos)                              gsx_summate_fourier_grid_f64 / _f32,
- *     src/gstools/field/generator.py:23, :67-75,  gsx_summate_fourier_grid_mfma
- *     call :681-688
+ *     src/gstools/field/generator.py:23, :67-75,  gsx_summate_fourier_grid_mfma,
+ *     call :681-688                               gsx_summate_fourier_tensor_f64
  *   (no reference counterpart: gradient of        gsx_summate_adjoint_f64 / _f32
  *    summate for calibration; formula is the
  *    derivative of generator.py:193-199)
@@ -194,6 +194,26 @@ int gsx_summate_fourier_grid_mfma(int64_t d, int64_t N, const int64_t* shape,
                                   const double* z2, int variant, double alpha,
                                   float* out, void* workspace,
                                   int64_t workspace_bytes, void* stream);
+
+/* Fourier method on a structured grid when the modes are a tensor grid (the
+ * Fourier generator's own: the 'ij' meshgrid of one wavenumber axis per
+ * dimension, generator.py:838-862, summed at generator.py:659-691).  The sum
+ * factorises into one contraction per axis, run as FP64 GEMMs on the matrix
+ * cores: mode_no[d] wavenumbers per axis, kaxes = their concatenation
+ * (sum(mode_no) doubles), spectrum_factor / z1 / z2 over all prod(mode_no)
+ * modes in the meshgrid's C order; shape / axes as for the grid entries;
+ * out (prod(shape),) float64, the unscaled sum.  FP64 throughout; it differs
+ * from the per-point sum only by the rounding of the phase sum.  d in 1..3.
+ * Workspace bytes from gsx_summate_fourier_tensor_workspace(). */
+int64_t gsx_summate_fourier_tensor_workspace(int64_t d, const int64_t* mode_no,
+                                             const int64_t* shape);
+int gsx_summate_fourier_tensor_f64(int64_t d, const int64_t* mode_no,
+                                   const double* kaxes, const int64_t* shape,
+                                   const double* axes,
+                                   const double* spectrum_factor,
+                                   const double* z1, const double* z2,
+                                   double* out, void* workspace,
+                                   int64_t workspace_bytes, void* stream);
 
 /* Adjoint of summate for an upstream gradient g (P,):
  *   gz1[j]   = sum_i g_i cos(k_j.x_i)

@@ -48,10 +48,13 @@ def test_known_values_and_periodicity(dim):
     model, mode_no, period, axes, want = KNOWN[dim]
     srf = SRF(model(), generator="Fourier", mode_no=mode_no, period=period, seed=SEED)
     mesh = srf(generate_grid(axes)).reshape([a.size for a in axes])  # meshgrid route
-    grid = srf(axes, mesh_type="structured")                         # grid kernel
+    grid = srf(axes, mesh_type="structured")                         # per-axis FP64 GEMMs
+    direct = srf.generator.structured(axes, method="direct").cpu().numpy()  # grid kernel
     assert grid.shape == mesh.shape
     assert mesh.flat[0] == pytest.approx(want, abs=1e-12)
-    assert np.array_equal(grid, mesh)
+    assert grid.flat[0] == pytest.approx(want, abs=1e-12)
+    assert np.array_equal(direct, mesh)
+    assert np.max(np.abs(grid - mesh)) <= 1e-12 * np.max(np.abs(mesh))
     if dim == 1:
         assert grid[0] == pytest.approx(grid[-1], abs=1e-7)
     elif dim == 2:
@@ -108,9 +111,54 @@ def test_grid_paths_match_oracle(shape, mode_no, precision):
         assert rel_max <= 1e-5 and rel_rms <= 1e-5
     else:
         assert rel_rms <= 1e-5 and rel_max <= 5e-5
-    st = gen.structured(axes, precision=precision, add_nugget=False).cpu().numpy().astype(np.float64)
+    st = gen.structured(axes, precision=precision, add_nugget=False, method="direct").cpu().numpy()
     assert st.shape == shape
-    assert np.array_equal(st.ravel(), got)
+    assert np.array_equal(st.astype(np.float64).ravel(), got)
+
+
+@pytest.mark.parametrize("shape,mode_no", [
+    ((37,), [10]),                 # 1D: the last stage on the coefficients
+    ((64, 300), [24, 20]),         # 2D: first and last stage
+    ((200, 129), [130, 6]),        # more modes than one K stage, ragged tiles
+    ((37, 40, 23), [8, 6, 10]),    # 3D: first, batched middle and last stage
+    ((130, 17, 33), [4, 4, 4]),
+    ((5, 3, 300), [2, 30, 18]),
+])
+def test_tensor_path_matches_oracle(shape, mode_no):
+    """The modes are a tensor grid, so the sum is one FP64 GEMM contraction per axis:
+    within 1e-12 of the oracle's per-point sum (it skips only the rounding of the phase
+    sum), through the generator and the device API; the output dtype follows precision."""
+    from gstools_amd import summate_fourier_tensor_device
+
+    _say(f"fourier tensor shape={shape} modes={mode_no}")
+    d = len(shape)
+    period = [float(s) * 0.7 for s in shape]
+    gen = _gen(d, mode_no, period, 3.0, 11 + d)
+    axes = [np.linspace(-2.0, 0.7 * s, s) for s in shape]
+    ref = oracle.summate_fourier_c(gen._spectrum_factor, gen._modes, gen._z_1, gen._z_2, generate_grid(axes))
+    got = gen.structured(axes, add_nugget=False).cpu().numpy()
+    assert got.shape == shape and got.dtype == np.float64
+    assert oracle.rel_errors(got.ravel(), ref)[0] <= 1e-12
+    dev = summate_fourier_tensor_device(gen._spectrum_factor, gen._kaxes, gen._z_1, gen._z_2, axes)
+    assert np.array_equal(dev.cpu().numpy(), got.ravel())
+    lo = gen.structured(axes, precision="bf16x3", add_nugget=False)
+    assert lo.dtype == torch.float32 and np.array_equal(lo.cpu().numpy(), got.astype(np.float32))
+
+
+def test_tensor_path_errors_and_empty():
+    from gstools_amd import summate_fourier_tensor_device
+
+    k = [np.arange(4.0), np.arange(3.0)]
+    z = np.ones(12)
+    with pytest.raises(ValueError):
+        summate_fourier_tensor_device(z, k, z, z, [np.arange(5.0)])          # axes / wavenumber axes
+    with pytest.raises(ValueError):
+        summate_fourier_tensor_device(z[:11], k, z, z, [np.arange(5.0)] * 2)  # mode count
+    zero = summate_fourier_tensor_device(np.zeros(0), [np.zeros(0), np.arange(3.0)], np.zeros(0), np.zeros(0),
+                                         [np.arange(5.0)] * 2)
+    assert np.array_equal(zero.cpu().numpy(), np.zeros(25))
+    empty = summate_fourier_tensor_device(z, k, z, z, [np.zeros(0), np.arange(5.0)])
+    assert empty.numel() == 0
 
 
 def test_empty_and_errors():

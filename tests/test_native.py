@@ -171,3 +171,19 @@ def test_mfma_plan_through_workspace_sizes():
     slab = q(3, 4096, shp(512, 512, 8))
     ks = 512 // (4 * 16)  # 4 row tiles x 16 flat column tiles of 256 -> 64 blocks
     assert slab == a256(tables(512, 512, 8)) + ks * 512 * 512 * 8 * 4
+
+
+def test_fourier_tensor_workspace():
+    """Workspace of the per-axis Fourier contraction: coefficients, rotation tables,
+    intermediate and last-axis table, each 256-byte aligned (host arithmetic)."""
+    q = _native.lib().gsx_summate_fourier_tensor_workspace
+    c3 = lambda *s: (ctypes.c_int64 * 3)(*(list(s) + [1] * (3 - len(s))))
+    al = lambda n: -(-n // 32) * 32
+    # d = 2, modes (24, 20), grid (64, 300): c 2*480, E1 (2*64 x 2*24), T1 2*64*20, last 2*20*300
+    assert q(2, c3(24, 20), c3(64, 300)) == 8 * (al(960) + al(6144) + al(2560) + al(12000))
+    # d = 3, modes (8, 6, 10), grid (37, 40, 23)
+    want = al(2 * 480) + al(2 * 37 * 2 * 8) + al(2 * 40 * 2 * 6) + al(2 * 37 * 60) + al(37 * 2 * 40 * 10) + al(2 * 10 * 23)
+    assert q(3, c3(8, 6, 10), c3(37, 40, 23)) == 8 * want
+    assert q(1, c3(10), c3(37)) == 8 * (al(20) + al(2 * 10 * 37))
+    assert q(4, c3(1, 1, 1), c3(1, 1, 1)) < 0 and q(2, c3(-1, 2), c3(3, 3)) < 0
+    assert q(2, c3(0, 3), c3(5, 5)) >= 0 and q(3, c3(4, 0, 2), c3(5, 0, 5)) >= 0  # no modes / points
