@@ -232,6 +232,28 @@ def bench_fourier(torch, steps, warmup):
                           "gemm_flops": flops, "fp64_tflops": flops / kert / 1e12,
                           "note": "per-axis contraction (modes on a tensor grid), FP64 matrix cores"}
     res["speedup_tensor_vs_bf16x3"] = res["tensor_fp64"]["value"] / res["bf16x3"]["value"]
+    # 3D: periodic Gaussian on a 256^3 grid with 64^3 modes (first, batched middle and last stage)
+    side3, m3 = 256, 64
+    gen3 = Fourier(Gaussian(dim=3, var=1.0, len_scale=20.0), period=float(side3), mode_no=m3, seed=1681903)
+    f3, z13, z23 = (torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+                    for a in (gen3._spectrum_factor, gen3._z_1, gen3._z_2))
+    kax3 = [torch.from_numpy(k).to(dev) for k in gen3._kaxes]
+    ax3 = np.linspace(0.0, float(side3), side3, endpoint=False)
+    axes3 = [torch.from_numpy(ax3).to(dev)] * 3
+    out3 = torch.empty(side3 ** 3, dtype=torch.float64, device=dev)
+    f3d = lambda: summate_fourier_tensor_device(f3, kax3, z13, z23, axes3, out=out3)
+    wall3, ker3 = time_kernel(f3d, steps, warmup, torch)
+    idx = np.array([0, 1, 77, 128, 255])
+    sel = np.stack(np.meshgrid(idx, idx, idx, indexing="ij")).reshape(3, -1)
+    ora3 = oracle.summate_fourier_c(gen3._spectrum_factor, gen3._modes, gen3._z_1, gen3._z_2, ax3[sel])
+    got3 = out3.reshape(side3, side3, side3)[tuple(torch.from_numpy(sel).to(dev))].cpu().numpy()
+    work3 = float(side3) ** 3 * m3 ** 3
+    flops3 = 2.0 * (2 * side3 * m3 * m3 * 2 * m3 + side3 * 2 * side3 * m3 * 2 * m3 + side3 * side3 * side3 * 2 * m3)
+    res["tensor_fp64_3d"] = {"workload": "periodic 3D Gaussian var 1 len 20, grid 256^3, period 256, mode_no 64^3",
+                             "value": work3 / wall3, "ms_per_step": wall3 * 1e3,
+                             "rel_max_vs_oracle_125_points": oracle.rel_errors(got3, ora3)[0],
+                             "gemm_flops": flops3, "fp64_tflops": flops3 / ker3 / 1e12,
+                             "per_point_bf16x3_estimate_ms": 12.0 * work3 / (res["bf16x3"]["mfma_tflops"] * 1e12) * 1e3}
     return res
 
 
