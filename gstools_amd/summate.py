@@ -25,7 +25,8 @@ from . import _native
 __all__ = [
     "summate", "summate_incompr", "summate_device", "summate_incompr_device",
     "summate_grid_device", "summate_adjoint_device", "summate_fourier", "summate_fourier_device",
-    "summate_fourier_grid_device", "summate_fourier_tensor_device", "PRECISIONS", "GRID_PRECISIONS",
+    "summate_fourier_grid_device", "summate_fourier_tensor_device", "tensor_grid_axes", "PRECISIONS",
+    "GRID_PRECISIONS",
 ]
 
 PRECISIONS = ("fp64", "fp32")
@@ -379,12 +380,64 @@ def summate(cov_samples, z_1, z_2, pos, num_threads=None):
     return summate_device(cov_samples, z_1, z_2, pos, "fp64").cpu().numpy()
 
 
+# the numpy Fourier seam takes the per-axis contraction for meshgrids this large
+TENSOR_MIN_POINTS = 4096
+TENSOR_MIN_MODES = 64
+
+
+def tensor_grid_axes(pts):
+    """The axes of ``pts`` (d, P) if it is exactly the flattened 'ij' meshgrid of one
+    array per dimension (as ``generate_grid`` and the Fourier generator's modes are),
+    else None.  Candidate sizes come from where the outer coordinates first change;
+    the candidate is then checked entry by entry, so repeated axis values can only
+    make it answer None."""
+    pts = np.asarray(pts)
+    if pts.ndim != 2 or pts.shape[1] == 0:
+        return None
+    d, n = pts.shape
+    sizes = [0] * d
+    inner = 1
+    for a in range(d - 1, 0, -1):  # run of constant outer coordinates = product of inner sizes
+        change = np.flatnonzero((pts[:a] != pts[:a, :1]).any(axis=0))
+        run = int(change[0]) if change.size else n
+        if run % inner:
+            return None
+        sizes[a] = run // inner
+        inner = run
+    if n % inner:
+        return None
+    sizes[0] = n // inner
+    axes = []
+    for a in range(d):
+        step = int(np.prod(sizes[a + 1:], dtype=np.int64))
+        ax = pts[a, ::step][:sizes[a]]
+        shape = [1] * d
+        shape[a] = sizes[a]
+        if not np.array_equal(pts[a].reshape(sizes), np.broadcast_to(ax.reshape(shape), sizes)):
+            return None
+        axes.append(np.ascontiguousarray(ax))
+    return axes
+
+
 def summate_fourier(spectrum_factor, modes, z_1, z_2, pos, num_threads=None):
-    """Drop-in for the backend ``summate_fourier``: FP64 sum on the GPU -> numpy (P,)."""
+    """Drop-in for the backend ``summate_fourier``: FP64 sum on the GPU -> numpy (P,).
+
+    When ``modes`` and ``pos`` are both flattened 'ij' meshgrids in 2D or 3D (the
+    Fourier generator on a structured mesh) of at least ``TENSOR_MIN_POINTS`` points
+    and ``TENSOR_MIN_MODES`` modes, the sum runs as one FP64 GEMM contraction per
+    axis (``summate_fourier_tensor_device``); otherwise as the per-point sum."""
     _check_numpy(modes, z_1, z_2, pos)
     if isinstance(spectrum_factor, np.ndarray) and spectrum_factor.dtype != np.float64:
         raise ValueError(f"Buffer dtype mismatch for spectrum_factor: expected 'double' but got "
                          f"'{spectrum_factor.dtype}'")
+    if isinstance(modes, np.ndarray) and isinstance(pos, np.ndarray) and modes.ndim == 2 and pos.ndim == 2 \
+            and 2 <= modes.shape[0] == pos.shape[0] <= 3 and pos.shape[1] >= TENSOR_MIN_POINTS \
+            and modes.shape[1] >= TENSOR_MIN_MODES and np.size(spectrum_factor) == modes.shape[1] \
+            and np.size(z_1) == modes.shape[1] and np.size(z_2) == modes.shape[1]:
+        kax = tensor_grid_axes(modes)
+        xax = tensor_grid_axes(pos) if kax is not None else None
+        if xax is not None:
+            return summate_fourier_tensor_device(spectrum_factor, kax, z_1, z_2, xax).cpu().numpy()
     return summate_fourier_device(spectrum_factor, modes, z_1, z_2, pos, "fp64").cpu().numpy()
 
 

@@ -161,3 +161,28 @@ def test_zero_variance_factors():
     gen = Fourier(Gaussian(dim=2, var=0.0, len_scale=3.0), period=[10, 10], mode_no=[4, 4], seed=1)
     assert np.array_equal(gen._spectrum_factor, np.zeros(16))
     assert np.array_equal(gen(np.zeros((2, 5)), add_nugget=False), np.zeros(5))
+
+
+def test_tensor_grid_detection():
+    """``tensor_grid_axes`` recovers the axes of exact 'ij' meshgrids (generate_grid and
+    the generator's modes), including one-point axes, and answers None for anything
+    else: a perturbed entry, scattered points, repeated axis values (the per-point sum
+    then runs)."""
+    from gstools_amd.summate import tensor_grid_axes
+
+    x, y, z = np.linspace(0, 1, 7), np.arange(5.0), np.array([2.0, -1.0, 3.0])
+    for axes in ([x], [x, y], [x, y, z], [y, x], [np.array([1.0])], [x, np.array([1.0])],
+                 [np.array([1.0]), y, z]):
+        got = tensor_grid_axes(generate_grid(axes))
+        assert got is not None and len(got) == len(axes)
+        assert all(np.array_equal(a, b) for a, b in zip(got, axes))
+    gen = Fourier(Gaussian(dim=3, var=1.0, len_scale=4.0), period=[20, 30, 40], mode_no=[4, 6, 8], seed=3)
+    kax = tensor_grid_axes(gen._modes)
+    assert [k.size for k in kax] == [4, 6, 8]
+    assert all(np.array_equal(a, b) for a, b in zip(kax, gen._kaxes))
+    g = generate_grid([x, y])
+    g[1, 7] += 1e-9
+    assert tensor_grid_axes(g) is None
+    assert tensor_grid_axes(np.random.RandomState(0).uniform(size=(2, 35))) is None
+    assert tensor_grid_axes(generate_grid([np.array([0.0, 0.0, 1.0]), y])) is None
+    assert tensor_grid_axes(np.zeros((2, 0))) is None

@@ -145,6 +145,26 @@ def test_tensor_path_matches_oracle(shape, mode_no):
     assert lo.dtype == torch.float32 and np.array_equal(lo.cpu().numpy(), got.astype(np.float32))
 
 
+def test_numpy_seam_runs_meshgrids_as_contractions():
+    """The drop-in ``summate_fourier`` (numpy in, numpy out) sees the generator's modes
+    and a structured mesh as two 'ij' meshgrids and sums per axis: bitwise the tensor
+    path, within 1e-12 of the oracle; scattered points take the per-point sum."""
+    from gstools_amd import summate_fourier_tensor_device
+
+    _say("fourier numpy seam: meshgrid -> contraction")
+    gen = _gen(2, [20, 14], [50.0, 40.0], 4.0, 21)
+    axes = [np.linspace(0.0, 50.0, 81), np.linspace(-3.0, 40.0, 67)]  # 5427 points, 280 modes
+    pos = generate_grid(axes)
+    args = (gen._spectrum_factor, gen._modes, gen._z_1, gen._z_2)
+    got = summate_fourier(*args, pos)
+    ten = summate_fourier_tensor_device(gen._spectrum_factor, gen._kaxes, gen._z_1, gen._z_2, axes)
+    assert np.array_equal(got, ten.cpu().numpy())
+    assert oracle.rel_errors(got, oracle.summate_fourier_c(*args, pos))[0] <= 1e-12
+    scattered = np.random.RandomState(4).uniform(0, 50, size=(2, 300))
+    per_point = summate_fourier_device(*args, scattered, "fp64").cpu().numpy()
+    assert np.array_equal(summate_fourier(*args, scattered), per_point)
+
+
 def test_tensor_path_errors_and_empty():
     from gstools_amd import summate_fourier_tensor_device
 
