@@ -34,30 +34,35 @@ namespace gsx {
 
 typedef __attribute__((ext_vector_type(4))) double f64x4;
 
-constexpr int kFR = 128;     // C rows per block (two row waves of 64)
-constexpr int kFC = 128;     // C columns per block (two column waves of 64)
+constexpr int kFR = 128;     // C rows per block
 constexpr int kFK = 16;      // K per LDS stage
 constexpr int kFApad = kFK + 1;
-constexpr int kFBpad = kFC + 16;
 
 // Batched real FP64 GEMM, row-major: C_b = A_b B_b, A_b (M x K, lda), B_b (K x N,
 // ldb), C_b (M x N, ldc), operand b at A + b sA, B + b sB, C + b sC.  Block =
-// (batch, row tile, column tile); 4 waves in 2 x 2, each 64 x 64 = 4 x 4
-// MFMA tiles; A and B tiles staged through LDS with the next stage prefetched
-// into registers; out-of-range operand entries are zero.
+// (batch, row tile, column tile) of 128 x 64 WC: 4 waves in (4 / WC) x WC, each
+// (128 WC / 4) rows x 64 columns of 16 x 16 MFMA tiles (WC = 2: 4 x 4; WC = 1,
+// for launches with few tiles or N <= 64: 2 x 4); A and B tiles staged through
+// LDS with the next stage prefetched into registers; out-of-range operand
+// entries are zero.
 // v_mfma_f64_16x16x4 layouts (g = lane / 16): A: lane holds row lane % 16, K
 // element g; B: column lane % 16, K element g; C: column lane % 16, register r
 // holds row g + 4 r.
+template <int WC>
 __global__ __launch_bounds__(256, 2) void k_fourier_dgemm(int64_t M, int64_t N, int64_t K,
                                                           const double* __restrict__ A, int64_t lda, int64_t sA,
                                                           const double* __restrict__ B, int64_t ldb, int64_t sB,
                                                           double* __restrict__ C, int64_t ldc, int64_t sC,
                                                           int64_t row_tiles, int64_t col_tiles) {
+    constexpr int kBC = 64 * WC;            // block columns
+    constexpr int kBpad = kBC + 16;
+    constexpr int kRF = kFR * WC / 4 / 16;  // 16-row MFMA tiles per wave
+    constexpr int kBT = 256 / kBC;          // B rows staged per pass
     __shared__ double As[kFR * kFApad];
-    __shared__ double Bs[kFK * kFBpad];
+    __shared__ double Bs[kFK * kBpad];
     const int t = threadIdx.x;
     const int lane = t & 63, wave = t >> 6;
-    const int wr = wave >> 1, wc = wave & 1;
+    const int wr = wave / WC, wc = wave % WC;
     const int g = lane >> 4, l16 = lane & 15;
     const int64_t per_batch = row_tiles * col_tiles;
     const int64_t bt = (int64_t)blockIdx.x / per_batch;
@@ -66,13 +71,13 @@ __global__ __launch_bounds__(256, 2) void k_fourier_dgemm(int64_t M, int64_t N, 
     A += bt * sA;
     B += bt * sB;
     C += bt * sC;
-    const int64_t i0 = rt * kFR, p0 = ct * kFC;
+    const int64_t i0 = rt * kFR, p0 = ct * kBC;
 
-    // staging: A 128 x 16 (two threads per row, 8 consecutive each); B 16 x 128
-    // (point t % 128 of rows t / 128 + 2u: unit-stride loads and LDS stores)
+    // staging: A 128 x 16 (two threads per row, 8 consecutive each); B 16 x kBC
+    // (column t % kBC of rows t / kBC + kBT u: unit-stride loads and LDS stores)
     const int ar = t >> 1, ac = (t & 1) * 8;
-    const int br = t >> 7, bc = t & 127;
-    double pa[8], pb[8];
+    const int br = t / kBC, bc = t % kBC;
+    double pa[8], pb[kFK / kBT];
     auto load = [&](int64_t j0) {
         const int64_t row = i0 + ar;
 #pragma unroll
@@ -82,8 +87,8 @@ __global__ __launch_bounds__(256, 2) void k_fourier_dgemm(int64_t M, int64_t N, 
         }
         const int64_t p = p0 + bc;
 #pragma unroll
-        for (int u = 0; u < 8; ++u) {
-            const int64_t j = j0 + br + 2 * u;
+        for (int u = 0; u < kFK / kBT; ++u) {
+            const int64_t j = j0 + br + kBT * u;
             pb[u] = (j < K && p < N) ? B[j * ldb + p] : 0.0;
         }
     };
@@ -91,12 +96,12 @@ __global__ __launch_bounds__(256, 2) void k_fourier_dgemm(int64_t M, int64_t N, 
 #pragma unroll
         for (int u = 0; u < 8; ++u) As[ar * kFApad + ac + u] = pa[u];
 #pragma unroll
-        for (int u = 0; u < 8; ++u) Bs[(br + 2 * u) * kFBpad + bc] = pb[u];
+        for (int u = 0; u < kFK / kBT; ++u) Bs[(br + kBT * u) * kBpad + bc] = pb[u];
     };
 
-    f64x4 acc[4][4];
+    f64x4 acc[kRF][4];
 #pragma unroll
-    for (int r = 0; r < 4; ++r)
+    for (int r = 0; r < kRF; ++r)
 #pragma unroll
         for (int c = 0; c < 4; ++c) acc[r][c] = f64x4{0.0, 0.0, 0.0, 0.0};
 
@@ -108,13 +113,13 @@ __global__ __launch_bounds__(256, 2) void k_fourier_dgemm(int64_t M, int64_t N, 
         if (j0 + kFK < K) load(j0 + kFK);
 #pragma unroll
         for (int kk = 0; kk < kFK; kk += 4) {
-            double a[4], b[4];
+            double a[kRF], b[4];
 #pragma unroll
-            for (int r = 0; r < 4; ++r) a[r] = As[(wr * 64 + r * 16 + l16) * kFApad + kk + g];
+            for (int r = 0; r < kRF; ++r) a[r] = As[(wr * kRF * 16 + r * 16 + l16) * kFApad + kk + g];
 #pragma unroll
-            for (int c = 0; c < 4; ++c) b[c] = Bs[(kk + g) * kFBpad + wc * 64 + c * 16 + l16];
+            for (int c = 0; c < 4; ++c) b[c] = Bs[(kk + g) * kBpad + wc * 64 + c * 16 + l16];
 #pragma unroll
-            for (int r = 0; r < 4; ++r)
+            for (int r = 0; r < kRF; ++r)
 #pragma unroll
                 for (int c = 0; c < 4; ++c)
                     acc[r][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[r], b[c], acc[r][c], 0, 0, 0);
@@ -125,10 +130,10 @@ __global__ __launch_bounds__(256, 2) void k_fourier_dgemm(int64_t M, int64_t N, 
         const int64_t p = p0 + wc * 64 + c * 16 + l16;
         if (p >= N) continue;
 #pragma unroll
-        for (int r = 0; r < 4; ++r)
+        for (int r = 0; r < kRF; ++r)
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
-                const int64_t row = i0 + wr * 64 + r * 16 + g + 4 * q;
+                const int64_t row = i0 + wr * kRF * 16 + r * 16 + g + 4 * q;
                 if (row < M) C[row * ldc + p] = acc[r][c][q];
             }
     }
@@ -219,10 +224,19 @@ inline bool fourier_tensor_plan(int64_t d, const int64_t* mode_no, const int64_t
 inline int dgemm(int64_t M, int64_t N, int64_t K, int64_t batch, const double* A, int64_t lda, int64_t sA,
                  const double* B, int64_t ldb, int64_t sB, double* C, int64_t ldc, int64_t sC, hipStream_t s) {
     if (M == 0 || N == 0 || batch == 0) return GSX_OK;
-    const int64_t rt = ceil_div(M, (int64_t)kFR), ct = ceil_div(N, (int64_t)kFC);
+    const int64_t rt = ceil_div(M, (int64_t)kFR);
+    // 128-column tiles unless that leaves the chip (256 CUs) under-filled or
+    // half of each tile empty: then 64-column tiles
+    const int64_t ct2 = ceil_div(N, (int64_t)128), ct1 = ceil_div(N, (int64_t)64);
+    const bool narrow = N <= 64 || batch * rt * ct2 < 256;
+    const int64_t ct = narrow ? ct1 : ct2;
     if (batch * rt * ct > 0x7fffffffLL) return GSX_E_INVALID;
-    hipLaunchKernelGGL(k_fourier_dgemm, dim3((unsigned)(batch * rt * ct)), dim3(256), 0, s, M, N, K, A, lda, sA, B,
-                       ldb, sB, C, ldc, sC, rt, ct);
+    if (narrow)
+        hipLaunchKernelGGL(k_fourier_dgemm<1>, dim3((unsigned)(batch * rt * ct)), dim3(256), 0, s, M, N, K, A, lda,
+                           sA, B, ldb, sB, C, ldc, sC, rt, ct);
+    else
+        hipLaunchKernelGGL(k_fourier_dgemm<2>, dim3((unsigned)(batch * rt * ct)), dim3(256), 0, s, M, N, K, A, lda,
+                           sA, B, ldb, sB, C, ldc, sC, rt, ct);
     return launch_status();
 }
 
