@@ -136,6 +136,15 @@ def bench_mfma_grid(torch, steps, warmup):
     res["fp64_direct"] = {"value": 64 * 512 * 512 * 4096 / wall64, "measured_on": "64 of 512 x-planes"}
     res["speedup_bf16x3_vs_fp64"] = res["bf16x3"]["value"] / res["fp64_direct"]["value"]
     res["speedup_fp16x3_vs_fp64"] = res["fp16x3"]["value"] / res["fp64_direct"]["value"]
+    # FP64 GEMM form (precision "fp64gemm"): 32 column chunks of 8192 columns
+    out64 = torch.empty(n, dtype=torch.float64, device=dev)
+    fg = lambda: summate_grid_device(cov, z1, z2, axes, "fp64gemm", out=out64)
+    wallg, kerg = time_kernel(fg, max(2, steps // 4), 1, torch)
+    got = out64.reshape(512, 512, 512)[:, :, zsub].cpu().numpy() * scale
+    res["fp64gemm"] = {"value": work / wallg, "ms_per_step": wallg * 1e3,
+                       "rel_max_vs_fp64": oracle.rel_errors(got, ref)[0],
+                       "gemm_tflops": 4.0 * work / kerg / 1e12,
+                       "speedup_vs_fp64_direct": (work / wallg) / res["fp64_direct"]["value"]}
     return res
 
 

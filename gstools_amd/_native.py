@@ -74,6 +74,8 @@ EXPORTS = {
     "gsx_summate_fourier_f32": (_i32, _FOUR),
     "gsx_summate_fourier_grid_f64": (_i32, _FOUR_GRID),
     "gsx_summate_fourier_grid_f32": (_i32, _FOUR_GRID),
+    "gsx_summate_grid_gemm_workspace": (_i64, [_i64, _i64, _i64p]),
+    "gsx_summate_grid_gemm_f64": (_i32, [_i64, _i64, _i64p, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i64, _vp]),
     "gsx_summate_fourier_tensor_workspace": (_i64, [_i64, _i64p, _i64p]),
     "gsx_summate_fourier_tensor_f64": (_i32, [_i64, _i64p, _vp, _i64p, _vp, _vp, _vp, _vp, _vp, _vp, _i64, _vp]),
     "gsx_summate_fourier_grid_mfma": (_i32, [_i64, _i64, _i64p, _vp, _vp, _vp, _vp, _vp, _i32, _f64, _vp, _vp, _i64,

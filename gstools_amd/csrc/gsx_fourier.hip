@@ -310,3 +310,175 @@ int gsx_summate_fourier_tensor_f64(int64_t d, const int64_t* mode_no, const doub
 }
 
 }  // extern "C"
+
+// ------------------------------------------------------------------------
+// Randomisation-method sum on a structured grid as an FP64 GEMM.
+//
+// u[x][(y z)] = Re sum_j c_j e^{i k0_j x} e^{i k1_j y} e^{i k2_j z},
+// c_j = f_j (z1_j - i z2_j): rows are axis 0, columns the flattened (y z),
+// A[x][(part, j)] = [cos(k0_j x), -sin(k0_j x)] (n0 x 2N) and
+// B[(part, j)][col] = [Re; Im] of c_j e^{i k1_j y} e^{i k2_j z} (2N x n1 n2),
+// built in column chunks of bounded size and multiplied on the FP64 matrix
+// cores (k_fourier_dgemm).  The phase of each axis is the rounded product
+// k_i x_i, as in the per-point sum; the sum of those phases is not rounded,
+// so the result differs from the per-point sum by about |k . x| 2^-52
+// (relative) per term.
+namespace gsx {
+
+constexpr int64_t kGemmChunkBytes = int64_t(1) << 29;  // B chunk: at most 512 MB
+
+// A (n0 x 2N): row x = [cos(k0_j x) (N), -sin(k0_j x) (N)]
+__global__ void k_grid_gemm_a(int64_t n0, int64_t N, const double* __restrict__ x, const double* __restrict__ k0,
+                              double* __restrict__ a) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n0 * N) return;
+    const int64_t i = e / N, j = e - i * N;
+    double s, c;
+    phase_sincos(k0[j] * x[i], &s, &c);
+    a[i * 2 * N + j] = c;
+    a[i * 2 * N + N + j] = -s;
+}
+
+// E (n x N) complex, [i][j] = (cos, sin)(k_j x_i) (k null: 1); with coeff, times
+// f_j (z1_j - i z2_j)
+__global__ void k_grid_gemm_e(int64_t n, int64_t N, const double* __restrict__ x, const double* __restrict__ k,
+                              const double* __restrict__ fac, const double* __restrict__ z1,
+                              const double* __restrict__ z2, bool coeff, double2* __restrict__ e) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n * N) return;
+    const int64_t i = t / N, j = t - i * N;
+    double s = 0.0, c = 1.0;
+    if (k) phase_sincos(k[j] * x[i], &s, &c);
+    if (coeff) {
+        const double f = fac ? fac[j] : 1.0;
+        const double cr = f * z1[j], ci = -(f * z2[j]);
+        e[t] = double2{cr * c - ci * s, cr * s + ci * c};
+    } else {
+        e[t] = double2{c, s};
+    }
+}
+
+// B chunk (2N x C): columns col0 .. col0 + C - 1 of the flattened (i1, i2);
+// e1: c_j e^{i k1_j y} (n1 x N), e2: e^{i k2_j z} (n2 x N) or null (2D)
+__global__ void k_grid_gemm_b(int64_t N, int64_t n2, int64_t col0, int64_t C, int64_t cols,
+                              const double2* __restrict__ e1, const double2* __restrict__ e2,
+                              double* __restrict__ b) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= N * C) return;
+    const int64_t j = t / C, cc = t - j * C;
+    const int64_t col = col0 + cc;
+    double re = 0.0, im = 0.0;
+    if (col < cols) {
+        const int64_t i1 = e2 ? col / n2 : col, i2 = e2 ? col - i1 * n2 : 0;
+        const double2 u = e1[i1 * N + j];
+        if (e2) {
+            const double2 v = e2[i2 * N + j];
+            re = u.x * v.x - u.y * v.y;
+            im = u.x * v.y + u.y * v.x;
+        } else {
+            re = u.x;
+            im = u.y;
+        }
+    }
+    b[j * C + cc] = re;
+    b[(N + j) * C + cc] = im;
+}
+
+struct GridGemmPlan {
+    int64_t d, N, n0, n1, n2, cols, C;
+    int64_t off_a, off_e1, off_e2, off_b, bytes;  // doubles
+};
+
+inline bool grid_gemm_plan(int64_t d, int64_t N, const int64_t* shape, GridGemmPlan& p) {
+    if (d < 1 || d > 3 || N < 0 || !shape) return false;
+    for (int i = 0; i < d; ++i)
+        if (shape[i] < 0) return false;
+    p.d = d;
+    p.N = N;
+    p.n0 = shape[0];
+    p.n1 = d >= 2 ? shape[1] : 1;
+    p.n2 = d == 3 ? shape[2] : 1;
+    p.cols = p.n1 * p.n2;
+    const int64_t per_col = 2 * (N > 0 ? N : 1) * 8;
+    int64_t C = kGemmChunkBytes / per_col;
+    C = C >= 128 ? C / 128 * 128 : (C > 0 ? C : 1);
+    p.C = C < p.cols ? C : (p.cols > 0 ? p.cols : 1);
+    int64_t o = 0;
+    auto take = [&](int64_t doubles) {
+        const int64_t at = o;
+        o += (doubles + 31) / 32 * 32;
+        return at;
+    };
+    p.off_a = take(p.n0 * 2 * N);
+    p.off_e1 = take(2 * p.n1 * N);
+    p.off_e2 = d == 3 ? take(2 * p.n2 * N) : 0;
+    p.off_b = take(2 * N * p.C);
+    p.bytes = o * 8;
+    return true;
+}
+
+}  // namespace gsx
+
+extern "C" {
+
+int64_t gsx_summate_grid_gemm_workspace(int64_t d, int64_t N, const int64_t* shape) {
+    gsx::GridGemmPlan p;
+    if (!gsx::grid_gemm_plan(d, N, shape, p)) return GSX_E_INVALID;
+    return p.bytes;
+}
+
+int gsx_summate_grid_gemm_f64(int64_t d, int64_t N, const int64_t* shape, const double* axes, const double* cov,
+                              const double* z1, const double* z2, const double* spectrum_factor, double* out,
+                              void* workspace, int64_t workspace_bytes, void* stream) {
+    gsx::GridGemmPlan p;
+    if (!gsx::grid_gemm_plan(d, N, shape, p)) return GSX_E_INVALID;
+    const int64_t P = p.n0 * p.cols;
+    if (P == 0) return GSX_OK;
+    if (!axes || !out) return GSX_E_INVALID;
+    hipStream_t s = gsx::as_stream(stream);
+    auto blocks = [](int64_t n) { return dim3((unsigned)gsx::ceil_div(n, (int64_t)256)); };
+    if (N == 0) {
+        if (gsx::ceil_div(P, (int64_t)256) > 0x7fffffffLL) return GSX_E_INVALID;
+        hipLaunchKernelGGL(gsx::k_fourier_zero, blocks(P), dim3(256), 0, s, out, P);
+        return gsx::launch_status();
+    }
+    if (!cov || !z1 || !z2) return GSX_E_INVALID;
+    if (!workspace || workspace_bytes < p.bytes) return GSX_E_WORKSPACE;
+    const int64_t big = p.n0 * N > p.n1 * N ? p.n0 * N : p.n1 * N;
+    if (gsx::ceil_div(big, (int64_t)256) > 0x7fffffffLL || gsx::ceil_div(p.n2 * N, (int64_t)256) > 0x7fffffffLL ||
+        gsx::ceil_div(N * p.C, (int64_t)256) > 0x7fffffffLL)
+        return GSX_E_INVALID;
+    double* ws = static_cast<double*>(workspace);
+    double* a = ws + p.off_a;
+    double2* e1 = reinterpret_cast<double2*>(ws + p.off_e1);
+    double2* e2 = d == 3 ? reinterpret_cast<double2*>(ws + p.off_e2) : nullptr;
+    double* b = ws + p.off_b;
+    const double* ax0 = axes;
+    hipLaunchKernelGGL(gsx::k_grid_gemm_a, blocks(p.n0 * N), dim3(256), 0, s, p.n0, N, ax0, cov, a);
+    int st = gsx::launch_status();
+    if (st != GSX_OK) return st;
+    // e1: c_j e^{i k1_j y} (d >= 2), or c_j alone for the one column of a 1D grid
+    if (d >= 2)
+        hipLaunchKernelGGL(gsx::k_grid_gemm_e, blocks(p.n1 * N), dim3(256), 0, s, p.n1, N, axes + p.n0, cov + N,
+                           spectrum_factor, z1, z2, true, e1);
+    else
+        hipLaunchKernelGGL(gsx::k_grid_gemm_e, blocks(N), dim3(256), 0, s, (int64_t)1, N, nullptr, nullptr,
+                           spectrum_factor, z1, z2, true, e1);
+    if ((st = gsx::launch_status()) != GSX_OK) return st;
+    if (d == 3) {
+        hipLaunchKernelGGL(gsx::k_grid_gemm_e, blocks(p.n2 * N), dim3(256), 0, s, p.n2, N, axes + p.n0 + p.n1,
+                           cov + 2 * N, nullptr, nullptr, nullptr, false, e2);
+        if ((st = gsx::launch_status()) != GSX_OK) return st;
+    }
+    for (int64_t c0 = 0; c0 < p.cols; c0 += p.C) {
+        const int64_t cn = p.cols - c0 < p.C ? p.cols - c0 : p.C;
+        hipLaunchKernelGGL(gsx::k_grid_gemm_b, blocks(N * p.C), dim3(256), 0, s, N, p.n2, c0, p.C, p.cols, e1, e2,
+                           b);
+        if ((st = gsx::launch_status()) != GSX_OK) return st;
+        st = gsx::dgemm(p.n0, cn, 2 * N, 1, a, 2 * N, 0, b, p.C, 0, out + c0, p.cols, 0, s);
+        if (st != GSX_OK) return st;
+    }
+    return GSX_OK;
+}
+
+}  // extern "C"

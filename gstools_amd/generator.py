@@ -30,6 +30,7 @@ import torch
 from .rng import RNG
 from .summate import (
     GRID_PRECISIONS,
+    INCOMPR_GRID_PRECISIONS,
     PRECISIONS,
     summate_device,
     summate_fourier_tensor_device,
@@ -163,7 +164,8 @@ class RandMeth(Generator):
         """Field on the 'ij' meshgrid of ``axes`` without materialising it.
 
         ``precision`` may additionally be ``"bf16x3"`` or ``"fp16x3"`` (matrix
-        cores, d = 2 or 3).  The positions must already be isotropic (no
+        cores, d = 2 or 3) or ``"fp64gemm"`` (FP64 GEMM on the matrix cores; it
+        skips the rounding of the phase sum, about |k.x| 2^-52 per term).  The positions must already be isotropic (no
         rotation or anisotropy left); returns a torch tensor shaped like the
         grid.
         """
@@ -337,8 +339,8 @@ class IncomprRandMeth(RandMeth):
         mode makes every value NaN, as the reference's 0 / 0 does.
         """
         precision = precision or self.precision
-        if precision not in GRID_PRECISIONS:
-            raise ValueError(f"precision must be one of {GRID_PRECISIONS}")
+        if precision not in INCOMPR_GRID_PRECISIONS:
+            raise ValueError(f"precision must be one of {INCOMPR_GRID_PRECISIONS}")
         d = self.model.dim
         shape = tuple(int(np.size(a)) for a in axes)
         if len(shape) != d:

@@ -26,11 +26,13 @@ __all__ = [
     "summate", "summate_incompr", "summate_device", "summate_incompr_device",
     "summate_grid_device", "summate_adjoint_device", "summate_fourier", "summate_fourier_device",
     "summate_fourier_grid_device", "summate_fourier_tensor_device", "tensor_grid_axes", "PRECISIONS",
-    "GRID_PRECISIONS",
+    "GRID_PRECISIONS", "INCOMPR_GRID_PRECISIONS",
 ]
 
 PRECISIONS = ("fp64", "fp32")
-GRID_PRECISIONS = ("fp64", "fp32", "bf16x3", "fp16x3")
+GRID_PRECISIONS = ("fp64", "fp32", "bf16x3", "fp16x3", "fp64gemm")
+# incompressible grids: no FP64 GEMM form
+INCOMPR_GRID_PRECISIONS = ("fp64", "fp32", "bf16x3", "fp16x3")
 
 
 def _device():
@@ -195,12 +197,26 @@ def summate_grid_device(cov_samples, z_1, z_2, axes, precision="fp64", alpha=1.0
     shape_c = (ctypes.c_int64 * 3)(*(shape + [1] * (3 - d)))
     if precision not in GRID_PRECISIONS:
         raise ValueError(f"precision must be one of {GRID_PRECISIONS}")
-    dtype = torch.float64 if precision == "fp64" else torch.float32
+    dtype = torch.float64 if precision in ("fp64", "fp64gemm") else torch.float32
     if out is None:
         out = torch.empty(npts, dtype=dtype, device=dev)
     elif out.dtype != dtype or out.numel() != npts or not out.is_contiguous():
         raise ValueError("out: wrong dtype, size or layout")
     lib = _native.lib()
+    if precision == "fp64gemm":
+        # FP64 GEMM on the matrix cores: rows axis 0, columns the other axes (their
+        # complex phase products built in bounded column chunks); the phase sum is not
+        # rounded, so it differs from the per-point sum by about |k.x| 2^-52 per term
+        if alpha != 1.0:
+            raise ValueError("alpha applies to the split-precision matrix-core path only")
+        ws_bytes = int(lib.gsx_summate_grid_gemm_workspace(d, n, shape_c))
+        if ws_bytes < 0:
+            _native.check(ws_bytes, "summate_grid_gemm")
+        ws = torch.empty(max(ws_bytes, 1), dtype=torch.uint8, device=dev)
+        _native.check(lib.gsx_summate_grid_gemm_f64(d, n, shape_c, _ptr(flat), _ptr(cov), _ptr(z1), _ptr(z2),
+                                                    _ptr(fac), _ptr(out), _ptr(ws), ws_bytes, _stream()),
+                      "summate_grid_gemm")
+        return out
     if precision in ("fp64", "fp32"):
         if alpha != 1.0:
             raise ValueError("alpha applies to the matrix-core path only")
@@ -252,8 +268,8 @@ def summate_incompr_grid_device(cov_samples, z_1, z_2, axes, precision="fp64", a
     npts = int(np.prod(shape))
     flat = torch.cat(axes)
     shape_c = (ctypes.c_int64 * 3)(*(shape + [1] * (3 - d)))
-    if precision not in GRID_PRECISIONS:
-        raise ValueError(f"precision must be one of {GRID_PRECISIONS}")
+    if precision not in INCOMPR_GRID_PRECISIONS:
+        raise ValueError(f"precision must be one of {INCOMPR_GRID_PRECISIONS}")
     dtype = torch.float64 if precision == "fp64" else torch.float32
     if out is None:
         out = torch.empty((d, npts), dtype=dtype, device=dev)

@@ -10,7 +10,7 @@
  *   summate_incompr(cov_samples, z_1, z_2, pos)  gsx_summate_incompr_f64 / _f32
  *     src/gstools/field/generator.py:24, :51-64, call :554-560
  *   summate on a meshgrid from Field.pre_pos     gsx_summate_grid_f64 / _f32,
- *     src/gstools/field/base.py:254-297 ->         gsx_summate_grid_mfma
+ *     src/gstools/field/base.py:254-297 ->         gsx_summate_grid_mfma, gsx_summate_grid_gemm_f64
  *     tools/geometric.py:340-356
  *   summate_incompr on a meshgrid                gsx_summate_incompr_grid_f64 / _f32,
  *     src/gstools/field/generator.py:529-567      gsx_summate_incompr_grid_mfma
@@ -194,6 +194,23 @@ int gsx_summate_fourier_grid_mfma(int64_t d, int64_t N, const int64_t* shape,
                                   const double* z2, int variant, double alpha,
                                   float* out, void* workspace,
                                   int64_t workspace_bytes, void* stream);
+
+/* Structured grid as an FP64 GEMM on the matrix cores (an opt-in FP64-class
+ * form of gsx_summate_grid_f64 for the same meshgrid sum, generate_grid,
+ * tools/geometric.py:340-356): rows are axis 0, columns the other axes, the
+ * per-column complex phase products built in column chunks of at most 512
+ * MB.  Each axis's phase is the rounded product k_i x_i; their sum is not
+ * rounded, so the result differs from the per-point sum by about
+ * |k . x| 2^-52 (relative) per term.  spectrum_factor may be null (the
+ * randomisation method) or the Fourier method's factors.  out float64.
+ * d in 1..3; workspace bytes from gsx_summate_grid_gemm_workspace(). */
+int64_t gsx_summate_grid_gemm_workspace(int64_t d, int64_t N, const int64_t* shape);
+int gsx_summate_grid_gemm_f64(int64_t d, int64_t N, const int64_t* shape,
+                              const double* axes, const double* cov,
+                              const double* z1, const double* z2,
+                              const double* spectrum_factor, double* out,
+                              void* workspace, int64_t workspace_bytes,
+                              void* stream);
 
 /* Fourier method on a structured grid when the modes are a tensor grid (the
  * Fourier generator's own: the 'ij' meshgrid of one wavenumber axis per

@@ -148,6 +148,48 @@ def test_mfma_uniform_flag_checked_and_ignored():
     assert np.array_equal(a, _grid_mfma_raw(cov3, z13, z23, ax3, _native.MFMA_FP16X3))
 
 
+@pytest.mark.parametrize("shape,n", [
+    ((300,), 500),            # 1D: one column (the coefficients)
+    ((64, 300), 333),         # 2D
+    ((37, 40, 23), 300),      # 3D, ragged tiles
+    ((8, 100, 100), 4096),    # 10000 columns of 2 x 4096 doubles: two 512 MB column chunks
+])
+def test_fp64gemm_grid_matches_oracle(shape, n):
+    """precision "fp64gemm": the grid sum as an FP64 GEMM over column chunks of the phase
+    products; it skips only the rounding of the phase sum, so with phases of at most a
+    few hundred radians it stays within 1e-12 of the oracle."""
+    _say(f"fp64gemm shape={shape} N={n}")
+    d = len(shape)
+    cov, z1, z2 = _modes(d, n, 5 + n, kscale=0.6)
+    axes = [np.linspace(-3.0 * (i + 1), 11.0 + 2 * i, s) for i, s in enumerate(shape)]
+    ref = oracle.summate_c(cov, z1, z2, generate_grid(axes))
+    got = summate_grid_device(cov, z1, z2, axes, "fp64gemm")
+    assert got.dtype == torch.float64
+    rel_max = oracle.rel_errors(got.cpu().numpy(), ref)[0]
+    _say(f"  rel_max={rel_max:.3e}")
+    assert rel_max <= 1e-12
+
+
+def test_fp64gemm_generators():
+    """RandMeth.structured and the Fourier method's direct route take "fp64gemm"; the
+    incompressible grid entry refuses it."""
+    from gstools_amd import Fourier, IncomprRandMeth, summate_fourier_grid_device, summate_incompr_grid_device
+
+    _say("fp64gemm generators")
+    rm = RandMeth(Gaussian(dim=2, var=1.5, len_scale=6.0), mode_no=700, seed=9)
+    axes = [np.linspace(0, 40, 50), np.linspace(0, 30, 61)]
+    a = rm.structured(axes, precision="fp64gemm", add_nugget=False).cpu().numpy()
+    b = rm.structured(axes, precision="fp64", add_nugget=False).cpu().numpy()
+    assert a.shape == b.shape and oracle.rel_errors(a.ravel(), b.ravel())[0] <= 1e-12
+    gen = Fourier(Gaussian(dim=2, var=1.0, len_scale=5.0), period=[40.0, 30.0], mode_no=[16, 12], seed=4)
+    args = (gen._spectrum_factor, gen._modes, gen._z_1, gen._z_2)
+    f = summate_fourier_grid_device(*args, axes, "fp64gemm").cpu().numpy()
+    ref = oracle.summate_fourier_c(*args, generate_grid(axes))
+    assert oracle.rel_errors(f, ref)[0] <= 1e-12
+    with pytest.raises(ValueError):
+        summate_incompr_grid_device(rm._cov_sample, rm._z_1, rm._z_2, axes, "fp64gemm")
+
+
 def test_mfma_matern_config_sample():
     """Matern(nu=1.5, len 8), N 4096 (MCMC modes): 4 z-planes of the 512^3 grid."""
     _say("mfma matern sample")
