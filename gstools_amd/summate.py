@@ -177,10 +177,12 @@ def summate_grid_device(cov_samples, z_1, z_2, axes, precision="fp64", alpha=1.0
     """Sum on the 'ij' meshgrid of ``axes`` without materialising it.
 
     ``precision``: "fp64" / "fp32" (direct sums; "fp64" reproduces the phases
-    of the materialised grid bit for bit) or "bf16x3" / "fp16x3" (matrix-core
-    GEMM over per-axis phase factors, d = 2 or 3).  Returns a flat tensor of
-    prod(shape) values in C order, multiplied by ``alpha`` (matrix-core path
-    only; the direct paths return the unscaled sum and require alpha == 1).
+    of the materialised grid bit for bit), "bf16x3" / "fp16x3" (matrix-core
+    GEMM over per-axis phase factors, d = 2 or 3) or "fp64gemm" (FP64 GEMM on
+    the matrix cores, float64 out; the phase sum is not rounded, about
+    |k.x| 2^-52 per term).  Returns a flat tensor of prod(shape) values in C
+    order, multiplied by ``alpha`` (split-precision matrix-core path only; the
+    other paths return the unscaled sum and require alpha == 1).
     ``spectrum_factor`` (N,) weights each term (the Fourier method's sum).
     """
     dev = _device()
