@@ -187,3 +187,18 @@ def test_fourier_tensor_workspace():
     assert q(1, c3(10), c3(37)) == 8 * (al(20) + al(2 * 10 * 37))
     assert q(4, c3(1, 1, 1), c3(1, 1, 1)) < 0 and q(2, c3(-1, 2), c3(3, 3)) < 0
     assert q(2, c3(0, 3), c3(5, 5)) >= 0 and q(3, c3(4, 0, 2), c3(5, 0, 5)) >= 0  # no modes / points
+
+
+def test_grid_gemm_workspace():
+    """Workspace of the FP64 GEMM grid form: the A table, the phase tables and one
+    column chunk of B (at most 512 MB, a multiple of 128 columns), 256-byte aligned."""
+    q = _native.lib().gsx_summate_grid_gemm_workspace
+    c3 = lambda *s: (ctypes.c_int64 * 3)(*(list(s) + [1] * (3 - len(s))))
+    al = lambda n: -(-n // 32) * 32
+    # 2D (64, 300), N 333: 300 columns fit one chunk
+    assert q(2, 333, c3(64, 300)) == 8 * (al(64 * 2 * 333) + al(2 * 300 * 333) + al(2 * 333 * 300))
+    # 3D (8, 100, 100), N 4096: chunks of 2^29 / (2 * 4096 * 8) = 8192 columns
+    want = al(8 * 2 * 4096) + al(2 * 100 * 4096) + al(2 * 100 * 4096) + al(2 * 4096 * 8192)
+    assert q(3, 4096, c3(8, 100, 100)) == 8 * want
+    assert q(1, 10, c3(7)) == 8 * (al(7 * 20) + al(20) + al(20))
+    assert q(0, 10, c3(7)) < 0 and q(2, -1, c3(3, 3)) < 0
