@@ -339,14 +339,14 @@ __global__ void k_grid_gemm_a(int64_t n0, int64_t N, const double* __restrict__ 
     a[i * 2 * N + N + j] = -s;
 }
 
-// E (n x N) complex, [i][j] = (cos, sin)(k_j x_i) (k null: 1); with coeff, times
-// f_j (z1_j - i z2_j)
+// E (N x n) complex, mode-major: [j][i] = (cos, sin)(k_j x_i) (k null: 1); with
+// coeff, times f_j (z1_j - i z2_j)
 __global__ void k_grid_gemm_e(int64_t n, int64_t N, const double* __restrict__ x, const double* __restrict__ k,
                               const double* __restrict__ fac, const double* __restrict__ z1,
                               const double* __restrict__ z2, bool coeff, double2* __restrict__ e) {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n * N) return;
-    const int64_t i = t / N, j = t - i * N;
+    const int64_t j = t / n, i = t - j * n;
     double s = 0.0, c = 1.0;
     if (k) phase_sincos(k[j] * x[i], &s, &c);
     if (coeff) {
@@ -359,7 +359,8 @@ __global__ void k_grid_gemm_e(int64_t n, int64_t N, const double* __restrict__ x
 }
 
 // B chunk (2N x C): columns col0 .. col0 + C - 1 of the flattened (i1, i2);
-// e1: c_j e^{i k1_j y} (n1 x N), e2: e^{i k2_j z} (n2 x N) or null (2D)
+// e1: c_j e^{i k1_j y} (N x n1), e2: e^{i k2_j z} (N x n2) or null (2D), mode-major
+// so that neighbouring columns read neighbouring entries
 __global__ void k_grid_gemm_b(int64_t N, int64_t n2, int64_t col0, int64_t C, int64_t cols,
                               const double2* __restrict__ e1, const double2* __restrict__ e2,
                               double* __restrict__ b) {
@@ -370,9 +371,9 @@ __global__ void k_grid_gemm_b(int64_t N, int64_t n2, int64_t col0, int64_t C, in
     double re = 0.0, im = 0.0;
     if (col < cols) {
         const int64_t i1 = e2 ? col / n2 : col, i2 = e2 ? col - i1 * n2 : 0;
-        const double2 u = e1[i1 * N + j];
+        const double2 u = e1[j * (cols / n2) + i1];
         if (e2) {
-            const double2 v = e2[i2 * N + j];
+            const double2 v = e2[j * n2 + i2];
             re = u.x * v.x - u.y * v.y;
             im = u.x * v.y + u.y * v.x;
         } else {
