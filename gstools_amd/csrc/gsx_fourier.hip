@@ -1,3 +1,7 @@
+// FP64 GEMMs on the matrix cores for structured grids: the Fourier method with
+// a tensor grid of modes (one contraction per axis, below) and, further down,
+// the randomisation-method grid sum as one FP64 GEMM (precision "fp64gemm").
+//
 // Fourier method on structured grids with a tensor grid of modes.
 //
 // The Fourier generator's modes are the 'ij' meshgrid of one wavenumber axis
