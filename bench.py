@@ -718,7 +718,7 @@ def main():
         },
         "roofline": {
             "bound": "valu",
-            "kernel": "k_forward_f32<3,4,false,UnstructuredPoints<3>>",
+            "kernel": "k_forward_f32<3,8,false,UnstructuredPoints<3>>",
             "achieved": achieved,
             "peak": FP32_VALU_PEAK_TFLOPS,
             "unit": "TFLOP/s",
