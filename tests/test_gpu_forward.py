@@ -82,6 +82,28 @@ def test_fp32_within_1e5(d):
     assert rel_max <= FP32_TOL and rel_rms <= FP32_TOL, (rel_max, rel_rms)
 
 
+@pytest.mark.parametrize("incompr,p_big", [(False, 4_500_000), (True, 2_500_000)])
+def test_fp32_full_chip_launch_matches_prefix(incompr, p_big):
+    """A launch of millions of points (thousands of blocks, every compute unit
+    busy) and a small launch of its first 30001 points agree with each other and
+    with the oracle.  N = 1000 leaves a ragged last chunk of 232 modes; 30001 is
+    ragged against the 2048- (or 1024-) point block."""
+    _say(f"fp32 full-chip launch vs prefix incompr={incompr} P={p_big}")
+    cov, z1, z2, pos = _case(3, 1000, p_big, 77, 2.0, 500.0)
+    fn = summate_incompr_device if incompr else summate_device
+    ref_fn = oracle.summate_incompr_c if incompr else oracle.summate_c
+    whole = fn(cov, z1, z2, pos, precision="fp32").cpu().numpy()
+    k = 30001
+    part = fn(cov, z1, z2, np.ascontiguousarray(pos[:, :k]), precision="fp32").cpu().numpy()
+    whole_k = whole[..., :k]
+    ref = ref_fn(cov, z1, z2, np.ascontiguousarray(pos[:, :k]))
+    for got in (whole_k, part):
+        rel_max, rel_rms = oracle.rel_errors(got, ref)
+        assert rel_max <= FP32_TOL and rel_rms <= FP32_TOL, (rel_max, rel_rms)
+    # a point's sum does not depend on the launch it belongs to
+    assert np.array_equal(part, whole_k)
+
+
 @pytest.mark.parametrize("d", [2, 3])
 def test_incompr_matches_oracle(d):
     _say(f"incompr d={d}")
