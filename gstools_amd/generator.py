@@ -32,6 +32,7 @@ from .summate import (
     GRID_PRECISIONS,
     INCOMPR_GRID_PRECISIONS,
     PRECISIONS,
+    host_array,
     summate_device,
     summate_fourier_tensor_device,
     summate_grid_device,
@@ -154,7 +155,7 @@ class RandMeth(Generator):
         summed = summate_device(cov, z1, z2, pos, precision=self.precision)
         field = summed.to(torch.float64) * float(self._scale())
         if not as_tensor:
-            field = field.cpu().numpy()
+            field = host_array(field)
         nugget = self.get_nugget(tuple(field.shape)) if add_nugget else 0.0
         if as_tensor and not np.isscalar(nugget):
             nugget = torch.as_tensor(nugget, device=field.device)
@@ -326,7 +327,7 @@ class IncomprRandMeth(RandMeth):
             if not np.isscalar(nugget):
                 nugget = torch.as_tensor(nugget, device=summed.device)
             return out + nugget
-        return self.mean_u * e1 + scale * summed.cpu().numpy() + nugget
+        return self.mean_u * e1 + scale * host_array(summed) + nugget
 
     def structured(self, axes, precision=None, add_nugget=True):
         """Vector field (d, *shape) on the 'ij' meshgrid of ``axes`` without materialising it.
@@ -423,7 +424,7 @@ class Fourier(Generator):
         field = summate_device(modes, z1, z2, pos, precision=self.precision, spectrum_factor=fac)
         field = field.to(torch.float64)
         if not as_tensor:
-            field = field.cpu().numpy()
+            field = host_array(field)
         nugget = self.get_nugget(tuple(field.shape)) if add_nugget else 0.0
         if as_tensor and not np.isscalar(nugget):
             nugget = torch.as_tensor(nugget, device=field.device)

@@ -32,7 +32,7 @@ from scipy.spatial.distance import cdist
 
 from . import _native
 from .geometry import generate_grid
-from .summate import _as_f64, _device, _ptr, _stream
+from .summate import _as_f64, _device, _ptr, _stream, host_array
 
 __all__ = [
     "calc_field_krige", "calc_field_krige_and_variance", "krige_sum_device", "krige_cov_device",
@@ -75,14 +75,14 @@ def _check_numpy(**arrays):
 def calc_field_krige(krig_mat, krig_vecs, cond, num_threads=None):
     """Drop-in for the backend ``calc_field_krige`` -> field (P,) float64."""
     _check_numpy(krig_mat=krig_mat, krig_vecs=krig_vecs, cond=cond)
-    return krige_sum_device(krig_mat, krig_vecs, cond, variance=False).cpu().numpy()
+    return host_array(krige_sum_device(krig_mat, krig_vecs, cond, variance=False))
 
 
 def calc_field_krige_and_variance(krig_mat, krig_vecs, cond, num_threads=None):
     """Drop-in for the backend ``calc_field_krige_and_variance`` -> (field, error)."""
     _check_numpy(krig_mat=krig_mat, krig_vecs=krig_vecs, cond=cond)
     field, error = krige_sum_device(krig_mat, krig_vecs, cond, variance=True)
-    return field.cpu().numpy(), error.cpu().numpy()
+    return host_array(field), host_array(error)
 
 
 def krige_cov_device(model, cond_pos, pos, exact=False, out=None, row0=0):
@@ -285,10 +285,10 @@ class Krige:
     def _summate(self, field, krige_var, c_slice, k_vec, return_var):
         if return_var:
             f, e = krige_sum_device(self._krige_mat, k_vec, self._krige_cond, variance=True)
-            field[c_slice] = f.cpu().numpy()
-            krige_var[c_slice] = e.cpu().numpy()
+            field[c_slice] = host_array(f)
+            krige_var[c_slice] = host_array(e)
         else:
-            field[c_slice] = krige_sum_device(self._krige_mat, k_vec, self._krige_cond, variance=False).cpu().numpy()
+            field[c_slice] = host_array(krige_sum_device(self._krige_mat, k_vec, self._krige_cond, variance=False))
 
     # ------------------------------------------------------- system ----
     def _inv(self, mat):

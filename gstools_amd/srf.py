@@ -16,6 +16,7 @@ import numpy as np
 import torch
 
 from .generator import Fourier, IncomprRandMeth, RandMeth
+from .summate import host_array
 from .geometry import generate_grid
 
 __all__ = ["SRF", "GENERATOR"]
@@ -51,7 +52,7 @@ class SRF:
             shape = tuple(a.size for a in axes)
             iso_free = np.allclose(self.model.anis, 1.0) and np.allclose(self.model.angles, 0.0)
             if iso_free and hasattr(self.generator, "structured"):
-                field = self.generator.structured(axes, precision=precision).cpu().numpy()
+                field = host_array(self.generator.structured(axes, precision=precision))
                 return self._post(np.asarray(field, dtype=np.double))
             pos = generate_grid(axes)
         else:

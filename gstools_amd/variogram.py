@@ -23,7 +23,7 @@ import torch
 from . import _native
 from .geometry import RADIAN_SCALE, chordal_to_great_circle, generate_grid, latlon2pos
 from .krige import eval_func
-from .summate import _as_f64, _device, _ptr, _stream
+from .summate import _as_f64, _device, _ptr, _stream, host_array
 
 __all__ = [
     "unstructured", "directional", "structured", "ma_structured",
@@ -110,14 +110,14 @@ def unstructured(f, bin_edges, pos, estimator_type="m", distance_type="e", num_t
     if distance_type not in ("e", "h"):
         raise ValueError(f"Unknown distance type: {distance_type}")
     gamma, counts = _pairs(f, bin_edges, pos, estimator_type, haversine=distance_type == "h")
-    return gamma.cpu().numpy(), counts.cpu().numpy()
+    return host_array(gamma), host_array(counts)
 
 
 def directional(f, bin_edges, pos, direction, angles_tol=np.pi / 8.0, bandwidth=-1.0, separate_dirs=False,
                 estimator_type="m", num_threads=None):
     """Drop-in for the backend ``directional`` -> (gamma (D, B), counts (D, B))."""
     gamma, counts = _pairs(f, bin_edges, pos, estimator_type, direction, angles_tol, bandwidth, separate_dirs)
-    return gamma.cpu().numpy(), counts.cpu().numpy()
+    return host_array(gamma), host_array(counts)
 
 
 def _lags(f, mask, estimator_type):
@@ -142,12 +142,12 @@ def _lags(f, mask, estimator_type):
 
 def structured(f, estimator_type="m", num_threads=None):
     """Drop-in for the backend ``structured``: lags along the first axis -> gamma (n0,)."""
-    return _lags(f, None, estimator_type)[0].cpu().numpy()
+    return host_array(_lags(f, None, estimator_type)[0])
 
 
 def ma_structured(f, mask, estimator_type="m", num_threads=None):
     """Drop-in for the backend ``ma_structured`` (masked values skipped) -> gamma (n0,)."""
-    return _lags(f, mask, estimator_type)[0].cpu().numpy()
+    return host_array(_lags(f, mask, estimator_type)[0])
 
 
 # ------------------------------------------------------------ helpers -----
