@@ -19,8 +19,7 @@ Extra sections (``--extras``, on by default): structured 512^3 Matern grid
 on the matrix cores, the periodic Fourier method on a 4096^2 grid, ordinary
 kriging sums (1000 conditioning points onto 1e6 targets), variogram pair
 binning (1e6 points), forward + adjoint
-on the 2D Stable configuration and the 256-realization batch; each prints its own JSON object on stderr and
-under ``gpurun_out/`` when that folder exists.
+on the 2D Stable configuration and the 256-realization batch; each prints its own JSON object on stderr.
 """
 
 import argparse
