@@ -18,6 +18,8 @@ GSX_E_LAUNCH = -2
 GSX_E_NO_DEVICE = -3
 GSX_E_WORKSPACE = -4
 GSX_E_UNSUPPORTED = -5
+GSX_E_SCRATCH = -6
+NOALLOC_SCRATCH_BYTES = 32 << 20  # GSX_NOALLOC_SCRATCH_BYTES
 MFMA_BF16X3 = 0
 MFMA_FP16X3 = 1
 MFMA_UNIFORM_LAST = 16  # flag: the last axis of a 2D grid is uniformly spaced
@@ -83,6 +85,7 @@ EXPORTS = {
     "gsx_summate_adjoint_workspace": (_i64, [_i64, _i64, _i64]),
     "gsx_summate_adjoint_f64": (_i32, _ADJ),
     "gsx_summate_adjoint_f32": (_i32, _ADJ),
+    "gsx_summate_adjoint_f32_noalloc": (_i32, [_i64, _i64, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "gsx_krige_workspace": (_i64, [_i64, _i64, _i32]),
     "gsx_krige_cov_f64": (_i32, [_i32, _f64, _f64, _f64, _f64, _i64, _i64, _i64, _vp, _vp, _vp, _i64, _vp]),
     "gsx_krige_f64": (_i32, [_i64, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _i64, _vp]),

@@ -20,6 +20,8 @@
 // library fallback for waves whose phases may exceed 2^46.  FP32: phase in
 // revolutions in FP64, v_fract_f64, hardware v_sin_f32 / v_cos_f32, FP32
 // partial sums per LDS chunk folded into FP64 accumulators.
+#include <mutex>
+
 #include "gsx_internal.h"
 
 namespace gsx {
@@ -31,12 +33,15 @@ struct AdjPlan {
     int64_t mode_tiles, chunks, chunk_pts;
 };
 
-inline AdjPlan adj_plan(int64_t N, int64_t P) {
+// Point chunks of the stage-1 grid.  max_chunks_cap (> 0) bounds the number
+// of chunks, and with it the partial sums (chunks x (2 + 2D) x N doubles).
+inline AdjPlan adj_plan(int64_t N, int64_t P, int64_t max_chunks_cap = 0) {
     AdjPlan pl;
     pl.mode_tiles = ceil_div(N, kThreads);
     const int64_t max_chunks = ceil_div(P, kAdjPts);
     int64_t chunks = ceil_div(kAdjTargetBlocks, pl.mode_tiles);
     if (chunks > max_chunks) chunks = max_chunks;
+    if (max_chunks_cap > 0 && chunks > max_chunks_cap) chunks = max_chunks_cap;
     if (chunks < 1) chunks = 1;
     pl.chunk_pts = ceil_div(ceil_div(P, chunks), kAdjPts) * kAdjPts;
     pl.chunks = ceil_div(P, pl.chunk_pts);
@@ -192,10 +197,9 @@ __global__ __launch_bounds__(kThreads) void k_adjoint_stage2(int64_t N, int64_t 
 }
 
 template <int D, typename GT>
-int adjoint_launch(int64_t N, int64_t P, const double* cov, const double* z1, const double* z2,
-                   const double* pos, const GT* g, double* gz1, double* gz2, double* gk, void* ws,
-                   hipStream_t s) {
-    const AdjPlan pl = adj_plan(N, P);
+int adjoint_launch(const AdjPlan& pl, int64_t N, int64_t P, const double* cov, const double* z1,
+                   const double* z2, const double* pos, const GT* g, double* gz1, double* gz2,
+                   double* gk, void* ws, hipStream_t s) {
     double* part = static_cast<double*>(ws);
     hipLaunchKernelGGL((k_adjoint_stage1<D, GT>), dim3((unsigned)pl.mode_tiles, (unsigned)pl.chunks),
                        dim3(kThreads), 0, s, N, P, pl.chunk_pts, cov, pos, g, part);
@@ -212,14 +216,28 @@ __global__ void k_zero(double* __restrict__ p, int64_t n) {
     if (i < n) p[i] = 0.0;
 }
 
+// Gradients when there are points; ws holds pl.chunks x (2 + 2d) x N doubles.
 template <typename GT>
-int adjoint_entry(int64_t d, int64_t N, int64_t P, const double* cov, const double* z1,
-                  const double* z2, const double* pos, const GT* g, double* gz1, double* gz2,
-                  double* gk, void* ws, int64_t ws_bytes, void* stream) {
+int adjoint_run(const AdjPlan& pl, int64_t d, int64_t N, int64_t P, const double* cov,
+                const double* z1, const double* z2, const double* pos, const GT* g, double* gz1,
+                double* gz2, double* gk, void* ws, hipStream_t s) {
+    switch (d) {
+        case 1: return adjoint_launch<1>(pl, N, P, cov, z1, z2, pos, g, gz1, gz2, gk, ws, s);
+        case 2: return adjoint_launch<2>(pl, N, P, cov, z1, z2, pos, g, gz1, gz2, gk, ws, s);
+        default: return adjoint_launch<3>(pl, N, P, cov, z1, z2, pos, g, gz1, gz2, gk, ws, s);
+    }
+}
+
+// Argument checks shared by both entry points; *done is set when nothing
+// is left to do (no modes, or no points: the gradients are zero-filled).
+template <typename GT>
+int adjoint_prologue(int64_t d, int64_t N, int64_t P, const double* cov, const double* z1,
+                     const double* z2, const double* pos, const GT* g, double* gz1, double* gz2,
+                     double* gk, hipStream_t s, bool* done) {
+    *done = true;
     if (d < 1 || d > 3 || N < 0 || P < 0) return GSX_E_INVALID;
     if (N == 0) return GSX_OK;
     if (!cov || !z1 || !z2 || !gz1 || !gz2 || !gk) return GSX_E_INVALID;
-    hipStream_t s = as_stream(stream);
     if (P == 0) {
         const unsigned b = (unsigned)ceil_div(N, kThreads);
         hipLaunchKernelGGL(k_zero, dim3(b), dim3(kThreads), 0, s, gz1, N);
@@ -228,13 +246,76 @@ int adjoint_entry(int64_t d, int64_t N, int64_t P, const double* cov, const doub
         return launch_status();
     }
     if (!pos || !g) return GSX_E_INVALID;
+    *done = false;
+    return GSX_OK;
+}
+
+template <typename GT>
+int adjoint_entry(int64_t d, int64_t N, int64_t P, const double* cov, const double* z1,
+                  const double* z2, const double* pos, const GT* g, double* gz1, double* gz2,
+                  double* gk, void* ws, int64_t ws_bytes, void* stream) {
+    hipStream_t s = as_stream(stream);
+    bool done;
+    const int st = adjoint_prologue<GT>(d, N, P, cov, z1, z2, pos, g, gz1, gz2, gk, s, &done);
+    if (done) return st;
     const int64_t need = gsx_summate_adjoint_workspace(d, N, P);
     if (!ws || ws_bytes < need) return GSX_E_WORKSPACE;
-    switch (d) {
-        case 1: return adjoint_launch<1>(N, P, cov, z1, z2, pos, g, gz1, gz2, gk, ws, s);
-        case 2: return adjoint_launch<2>(N, P, cov, z1, z2, pos, g, gz1, gz2, gk, ws, s);
-        default: return adjoint_launch<3>(N, P, cov, z1, z2, pos, g, gz1, gz2, gk, ws, s);
+    return adjoint_run<GT>(adj_plan(N, P), d, N, P, cov, z1, z2, pos, g, gz1, gz2, gk, ws, s);
+}
+
+// ---------------------------------------------------------------------------
+// Allocation-free variant: the partial sums live in a fixed device array.
+//
+// `scratch` is one array per GPU (each device that loads this code object
+// holds its own copy until the process ends).  Its host-side name is only a
+// stand-in for the symbol: the device address comes from
+// hipGetSymbolAddress, once per device with that device current, and only
+// that address is handed to kernels.  Stage 1 writes every partial sum that
+// stage 2 reads, so nothing needs clearing between calls; two calls that
+// overlap on one GPU share the array and race.
+constexpr int64_t kScratchBytes = GSX_NOALLOC_SCRATCH_BYTES;
+static __device__ __align__(256) double scratch[kScratchBytes / sizeof(double)];
+
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mutex;
+static void* g_scratch_ptr[kMaxDevices] = {};
+
+// Device address of this device's scratch array (resolved on first use).
+static int scratch_address(void** out) {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return GSX_E_NO_DEVICE;
+    if (dev < 0 || dev >= kMaxDevices) return GSX_E_NO_DEVICE;
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (!g_scratch_ptr[dev]) {
+        void* ptr = nullptr;
+        if (hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)) != hipSuccess || !ptr) return GSX_E_LAUNCH;
+        g_scratch_ptr[dev] = ptr;
     }
+    *out = g_scratch_ptr[dev];
+    return GSX_OK;
+}
+
+template <typename GT>
+int adjoint_noalloc_entry(int64_t d, int64_t N, int64_t P, const double* cov, const double* z1,
+                          const double* z2, const double* pos, const GT* g, double* gz1,
+                          double* gz2, double* gk, void* stream) {
+    if (d >= 1 && d <= 3 && N > 0 && P > 0) {
+        // one chunk of partial sums per mode must fit the array
+        const int64_t per_chunk = (2 + 2 * d) * N * (int64_t)sizeof(double);
+        if (per_chunk > kScratchBytes) return GSX_E_SCRATCH;
+    }
+    hipStream_t s = as_stream(stream);
+    bool done;
+    const int st = adjoint_prologue<GT>(d, N, P, cov, z1, z2, pos, g, gz1, gz2, gk, s, &done);
+    if (done) return st;
+    void* ws = nullptr;
+    const int sa = scratch_address(&ws);
+    if (sa != GSX_OK) return sa;
+    // the workspace entry point's plan whenever its partial sums fit, else
+    // as many point chunks as fit (the sums then differ in rounding only)
+    const int64_t per_chunk = (2 + 2 * d) * N * (int64_t)sizeof(double);
+    const AdjPlan pl = adj_plan(N, P, kScratchBytes / per_chunk);
+    return adjoint_run<GT>(pl, d, N, P, cov, z1, z2, pos, g, gz1, gz2, gk, ws, s);
 }
 
 }  // namespace gsx
@@ -262,6 +343,13 @@ int gsx_summate_adjoint_f32(int64_t d, int64_t N, int64_t P, const double* cov, 
                             void* stream) {
     return gsx::adjoint_entry<float>(d, N, P, cov, z1, z2, pos, g, gz1, gz2, gk, workspace,
                                      workspace_bytes, stream);
+}
+
+int gsx_summate_adjoint_f32_noalloc(int64_t d, int64_t N, int64_t P, const double* cov,
+                                    const double* z1, const double* z2, const double* pos,
+                                    const float* g, double* gz1, double* gz2, double* gk,
+                                    void* stream) {
+    return gsx::adjoint_noalloc_entry<float>(d, N, P, cov, z1, z2, pos, g, gz1, gz2, gk, stream);
 }
 
 }  // extern "C"

@@ -11,6 +11,7 @@ const char* gsx_status_string(int status) {
         case GSX_E_NO_DEVICE: return "no usable HIP device";
         case GSX_E_WORKSPACE: return "workspace missing or too small";
         case GSX_E_UNSUPPORTED: return "request not supported by this build";
+        case GSX_E_SCRATCH: return "request exceeds the library's static scratch array";
         default: return "unknown status";
     }
 }

@@ -70,6 +70,8 @@ extern "C" {
 #define GSX_E_NO_DEVICE (-3) /* no usable HIP device                       */
 #define GSX_E_WORKSPACE (-4) /* workspace pointer null or too small        */
 #define GSX_E_UNSUPPORTED (-5) /* valid request this build does not cover  */
+#define GSX_E_SCRATCH (-6)   /* request exceeds the static scratch array of
+                                gsx_summate_adjoint_f32_noalloc           */
 
 /* MFMA split-precision variants for gsx_summate_grid_mfma. */
 #define GSX_MFMA_BF16X3 0 /* bf16 hi/lo split, hi*hi + hi*lo + lo*hi */
@@ -249,6 +251,29 @@ int gsx_summate_adjoint_f32(int64_t d, int64_t N, int64_t P, const double* cov,
                             const double* pos, const float* g, double* gz1,
                             double* gz2, double* gk, void* workspace,
                             int64_t workspace_bytes, void* stream);
+
+/* Allocation-free form of gsx_summate_adjoint_f32 (no counterpart in the
+ * reference).  For callers that cannot allocate device memory: code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc.
+ * Same arguments and results as gsx_summate_adjoint_f32 minus the
+ * workspace: the partial sums live in a static device array of
+ * GSX_NOALLOC_SCRATCH_BYTES per GPU, held from the library's first use on
+ * that GPU until the process ends.  It calls no hipMalloc / hipFree.
+ *   - limit: (2 + 2d) * N * 8 bytes must fit the array, i.e.
+ *     N <= GSX_NOALLOC_SCRATCH_BYTES / (16 (1 + d)) (1048576 modes for
+ *     d = 1, 524288 for d = 3); larger N returns GSX_E_SCRATCH;
+ *   - any P; whenever the workspace form's partial sums fit the array the
+ *     results equal gsx_summate_adjoint_f32's bit for bit, otherwise they
+ *     use fewer point chunks and differ in rounding only;
+ *   - every call on one GPU shares that GPU's array: calls that overlap on
+ *     one GPU (two streams, two threads) race and must be serialised by
+ *     the caller. */
+#define GSX_NOALLOC_SCRATCH_BYTES (32LL << 20)
+int gsx_summate_adjoint_f32_noalloc(int64_t d, int64_t N, int64_t P,
+                                    const double* cov, const double* z1,
+                                    const double* z2, const double* pos,
+                                    const float* g, double* gz1, double* gz2,
+                                    double* gk, void* stream);
 
 /* Kriging sums, FP64: for every target point k (column of krig_vecs),
  *   w = krig_mat . v_k,  field[k] = cond . w,  error[k] = v_k . w.

@@ -55,6 +55,13 @@ def test_invalid_arguments_are_rejected_on_the_host():
     assert lib.gsx_summate_adjoint_workspace(2, 2048, 10**6) > 0
     shape = (ctypes.c_int64 * 3)(10, 10, 1)
     assert lib.gsx_summate_grid_f64(2, 5, shape, null, null, null, null, null, null) == -1
+    # allocation-free adjoint: bad d, and more modes than its static array holds
+    args = [null] * 8 + [None]
+    assert lib.gsx_summate_adjoint_f32_noalloc(4, 10, 10, *args) == -1
+    too_many = _native.NOALLOC_SCRATCH_BYTES // (8 * 8) + 1
+    assert lib.gsx_summate_adjoint_f32_noalloc(3, too_many, 10, *args) == _native.GSX_E_SCRATCH
+    assert lib.gsx_summate_adjoint_f32_noalloc(1, 2 * too_many, 10, *args) == _native.GSX_E_SCRATCH
+    assert b"scratch" in lib.gsx_status_string(_native.GSX_E_SCRATCH)
 
 
 def test_python_boundary_validates_like_the_reference():
