@@ -65,6 +65,20 @@ def _is_covmodel(model):
     return isinstance(model, CovModel)
 
 
+def _cached_upload(gen, arrays):
+    """Device copies of a generator's mode arrays, uploaded again whenever an
+    array was replaced or written in place (the reference reads them on
+    every call).  The check compares against a host snapshot of what was
+    uploaded: O(N) per call, against the O(N P) sum."""
+    dev = torch.device("cuda", torch.cuda.current_device())
+    cached = gen._dev_modes
+    if (cached is None or cached[0] != dev or len(cached[1]) != len(arrays)
+            or not all(np.array_equal(a, s, equal_nan=True) for a, s in zip(arrays, cached[1]))):
+        snap = tuple(np.array(a, dtype=np.float64, copy=True, order="C") for a in arrays)
+        gen._dev_modes = (dev, snap, tuple(torch.from_numpy(a).to(dev) for a in snap))
+    return gen._dev_modes[2]
+
+
 class _GeneratorBase(ABC):
     """Abstract generator (``generator.py:78-162``)."""
 
@@ -128,13 +142,16 @@ class RandMeth(Generator):
 
     # --------------------------------------------------------------- call --
     def _device_modes(self):
-        dev = torch.device("cuda", torch.cuda.current_device())
-        key = (id(self._cov_sample), id(self._z_1), id(self._z_2), dev)
-        if self._dev_modes is None or self._dev_modes[0] != key:
-            tens = tuple(torch.from_numpy(np.ascontiguousarray(a, dtype=np.float64)).to(dev)
-                         for a in (self._cov_sample, self._z_1, self._z_2))
-            self._dev_modes = (key, tens)
-        return self._dev_modes[1]
+        return _cached_upload(self, (self._cov_sample, self._z_1, self._z_2))
+
+    def _folded_modes(self, iso_matrix):
+        """Device modes with the isometrize matrix folded in: k' = M^T k, so that
+        k'.x = k.(M x) and a rotated or anisotropic grid stays a tensor grid."""
+        cov, z1, z2 = self._device_modes()
+        mat = np.asarray(iso_matrix, dtype=np.float64)
+        if mat.shape != (cov.shape[0],) * 2:
+            raise ValueError(f"iso_matrix must be {cov.shape[0]} x {cov.shape[0]}")
+        return torch.as_tensor(mat.T.copy(), device=cov.device) @ cov, z1, z2
 
     def _scale(self):
         return np.sqrt(self.model.var / self._mode_no)
@@ -161,14 +178,17 @@ class RandMeth(Generator):
             nugget = torch.as_tensor(nugget, device=field.device)
         return field + nugget
 
-    def structured(self, axes, precision=None, add_nugget=True):
+    def structured(self, axes, precision=None, add_nugget=True, iso_matrix=None):
         """Field on the 'ij' meshgrid of ``axes`` without materialising it.
 
         ``precision`` may additionally be ``"bf16x3"`` or ``"fp16x3"`` (matrix
         cores, d = 2 or 3) or ``"fp64gemm"`` (FP64 GEMM on the matrix cores; it
-        skips the rounding of the phase sum, about |k.x| 2^-52 per term).  The positions must already be isotropic (no
-        rotation or anisotropy left); returns a torch tensor shaped like the
-        grid.
+        skips the rounding of the phase sum, about |k.x| 2^-52 per term).
+        ``iso_matrix`` (d, d): the field at the isometrized positions
+        ``iso_matrix @ x`` (rotation / anisotropy, ``covmodel/base.py:572-582``),
+        evaluated as k'.x with k' = iso_matrix^T k; the phases then differ from
+        the reference's k.(M x) in rounding only (about |k| |x| 2^-52).  Returns
+        a torch tensor shaped like the grid.
         """
         precision = precision or self.precision
         if precision not in GRID_PRECISIONS:
@@ -177,7 +197,7 @@ class RandMeth(Generator):
         if self.zero_var:
             out = torch.zeros(shape, dtype=torch.float64, device="cuda")
         else:
-            cov, z1, z2 = self._device_modes()
+            cov, z1, z2 = self._device_modes() if iso_matrix is None else self._folded_modes(iso_matrix)
             if precision in ("bf16x3", "fp16x3"):
                 out = summate_grid_device(cov, z1, z2, axes, precision, alpha=float(self._scale()))
             else:
@@ -400,13 +420,7 @@ class Fourier(Generator):
 
     # --------------------------------------------------------------- call --
     def _device_modes(self):
-        dev = torch.device("cuda", torch.cuda.current_device())
-        key = (id(self._spectrum_factor), id(self._modes), id(self._z_1), id(self._z_2), dev)
-        if self._dev_modes is None or self._dev_modes[0] != key:
-            tens = tuple(torch.from_numpy(np.ascontiguousarray(a, dtype=np.float64)).to(dev)
-                         for a in (self._spectrum_factor, self._modes, self._z_1, self._z_2))
-            self._dev_modes = (key, tens)
-        return self._dev_modes[1]
+        return _cached_upload(self, (self._spectrum_factor, self._modes, self._z_1, self._z_2))
 
     def __call__(self, pos, add_nugget=True):
         """Field at ``pos`` (d, n): ``summate_fourier(...) + nugget`` (``generator.py:659-691``).
@@ -430,7 +444,7 @@ class Fourier(Generator):
             nugget = torch.as_tensor(nugget, device=field.device)
         return field + nugget
 
-    def structured(self, axes, precision=None, add_nugget=True, method="tensor"):
+    def structured(self, axes, precision=None, add_nugget=True, method="tensor", iso_matrix=None):
         """Field on the 'ij' meshgrid of ``axes`` without materialising it.
 
         ``method="tensor"`` (default): the modes are a tensor grid, so the sum
@@ -439,7 +453,11 @@ class Fourier(Generator):
         which only picks the output dtype (float64 for "fp64" / "fp32", float32
         for "bf16x3" / "fp16x3").  ``method="direct"``: the per-point sum as in
         :meth:`RandMeth.structured` (``"bf16x3"`` / ``"fp16x3"`` on the matrix
-        cores).  Returns a torch tensor shaped like the grid.
+        cores).  ``iso_matrix`` (d, d): the field at ``iso_matrix @ x``,
+        evaluated with the modes k' = iso_matrix^T k; a diagonal matrix
+        (anisotropy only) keeps the modes a tensor grid, a rotation turns
+        ``method="tensor"`` into the direct sum.  Returns a torch tensor shaped
+        like the grid.
         """
         precision = precision or self.precision
         if precision not in GRID_PRECISIONS:
@@ -447,16 +465,23 @@ class Fourier(Generator):
         if method not in ("tensor", "direct"):
             raise ValueError("method must be 'tensor' or 'direct'")
         shape = tuple(int(np.size(a)) for a in axes)
+        mat = None if iso_matrix is None else np.asarray(iso_matrix, dtype=np.float64)
+        if mat is not None and mat.shape != (len(shape),) * 2:
+            raise ValueError(f"iso_matrix must be {len(shape)} x {len(shape)}")
+        diagonal = mat is None or np.array_equal(mat, np.diag(np.diag(mat)))
         if self.zero_var:
             out = torch.zeros(shape, dtype=torch.float64, device="cuda")
-        elif method == "tensor":
+        elif method == "tensor" and diagonal:
             fac, _, z1, z2 = self._device_modes()
-            out = summate_fourier_tensor_device(fac, self._kaxes, z1, z2, axes)
+            kaxes = self._kaxes if mat is None else [k * mat[i, i] for i, k in enumerate(self._kaxes)]
+            out = summate_fourier_tensor_device(fac, kaxes, z1, z2, axes)
             if precision in ("bf16x3", "fp16x3"):
                 out = out.to(torch.float32)
             out = out.reshape(shape)
         else:
             fac, modes, z1, z2 = self._device_modes()
+            if mat is not None:
+                modes = torch.as_tensor(mat.T.copy(), device=modes.device) @ modes
             out = summate_grid_device(modes, z1, z2, axes, precision, spectrum_factor=fac)
             if precision in ("fp64", "fp32"):
                 out = out.to(torch.float64)
