@@ -31,7 +31,7 @@ Public surface:
 
 from . import geometry, krige, variogram  # noqa: F401
 from .condsrf import CondSRF  # noqa: F401
-from .covmodel import CovModel, Exponential, Gaussian, Matern, Spherical, Stable  # noqa: F401
+from .covmodel import HANKEL_DEFAULT, CovModel, Exponential, Gaussian, Matern, Spherical, Stable  # noqa: F401
 from .generator import Fourier, Generator, IncomprRandMeth, RandMeth  # noqa: F401
 from .geometry import DEGREE_SCALE, EARTH_RADIUS, KM_SCALE, RADIAN_SCALE  # noqa: F401
 from .rng import RNG, MasterRNG  # noqa: F401

@@ -27,7 +27,11 @@ import scipy.special as sps
 from .geometry import matrix_anisometrize, matrix_isometrize, set_angles, set_anis
 from .hankel import SymmetricFourierTransform
 
-__all__ = ["CovModel", "Gaussian", "Exponential", "Stable", "Matern", "set_len_anis", "rad_fac"]
+__all__ = ["CovModel", "Gaussian", "Exponential", "Stable", "Matern", "set_len_anis", "rad_fac", "HANKEL_DEFAULT"]
+
+# Keyword arguments of the radial Fourier transform behind numerical spectral
+# densities: the values GSTools uses (``covmodel/base.py:59``).
+HANKEL_DEFAULT = {"a": -1, "b": 1, "N": 200, "h": 0.001, "alt": True}
 
 
 def rad_fac(dim, r):
@@ -71,10 +75,9 @@ class CovModel:
         self._len_scale = None
         self._anis = None
         self._angles = None
-        self._hankel_kw = {"N": 200, "h": 0.001}
-        if hankel_kw:
-            self._hankel_kw.update({k: v for k, v in hankel_kw.items() if k in ("N", "h")})
+        self._hankel_kw = None
         self._sft = None
+        self.hankel_kw = hankel_kw
         for name, default in self._opt_defaults.items():
             setattr(self, name, float(opt_arg.pop(name, default)))
         if opt_arg:
@@ -110,6 +113,28 @@ class CovModel:
         if self._init:
             self._len_scale, self._anis = set_len_anis(self._dim, self._len_scale, self._anis)
             self._angles = set_angles(self._dim, self._angles)
+
+    @property
+    def hankel_kw(self):
+        """Keyword arguments of the radial Fourier transform (``covmodel/base.py:1040-1052``).
+
+        ``None`` for models with an analytic spectral density.  Assigning a
+        dict updates the current keywords (starting from
+        :data:`HANKEL_DEFAULT`), assigning ``None`` restores the defaults;
+        either rebuilds the transform.
+        """
+        return self._hankel_kw
+
+    @hankel_kw.setter
+    def hankel_kw(self, hankel_kw):
+        if not self.needs_fourier_transform:
+            return
+        if self._hankel_kw is None or hankel_kw is None:
+            self._hankel_kw = dict(HANKEL_DEFAULT)
+        if hankel_kw is not None:
+            self._hankel_kw.update(hankel_kw)
+        if self._dim is not None:
+            self._sft = SymmetricFourierTransform(ndim=self._dim, **self._hankel_kw)
 
     @property
     def var(self):
@@ -418,28 +443,47 @@ class Stable(CovModel):
     """Stable model exp(-h^alpha) (``covmodel/models.py:276-348``).
 
     The reference has no analytic spectral density for this model and takes
-    the Hankel transform of the correlation (Ogata quadrature, N = 200,
-    h = 0.001, ``covmodel/base.py:59``, ``:535-551``).  That quadrature only
-    reaches r = 192 / k, so beyond k L ~ 20 it truncates the integral and
-    returns an O(k^-2) artefact (the transform of the correlation's value at
-    r = 0); through the |.| in the radial pdf (``covmodel/tools.py:395``) the
-    artefact makes the pdf decay like 1/k, which is not normalisable, and the
-    MCMC radii drift without bound.  Here the same quadrature is used where
-    it converges and the stable-density series (:func:`stable_density_tail`)
+    the radial Fourier transform of the correlation by Ogata's quadrature
+    with ``HANKEL_DEFAULT`` (``covmodel/base.py:59``, ``:535-551``).  That is
+    the default here too (``density="hankel"``), so the drop-in draws the
+    reference's modes.  Its nodes reach r = 197 / k only, so beyond
+    k L ~ 20 the quadrature truncates the integral and returns an O(k^-2)
+    artefact (the transform of the correlation's value near r = 0); through
+    the |.| in the radial pdf (``covmodel/tools.py:395``) the pdf then
+    decays like 1/k, which is not normalisable, and MCMC radii drift without
+    bound (2D, alpha 1.5, len 15, 2048 modes: median |k| ~ 21 where ~0.1 is
+    expected).
+
+    ``density="series"`` (opt-in) keeps the quadrature where it converges
+    and switches to the stable-density series (:func:`stable_density_tail`)
     where that series is accurate to 1e-8; alpha = 2 is the Gaussian closed
     form.  Both branches are checked against mpmath in the tests.
     """
 
     _opt_defaults = {"alpha": 1.5}
+    DENSITIES = ("hankel", "series")
+
+    def __init__(self, *args, density="hankel", **kwargs):
+        if density not in self.DENSITIES:
+            raise ValueError(f"Stable: density must be one of {self.DENSITIES}, got {density!r}")
+        self._density = density
+        super().__init__(*args, **kwargs)
+
+    @property
+    def density(self):
+        """``"hankel"`` (the reference's quadrature) or ``"series"`` (corrected tail)."""
+        return self._density
 
     @property
     def needs_fourier_transform(self):
-        return True  # the quadrature branch below
+        return True  # both densities use the quadrature at small k L
 
     def cor(self, h):
         return np.exp(-np.power(h, self.alpha))
 
     def spectral_density(self, k):
+        if self._density == "hankel":
+            return CovModel.spectral_density(self, k)
         k = np.asarray(np.abs(k), dtype=np.double)
         shape = k.shape
         k = np.atleast_1d(k).ravel()
@@ -462,6 +506,16 @@ class Stable(CovModel):
         if np.any(rest):
             res[rest] = self._sft.transform(self.correlation, k[rest])
         return res.reshape(shape)
+
+    def __eq__(self, other):
+        # a model without the switch (the reference's) uses the quadrature
+        return super().__eq__(other) and getattr(other, "density", "hankel") == self._density
+
+    __hash__ = None
+
+    def __repr__(self):
+        base = super().__repr__()
+        return base if self._density == "hankel" else base[:-1] + f", density={self._density!r})"
 
 
 class Matern(CovModel):

@@ -191,9 +191,10 @@ def test_hankel_stable_special_cases(dim):
 def test_hankel_matches_matern_density():
     m = Matern(dim=3, len_scale=8.0, nu=1.5)
     k = np.logspace(-3, 0, 20)
+    from gstools_amd import HANKEL_DEFAULT
     from gstools_amd.hankel import SymmetricFourierTransform
 
-    sft = SymmetricFourierTransform(ndim=3)
+    sft = SymmetricFourierTransform(ndim=3, **HANKEL_DEFAULT)
     assert np.allclose(sft.transform(m.correlation, k), m.spectral_density(k), rtol=5e-3)
 
 
@@ -216,7 +217,7 @@ def test_stable_spectral_density_matches_mpmath(dim, alpha):
     import os
 
     length = 15.0
-    m = Stable(dim=dim, len_scale=length, alpha=alpha)
+    m = Stable(dim=dim, len_scale=length, alpha=alpha, density="series")
     if alpha == 1.0:  # closed form: the Exponential model
         ys = np.array([0.5, 3.0, 20.0, 60.0, 500.0, 1e4])
         ref = Exponential(dim=dim, len_scale=length).spectral_density(ys / length)
@@ -230,8 +231,67 @@ def test_stable_spectral_density_matches_mpmath(dim, alpha):
 
 
 def test_stable_mcmc_radii_stay_physical():
-    """With a normalisable pdf the MCMC radii concentrate around 1 / len_scale."""
-    rm = RandMeth(Stable(dim=2, var=1.0, len_scale=15.0, alpha=1.5), mode_no=2048, seed=20170519)
+    """With the series density (normalisable pdf) the MCMC radii concentrate
+    around 1 / len_scale."""
+    rm = RandMeth(Stable(dim=2, var=1.0, len_scale=15.0, alpha=1.5, density="series"), mode_no=2048,
+                  seed=20170519)
     kn = np.linalg.norm(rm._cov_sample, axis=0)
     assert 0.02 < np.median(kn) < 0.2
     assert np.max(kn) < 1e3
+
+
+def test_stable_default_is_the_reference_quadrature():
+    """The default density is the reference's truncated quadrature for every
+    k (HANKEL_DEFAULT), which carries the O(k^-2) artefact beyond k L ~ 20:
+    negative values that the radial pdf turns positive, and MCMC radii far
+    above 1 / len_scale.  The series density agrees where the quadrature
+    converges."""
+    from gstools_amd import HANKEL_DEFAULT
+    from gstools_amd.hankel import SymmetricFourierTransform
+
+    m = Stable(dim=2, var=1.0, len_scale=15.0, alpha=1.5)
+    assert m.density == "hankel" and m.hankel_kw == HANKEL_DEFAULT
+    sft = SymmetricFourierTransform(ndim=2, a=-1, b=1, N=200, h=0.001)
+    k = np.array([0.01, 0.1, 1.0, 10.0, 100.0])
+    assert np.array_equal(m.spectral_density(k), sft.transform(m.correlation, k))
+    ser = Stable(dim=2, var=1.0, len_scale=15.0, alpha=1.5, density="series")
+    assert np.allclose(m.spectral_density(k[:3]), ser.spectral_density(k[:3]), rtol=1e-6)
+    assert m.spectral_density(10.0) < 0 < ser.spectral_density(10.0)
+    assert m != ser and m == Stable(dim=2, var=1.0, len_scale=15.0, alpha=1.5)
+    # the truncated density's tail dominates the MCMC draws
+    rm = RandMeth(m, mode_no=2048, seed=20170519)
+    assert np.median(np.linalg.norm(rm._cov_sample, axis=0)) > 1.0
+
+
+@pytest.mark.parametrize("dim", [1, 2, 3])
+@pytest.mark.parametrize("a,b", [(-1, 1), (1, 1), (0, 1), (-1, 2), (1, -0.5)])
+def test_hankel_fourier_convention(dim, a, b):
+    """(a, b) convention pinned by the Gaussian closed form:
+    (|b| / (2 pi)^(1-a))^(n/2) int exp(-r^2) e^(i b k.r) d^n r
+    = (|b| / (2 pi)^(1-a))^(n/2) pi^(n/2) exp(-b^2 k^2 / 4)."""
+    from gstools_amd.hankel import SymmetricFourierTransform
+
+    sft = SymmetricFourierTransform(ndim=dim, a=a, b=b, N=200, h=0.001)
+    k = np.array([0.2, 0.7, 1.5, 3.0])
+    ref = (abs(b) / (2 * np.pi) ** (1 - a)) ** (dim / 2) * np.pi ** (dim / 2) * np.exp(-(b * k) ** 2 / 4)
+    got = sft.transform(lambda r: np.exp(-r**2), k)
+    # in 1D the quadrature itself is only O(h) accurate (2e-3 at h = 0.001);
+    # the convention's factors are powers of 2 pi and |b|, far outside that
+    tol = 5e-3 if dim == 1 else 1e-5
+    ref0 = (abs(b) / (2 * np.pi) ** (1 - a)) ** (dim / 2) * np.pi ** (dim / 2)  # the value at k = 0
+    assert np.allclose(got, ref, rtol=tol, atol=tol * ref0)
+    with pytest.raises(NotImplementedError):
+        SymmetricFourierTransform(ndim=dim, alt=False)
+
+
+def test_hankel_kw_is_honoured():
+    """``hankel_kw`` updates the defaults and rebuilds the transform, as
+    ``covmodel/base.py:1044-1052`` does; analytic models keep ``None``."""
+    m = Stable(dim=3, len_scale=4.0, hankel_kw={"N": 400})
+    assert m.hankel_kw == {"a": -1, "b": 1, "N": 400, "h": 0.001, "alt": True}
+    assert m._sft.N == 400
+    m.hankel_kw = {"h": 0.002}
+    assert m.hankel_kw["N"] == 400 and m._sft.h == 0.002
+    m.hankel_kw = None
+    assert m.hankel_kw == {"a": -1, "b": 1, "N": 200, "h": 0.001, "alt": True}
+    assert Gaussian(dim=2).hankel_kw is None
