@@ -55,7 +55,7 @@ class BatchItem:
             self.P = int(np.prod(dims))
         self.incompr = bool(incompr)
         if self.incompr and self.d < 2:
-            raise ValueError("Only 2D and 3D incompressible fields can be generated.")
+            raise ValueError(f"BatchItem: incompressible items need d = 2 or 3, got d = {self.d}")
         nout = self.d if self.incompr else 1
         self.out = out if out is not None else torch.empty((nout, self.P), dtype=torch.float32, device=dev)
 

@@ -1,16 +1,23 @@
 """Conditioned spatial random fields: kriging plus a scaled unconditioned field.
 
-Restates ``CondSRF`` (``src/gstools/field/cond_srf.py:27-311``): the kriged
-field of the conditioning data plus an unconditioned RandMeth field (no
-nugget) scaled by the kriging standard deviation,
+The conditioned field at a target x (``src/gstools/field/cond_srf.py:27-311``)
+is built from three parts:
 
-    field = krige + sqrt(krige_var / var) * raw_field   (+ nugget noise),
+* the kriged estimate k(x) of the conditioning data and its kriging
+  variance s2(x) (:class:`gstools_amd.krige.Krige`, sums on the GPU);
+* an unconditioned field u(x) of the same model without nugget
+  (:class:`gstools_amd.RandMeth`, summation on the GPU);
+* nugget noise where the model has a nugget n.
 
-with the nugget split of ``get_scaling`` (``cond_srf.py:155-183``).  The
-random field comes from :class:`gstools_amd.RandMeth` (summation on the GPU)
-and the kriging sums from :mod:`gstools_amd.krige` (GPU).  When the positions
-are unchanged the kriging result of the previous call is reused, as in the
-reference.
+The kriging variance is shared between the random field and the noise: the
+field part carries max(s2 - n, 0), the noise the rest (min(s2, n)), so
+
+    field(x) = k(x) + sqrt(max(s2 - n, 0) / var) u(x) + sqrt(min(s2, n)) e(x)
+
+with e standard normal noise drawn by the generator (its ``get_nugget``,
+the same draw as the reference); without nugget the field part carries all
+of s2.  Mean and trend are added afterwards.  A call on the positions of the
+previous call reuses its kriging result.
 """
 
 import numpy as np
@@ -23,14 +30,26 @@ __all__ = ["CondSRF"]
 GENERATOR = {"RandMeth": RandMeth}
 
 
+def _same_positions(a, b):
+    """Whether two position arguments hold the same coordinates."""
+    if isinstance(a, (tuple, list)) != isinstance(b, (tuple, list)):
+        return False
+    if isinstance(a, (tuple, list)):
+        return len(a) == len(b) and all(np.array_equal(np.asarray(x), np.asarray(y)) for x, y in zip(a, b))
+    try:
+        return np.array_equal(np.asarray(a), np.asarray(b))
+    except ValueError:
+        return False
+
+
 class CondSRF:
-    """Conditioned SRF (``cond_srf.py:27-311``)."""
+    """Conditioned random field of a kriging setup (``cond_srf.py:27-311``)."""
 
     default_field_names = ["field", "raw_field", "raw_krige"]
 
     def __init__(self, krige, generator="RandMeth", **generator_kwargs):
         if not isinstance(krige, Krige):
-            raise ValueError("CondSRF: krige should be an instance of Krige.")
+            raise ValueError(f"CondSRF: needs a Krige instance, got {type(krige).__name__}")
         self._krige = krige
         self.fields = {}
         self.pos = None
@@ -39,13 +58,16 @@ class CondSRF:
         self.set_generator(generator, **generator_kwargs)
 
     def set_generator(self, generator, **generator_kwargs):
-        gen = GENERATOR[generator] if generator in GENERATOR else generator
-        if not isinstance(gen, type):
-            raise ValueError(f"gstools.CondSRF: Unknown or wrong generator: {generator}")
-        self._generator = gen(self.model, **generator_kwargs)
-        self.value_type = self._generator.value_type
-        if self.value_type != "scalar":
-            raise ValueError("CondSRF: only scalar generators can be conditioned")
+        """Build the unconditioned generator (a name or a generator class)."""
+        cls = GENERATOR.get(generator, generator) if isinstance(generator, str) else generator
+        if not isinstance(cls, type):
+            raise ValueError(f"CondSRF: unknown generator {generator!r}")
+        gen = cls(self.model, **generator_kwargs)
+        if gen.value_type != "scalar":
+            raise ValueError(f"CondSRF: conditioning needs a scalar field, {cls.__name__} gives "
+                             f"{gen.value_type} values")
+        self._generator = gen
+        self.value_type = gen.value_type
 
     @property
     def krige(self):
@@ -68,50 +90,47 @@ class CondSRF:
 
     def __call__(self, pos=None, seed=np.nan, mesh_type="unstructured", post_process=True, store=True,
                  krige_store=True, **kwargs):
-        """The conditioned field at ``pos`` (``cond_srf.py:68-153``)."""
+        """The conditioned field at ``pos`` (``cond_srf.py:63-153``)."""
         if pos is None:
             if self.pos is None:
-                raise ValueError("CondSRF: no positions given")
+                raise ValueError("CondSRF: no positions given and none stored")
             pos, mesh_type = self.pos, self.mesh_type
-        same = (self.pos is not None and self.mesh_type == mesh_type
-                and _same_pos(self.pos, pos))
-        if not same:
+        repeat = self.pos is not None and self.mesh_type == mesh_type and _same_positions(self.pos, pos)
+        if not repeat:
             self.fields = {}
         self.pos, self.mesh_type = pos, mesh_type
+        orig, shape = _positions(self.model.dim, pos, mesh_type)
         self.generator.update(self.model, seed)
-        xyz, shape = _positions(self.model.dim, pos, mesh_type)
-        iso = self.model.isometrize(xyz)
-        rawfield = np.reshape(self.generator(iso, add_nugget=False), shape)
-        if same and "raw_krige" in self.fields and "krige_var" in self.krige.fields:
-            rawkrige, krige_var = self.fields["raw_krige"], self.krige.fields["krige_var"]
-        else:
-            rawkrige, krige_var = self.krige(pos, mesh_type=mesh_type, only_mean=False, return_var=True,
-                                             post_process=False, store=bool(krige_store), **kwargs)
-            if krige_store:
-                self.krige.fields["field"] = self.krige._post(rawkrige.copy(), xyz, shape)
-        var_scale, nugget = self.get_scaling(krige_var, shape)
-        if store:
-            self.fields["raw_krige"] = rawkrige
-            self.fields["raw_field"] = rawfield
-        field = rawkrige + var_scale * rawfield + nugget
+        raw_field = np.reshape(self.generator(self.model.isometrize(orig), add_nugget=False), shape)
+        krige_est, krige_var = self._kriging(pos, mesh_type, orig, shape, repeat, krige_store, kwargs)
+        scale, noise = self.get_scaling(krige_var, shape)
+        field = krige_est + scale * raw_field + noise
         if post_process:
-            field = field + eval_func(self.krige.mean, xyz, self.model.dim, broadcast=True).reshape(shape)
-            field = field + eval_func(self.krige.trend, xyz, self.model.dim, broadcast=True).reshape(shape)
+            field = field + (eval_func(self.krige.mean, orig, self.model.dim, broadcast=True)
+                             + eval_func(self.krige.trend, orig, self.model.dim, broadcast=True)).reshape(shape)
         if store:
-            self.fields["field"] = field
+            self.fields.update(raw_krige=krige_est, raw_field=raw_field, field=field)
         return field
 
+    def _kriging(self, pos, mesh_type, orig, shape, repeat, krige_store, kwargs):
+        """Kriged estimate (no mean / trend) and variance, reused for repeated positions."""
+        if repeat and "raw_krige" in self.fields and "krige_var" in self.krige.fields:
+            return self.fields["raw_krige"], self.krige.fields["krige_var"]
+        est, var = self.krige(pos, mesh_type=mesh_type, only_mean=False, return_var=True, post_process=False,
+                              store=bool(krige_store), **kwargs)
+        if krige_store:
+            self.krige.fields["field"] = self.krige._post(est.copy(), orig, shape)
+        return est, var
+
     def get_scaling(self, krige_var, shape):
-        """Scale of the random field and nugget noise (``cond_srf.py:155-183``)."""
-        if self.model.nugget > 0:
-            var_scale = np.maximum(krige_var - self.model.nugget, 0)
-            nug_scale = np.sqrt((krige_var - var_scale) / self.model.nugget)
-            var_scale = np.sqrt(var_scale / self.model.var)
-            nugget = nug_scale * self.generator.get_nugget(shape)
-        else:
-            var_scale = np.sqrt(krige_var / self.model.var)
-            nugget = 0
-        return var_scale, nugget
+        """(field scale, nugget noise) for a kriging variance (``cond_srf.py:155-183``)."""
+        nug, var = self.model.nugget, self.model.var
+        if nug <= 0:
+            return np.sqrt(krige_var / var), 0
+        field_part = np.maximum(krige_var - nug, 0)
+        noise_std = np.sqrt(krige_var - field_part)  # min(s2, n)
+        # the generator's nugget draw is sqrt(n) e: scale it to sqrt(min(s2, n)) e
+        return np.sqrt(field_part / var), noise_std / np.sqrt(nug) * self.generator.get_nugget(shape)
 
     def structured(self, *args, **kwargs):
         return self(*args, mesh_type="structured", **kwargs)
@@ -121,12 +140,3 @@ class CondSRF:
 
     def __repr__(self):
         return f"CondSRF(krige={self.krige!r}, generator={self.generator.name})"
-
-
-def _same_pos(a, b):
-    try:
-        if isinstance(a, (tuple, list)) and isinstance(b, (tuple, list)):
-            return len(a) == len(b) and all(np.array_equal(np.asarray(x), np.asarray(y)) for x, y in zip(a, b))
-        return np.array_equal(np.asarray(a), np.asarray(b))
-    except ValueError:
-        return False

@@ -125,7 +125,7 @@ class RandMeth(Generator):
 
     def __init__(self, model, *, mode_no=1000, seed=None, sampling="auto", precision="fp64", **kwargs):
         if kwargs:
-            warnings.warn("gstools.RandMeth: **kwargs are ignored")
+            warnings.warn(f"{type(self).__name__}: unused keyword arguments {sorted(kwargs)}")
         self._mode_no = int(mode_no)
         self._model = None
         self._seed = None
@@ -231,13 +231,13 @@ class RandMeth(Generator):
             if _is_covmodel(self._model):
                 self.seed = seed
             else:
-                raise ValueError("gstools.field.generator.RandMeth: no 'model' given")
+                raise ValueError(f"{self.name}.update: a seed was given but the generator has no model yet")
         elif model is None and np.isnan(seed):
             if not (_is_covmodel(self._model) and self._z_1 is not None
                     and self._z_2 is not None and self._cov_sample is not None):
-                raise ValueError("gstools.field.generator.RandMeth: neither 'model' nor 'seed' given!")
+                raise ValueError(f"{self.name}.update: needs a model or a seed (no modes to keep)")
         else:
-            raise ValueError("gstools.field.generator.RandMeth: 'model' is not an instance of 'gstools.CovModel'")
+            raise ValueError(f"{self.name}.update: model must be a CovModel, got {type(model).__name__}")
 
     def reset_seed(self, seed=np.nan):
         """Redraw z_1, z_2 and the wavenumbers (``generator.py:346-387``)."""
@@ -269,7 +269,7 @@ class RandMeth(Generator):
     @sampling.setter
     def sampling(self, sampling):
         if sampling not in SAMPLING:
-            raise ValueError(f"RandMeth: sampling not in {SAMPLING}.")
+            raise ValueError(f"RandMeth: sampling must be one of {SAMPLING}, got {sampling!r}")
         self._sampling = sampling
 
     @property
@@ -323,7 +323,7 @@ class IncomprRandMeth(RandMeth):
     def __init__(self, model, *, mean_velocity=1.0, mode_no=1000, seed=None, sampling="auto",
                  precision="fp64", **kwargs):
         if model.dim < 2 or model.dim > 3:
-            raise ValueError("Only 2D and 3D incompressible fields can be generated.")
+            raise ValueError(f"IncomprRandMeth: incompressible fields need dim 2 or 3, got {model.dim}")
         super().__init__(model=model, mode_no=mode_no, seed=seed, sampling=sampling,
                          precision=precision, **kwargs)
         self.mean_u = mean_velocity
@@ -402,7 +402,7 @@ class Fourier(Generator):
 
     def __init__(self, model, period, mode_no=32, seed=None, precision="fp64", **kwargs):
         if kwargs:
-            warnings.warn("gstools.Fourier: **kwargs are ignored")
+            warnings.warn(f"Fourier: unused keyword arguments {sorted(kwargs)}")
         self._modes = None
         self._period = None
         self._mode_no = None
@@ -512,7 +512,7 @@ class Fourier(Generator):
         if mode_no is not None:
             mode_no = self._fill_to_dim(mode_no, dim)
             if (np.asarray([m % 2 for m in mode_no]) != 0).any():
-                raise ValueError("Fourier: Odd mode_no not supported.")
+                raise ValueError(f"Fourier: mode_no must be even in every dimension, got {list(mode_no)}")
             self._set_modes(mode_no, dim)
 
         if _is_covmodel(model):
@@ -528,7 +528,7 @@ class Fourier(Generator):
             if _is_covmodel(self._model):
                 self.seed = seed
             else:
-                raise ValueError("gstools.field.generator.Fourier: no 'model' given")
+                raise ValueError(f"{self.name}.update: a seed was given but the generator has no model yet")
         elif mode_no is not None or period is not None:
             if seed is None or not np.isnan(seed):
                 self.reset_seed(seed)
@@ -537,9 +537,9 @@ class Fourier(Generator):
         elif model is None and np.isnan(seed):
             if (_is_covmodel(self._model) and self._z_1 is not None and self._z_2 is not None
                     and self._spectrum_factor is not None):
-                raise ValueError("gstools.field.generator.Fourier: neither 'model' nor 'seed' given!")
+                raise ValueError(f"{self.name}.update: needs a model, a seed, a period or mode numbers")
         else:
-            raise ValueError("gstools.field.generator.Fourier: 'model' is not an instance of 'gstools.CovModel'")
+            raise ValueError(f"{self.name}.update: model must be a CovModel, got {type(model).__name__}")
 
     def reset_seed(self, seed=np.nan):
         """Redraw z_1, z_2 and recompute the spectrum factors (``generator.py:791-820``)."""
@@ -561,7 +561,7 @@ class Fourier(Generator):
         r = np.atleast_1d(values)
         if values is None:
             if default_value is None:
-                raise ValueError("Fourier: Value has to be provided")
+                raise ValueError("Fourier: a value or a default is required")
             r = default_value
         r = np.atleast_1d(np.array(r, dtype=dtype))[:dim]
         if len(r) < dim:

@@ -36,10 +36,8 @@ from .summate import _as_f64, _device, _ptr, _stream, host_array
 
 __all__ = [
     "calc_field_krige", "calc_field_krige_and_variance", "krige_sum_device", "krige_cov_device",
-    "Krige", "Simple", "Ordinary", "Universal", "ExtDrift", "Detrended",
+    "Krige", "KrigingSystem", "Simple", "Ordinary", "Universal", "ExtDrift", "Detrended",
 ]
-
-P_INV = {"pinv": spl.pinv, "pinvh": spl.pinvh}
 
 
 # ------------------------------------------------------------------ sums ----
@@ -125,49 +123,54 @@ def _device_cov_kind(model):
 
 
 # ---------------------------------------------------------------- helpers ---
-def set_condition(cond_pos, cond_val, dim):
-    """Shape checks and removal of non-finite values (``krige/tools.py:16-47``)."""
-    cond_val = np.asarray(cond_val, dtype=np.double).reshape(-1)
-    cond_pos = np.asarray(cond_pos, dtype=np.double).reshape(dim, -1)
-    if len(cond_pos[0]) != len(cond_val):
-        raise ValueError("Please check your 'cond_pos' and 'cond_val' parameters. The shapes do not match.")
-    mask = np.isfinite(cond_val)
-    return cond_pos[:, mask], cond_val[mask]
+def clean_conditions(cond_pos, cond_val, dim):
+    """Conditioning data as (dim, K) positions and (K,) values, dropping
+    points whose value is not finite (``krige/tools.py:16-47``)."""
+    vals = np.asarray(cond_val, dtype=np.double).ravel()
+    pts = np.asarray(cond_pos, dtype=np.double).reshape(dim, -1)
+    if pts.shape[1] != vals.size:
+        raise ValueError(f"Krige: {pts.shape[1]} conditioning positions but {vals.size} values")
+    keep = np.isfinite(vals)
+    return pts[:, keep], vals[keep]
 
 
-def _monomial(select):
-    def f(*pos):
-        res = 1.0
-        for i in select:
-            res = res * np.asarray(pos[i])
-        return res
+def polynomial_drifts(dim, order):
+    """Monomials x_a x_b ... of total degree 1..order in ``dim`` variables
+    (``krige/tools.py:50-85``); ``order`` may be "linear" / "quadratic"."""
+    named = {"lin": 1, "linear": 1, "quad": 2, "quadratic": 2}
+    order = named[order] if order in named else int(order)
+    out = []
+    for degree in range(1, order + 1):
+        for axes in combinations_with_replacement(range(dim), degree):
+            out.append(_Monomial(axes))
+    return out
 
-    return f
 
+class _Monomial:
+    """Product of the coordinates listed in ``axes`` (repeats allowed)."""
 
-def get_drift_functions(dim, drift_type):
-    """Polynomial drift monomials up to the given order (``krige/tools.py:50-85``)."""
-    if drift_type in ["lin", "linear"]:
-        drift_type = 1
-    elif drift_type in ["quad", "quadratic"]:
-        drift_type = 2
-    else:
-        drift_type = int(drift_type)
-    funcs = []
-    for d in range(drift_type):
-        for select in combinations_with_replacement(range(dim), d + 1):
-            funcs.append(_monomial(select))
-    return funcs
+    def __init__(self, axes):
+        self.axes = tuple(axes)
+
+    def __call__(self, *coords):
+        val = 1.0
+        for a in self.axes:
+            val = val * np.asarray(coords[a], dtype=np.double)
+        return val
+
+    def __repr__(self):
+        return "*".join(f"x{a}" for a in self.axes)
 
 
 def eval_func(func_val, pos, dim, broadcast=False):
-    """Mean / trend at positions (d, n) (``tools/misc.py:62-141``): a callable is evaluated,
-    ``None`` is 0, any other value is a constant (its first element, as for a scalar field)."""
-    pos = np.asarray(pos, dtype=np.double).reshape(dim, -1)
+    """A mean or trend at positions (d, n) (``tools/misc.py:62-141``): callables are
+    evaluated, ``None`` is 0, anything else a constant (its first element)."""
+    pts = np.asarray(pos, dtype=np.double).reshape(dim, -1)
+    n = pts.shape[1]
     if callable(func_val):
-        return np.asarray(func_val(*pos), dtype=np.double).reshape(-1) * np.ones(pos.shape[1])
-    val = 0.0 if func_val is None else float(np.asarray(func_val, dtype=np.double).ravel()[0])
-    return np.full(pos.shape[1], val) if broadcast else val
+        return np.broadcast_to(np.asarray(func_val(*pts), dtype=np.double).ravel(), (n,)).copy()
+    const = 0.0 if func_val is None else float(np.ravel(np.asarray(func_val, dtype=np.double))[0])
+    return np.full(n, const) if broadcast else const
 
 
 def _positions(dim, pos, mesh_type):
@@ -186,9 +189,101 @@ def _positions(dim, pos, mesh_type):
     return arr, (arr.shape[1],)
 
 
+# -------------------------------------------------------- linear system ----
+class KrigingSystem:
+    """The kriging equations of one conditioning set.
+
+    K conditioning points carry K weights; r = u + m + q constraints (u = 1
+    for unbiasedness, m drift functions, q external drifts) carry Lagrange
+    multipliers.  With C the covariance between the conditioning points,
+    E the diagonal of measurement-error variances and F (K x r) the
+    constraint values at the conditioning points,
+
+        A = [[C + E, F],
+             [F^T,   0]],       M = A^-1 (or a pseudo-inverse).
+
+    A target x has the right-hand side b(x) = [c(x); f(x)]: its covariances
+    to the conditioning points (with the sill at distance 0 for an exact
+    interpolator) and its constraint values.  With the residual data
+    y = [v - mean - trend; 0 (r entries)],
+
+        estimate(x) = y . M b(x),     error(x) = b(x) . M b(x),
+
+    and the kriging variance is max(sill - error, 0).  This is the system
+    ``krige/base.py:330-388`` builds; the per-target sums are
+    ``gsx_krige_f64``.
+    """
+
+    def __init__(self, model, iso_pos, constraints, errors, inverse):
+        self.model = model
+        self.iso_pos = iso_pos                  # (d, K) isometrized conditioning points
+        self.constraints = constraints          # (K, r)
+        k, r = constraints.shape
+        a = np.zeros((k + r, k + r), dtype=np.double)
+        a[:k, :k] = model.covariance(cdist(iso_pos.T, iso_pos.T))
+        a[np.arange(k), np.arange(k)] += errors
+        a[:k, k:] = constraints
+        a[k:, :k] = constraints.T
+        self.matrix = a
+        self.inverse = inverse(a)
+
+    @property
+    def size(self):
+        return self.matrix.shape[0]
+
+    @property
+    def cond_no(self):
+        return self.constraints.shape[0]
+
+    def rhs_host(self, iso, target_constraints, exact, covariance=True):
+        """b(x) for the columns of ``iso`` (d, n) as a host array (K + r, n)."""
+        k = self.cond_no
+        out = np.empty((self.size, iso.shape[1]), dtype=np.double)
+        if covariance:
+            cov = self.model.cov_nugget if exact else self.model.covariance
+            out[:k] = cov(cdist(self.iso_pos.T, iso.T))
+        else:
+            out[:k] = 0.0
+        out[k:] = target_constraints
+        return out
+
+    def rhs_device(self, iso, target_constraints, exact):
+        """b(x) on the GPU (covariance rows from ``gsx_krige_cov_f64``), or
+        None when the model has no device covariance."""
+        if _device_cov_kind(self.model) is None or self.cond_no > 65535:
+            return None
+        k = self.cond_no
+        out = torch.empty((self.size, iso.shape[1]), dtype=torch.float64, device=_device())
+        krige_cov_device(self.model, self.iso_pos, np.ascontiguousarray(iso), exact=exact, out=out, row0=0)
+        if self.size > k:
+            out[k:] = torch.from_numpy(np.ascontiguousarray(target_constraints)).to(out.device)
+        return out
+
+    def estimate(self, residual, rhs, variance):
+        """``y . M b`` (and ``b . M b``) for every column of ``rhs``, on the GPU."""
+        return krige_sum_device(self.inverse, rhs, residual, variance=variance)
+
+    def unit_estimate(self, residual, row):
+        """``y . M e_row``: the estimate for a right-hand side that is the unit
+        vector of constraint ``row`` (the fitted constant of ordinary kriging)."""
+        unit = np.zeros(self.size)
+        unit[self.cond_no + row] = 1.0
+        return float(residual @ (self.inverse @ unit))
+
+
+_INVERSES = {"pinv": spl.pinv, "pinvh": spl.pinvh}
+
+
 # ----------------------------------------------------------------- Krige ----
 class Krige:
-    """Kriging front end (``krige/base.py:64-753``) with the sums on the GPU."""
+    """Kriging front end with the per-target sums on the GPU.
+
+    Same constructor, call signature and properties as the reference's
+    ``Krige`` (``krige/base.py:64-753``); the equations are documented on
+    :class:`KrigingSystem`.  Conditioning points with non-finite values are
+    dropped; polynomial / callable drifts are evaluated at the original
+    (not isometrized) positions, external drifts are given per point.
+    """
 
     def __init__(self, model, cond_pos, cond_val, drift_functions=None, ext_drift=None, mean=None,
                  normalizer=None, trend=None, unbiased=True, exact=False, cond_err="nugget", pseudo_inv=True,
@@ -197,84 +292,165 @@ class Krige:
             raise NotImplementedError("Krige: data normalizers are not part of this build")
         if fit_variogram:
             raise NotImplementedError("Krige: variogram fitting is not part of this build")
-        if cond_pos is None or cond_val is None:
-            raise ValueError("Krige.set_condition: missing cond_pos/cond_val.")
         self.model = model
         self.mean = mean
         self.trend = trend
         self._unbiased = bool(unbiased)
         self._exact = bool(exact)
         self._pseudo_inv = bool(pseudo_inv)
-        self._pseudo_inv_type = None
         self.pseudo_inv_type = pseudo_inv_type
-        self._cond_pos = None
-        self._cond_val = None
-        self._cond_err = None
-        self._krige_mat = None
-        self._krige_pos = None
-        self._cond_ext_drift = np.array([])
-        self._drift_functions = []
+        self._drifts = []
+        self._cond = None            # (positions, values) after cleaning
+        self._err_spec = "nugget"    # "nugget", a scalar or one variance per point
+        self._cond_ext = np.empty((0, 0))
+        self._system = None
         self.fields = {}
         self.pos = None
         self.mesh_type = "unstructured"
         self.set_drift_functions(drift_functions)
         self.set_condition(cond_pos, cond_val, ext_drift, cond_err)
 
-    @property
-    def dim(self):
-        return self.model.dim
+    # -------------------------------------------------------- setup --
+    def set_drift_functions(self, drift_functions=None):
+        """Internal drifts: None, a polynomial order ("linear", "quadratic",
+        an int) or callables of the coordinates (``krige/base.py:588-624``)."""
+        if drift_functions is None:
+            drifts = []
+        elif isinstance(drift_functions, (str, int, np.integer)):
+            drifts = polynomial_drifts(self.dim, drift_functions)
+        else:
+            drifts = [drift_functions] if callable(drift_functions) else list(drift_functions)
+            bad = [f for f in drifts if not callable(f)]
+            if bad:
+                raise ValueError(f"Krige: drift functions must be callable, got {bad[0]!r}")
+        self._drifts = drifts
+        if self._cond is not None:
+            self._rebuild()
 
-    @property
-    def name(self):
-        return self.__class__.__name__
+    def set_condition(self, cond_pos=None, cond_val=None, ext_drift=None, cond_err=None):
+        """Replace (parts of) the conditioning data and rebuild the system
+        (``krige/base.py:489-586``); omitted parts keep their values, and
+        the external drift is kept only when the positions are."""
+        if cond_pos is None and ext_drift is None:
+            ext_drift = self._cond_ext if self._cond_ext.size else None
+        pos_in = self._cond[0] if cond_pos is None and self._cond is not None else cond_pos
+        val_in = self._cond[1] if cond_val is None and self._cond is not None else cond_val
+        if pos_in is None or val_in is None:
+            raise ValueError("Krige: conditioning positions and values are required")
+        self._cond = clean_conditions(pos_in, val_in, self.dim)
+        k = self._cond[1].size
+        if ext_drift is None:
+            self._cond_ext = np.empty((0, k))
+        else:
+            ext = np.atleast_2d(np.asarray(ext_drift, dtype=np.double))
+            if ext.ndim != 2 or ext.shape[1] != k:
+                raise ValueError(f"Krige: external drift at the conditioning points must be (q, {k}), "
+                                 f"got shape {ext.shape}")
+            self._cond_ext = ext
+        self.cond_err = self._err_spec if cond_err is None else cond_err
+        self._rebuild()
 
-    # ----------------------------------------------------------- call --
+    def _rebuild(self):
+        inv = spl.inv if not self._pseudo_inv else (
+            self._pseudo_inv_type if callable(self._pseudo_inv_type) else _INVERSES[self._pseudo_inv_type])
+        iso = self.model.isometrize(self.cond_pos)
+        errors = np.broadcast_to(np.asarray(self.cond_err, dtype=np.double), (self.cond_no,))
+        constraints = self._constraint_rows(self.cond_pos, self._cond_ext).T
+        self._system = KrigingSystem(self.model, iso, np.ascontiguousarray(constraints), errors, inv)
+        self.fields = {}
+
+    def _constraint_rows(self, orig_pos, ext):
+        """(r, n) constraint values: ones, drifts at ``orig_pos``, external drifts."""
+        n = orig_pos.shape[1]
+        rows = [np.ones(n)] if self._unbiased else []
+        rows += [np.broadcast_to(np.asarray(f(*orig_pos), dtype=np.double).ravel(), (n,)) for f in self._drifts]
+        rows = np.array(rows, dtype=np.double).reshape(len(rows), n)
+        return np.concatenate([rows, ext], axis=0) if ext.size else rows
+
+    def _target_ext_drift(self, n, ext_drift):
+        """External drift at n targets as (q, n); required iff the conditioning has one."""
+        q = self.ext_drift_no
+        if ext_drift is None:
+            if q:
+                raise ValueError(f"Krige: {q} external drift(s) needed at the targets")
+            return np.empty((0, n))
+        ext = np.asarray(ext_drift, dtype=np.double)
+        if ext.size == 0 and q == 0:
+            return np.empty((0, n))
+        if ext.size != q * n or (ext.ndim == 2 and q > 1 and ext.shape[0] != q):
+            raise ValueError(f"Krige: expected {q} external drift value(s) for each of {n} targets, "
+                             f"got shape {ext.shape}")
+        return ext.reshape(q, n)
+
+    # ---------------------------------------------------------- call --
     def __call__(self, pos=None, mesh_type="unstructured", ext_drift=None, chunk_size=None, only_mean=False,
                  return_var=True, post_process=True, store=True):
-        """Kriged field (and variance) at ``pos`` (``krige/base.py:209-305``)."""
-        return_var &= not only_mean
+        """Kriged field (and kriging variance) at ``pos`` (``krige/base.py:209-305``)."""
         if pos is None:
             if self.pos is None:
                 raise ValueError("Krige: no positions given")
             pos, mesh_type = self.pos, self.mesh_type
         self.pos, self.mesh_type = pos, mesh_type
-        xyz, shape = _positions(self.dim, pos, mesh_type)
-        iso_pos = self.model.isometrize(xyz)
-        pnt_cnt = iso_pos.shape[1]
-        field = np.empty(pnt_cnt, dtype=np.double)
-        krige_var = np.empty(pnt_cnt, dtype=np.double) if return_var else None
+        want_var = bool(return_var) and not only_mean
+        orig, shape = _positions(self.dim, pos, mesh_type)
+        iso = self.model.isometrize(orig)
+        n = iso.shape[1]
+        field = np.empty(n)
+        error = np.empty(n) if want_var else None
         if only_mean and self.drift_no == 0:
-            field[...] = self.get_mean(post_process=False)
+            field[:] = self.get_mean(post_process=False)
         else:
-            chunk_size = pnt_cnt if chunk_size is None else int(chunk_size)
-            chunk_no = int(np.ceil(pnt_cnt / chunk_size)) if pnt_cnt else 0
-            ext_drift = self._pre_ext_drift(pnt_cnt, ext_drift)
-            on_device = not only_mean and _device_cov_kind(self.model) is not None and self.cond_no <= 65535
-            for i in range(chunk_no):
-                sl = (i * chunk_size, min(pnt_cnt, (i + 1) * chunk_size))
-                if on_device:
-                    k_vec = self._get_krige_vecs_device(iso_pos, sl, ext_drift)
+            ext = self._target_ext_drift(n, ext_drift)
+            step = n if chunk_size is None else max(int(chunk_size), 1)
+            residual = self._residual()
+            for lo in range(0, n, step):
+                hi = min(n, lo + step)
+                cons = self._constraint_rows(orig[:, lo:hi], ext[:, lo:hi])
+                rhs = None if only_mean else self._system.rhs_device(iso[:, lo:hi], cons, self._exact)
+                if rhs is None:
+                    rhs = self._system.rhs_host(iso[:, lo:hi], cons, self._exact, covariance=not only_mean)
+                res = self._system.estimate(residual, rhs, want_var)
+                if want_var:
+                    field[lo:hi], error[lo:hi] = host_array(res[0]), host_array(res[1])
                 else:
-                    k_vec = self._get_krige_vecs(iso_pos, sl, ext_drift, only_mean)
-                self._summate(field, krige_var, slice(*sl), k_vec, return_var)
-        field = np.reshape(field, shape)
+                    field[lo:hi] = host_array(res)
+        field = field.reshape(shape)
         if post_process:
-            field = self._post(field, xyz, shape)
+            field = self._post(field, orig, shape)
         if store:
             self.fields["mean_field" if only_mean else "field"] = field
-        if return_var:
-            krige_var = np.reshape(np.maximum(self.model.sill - krige_var, 0), shape)
-            if store:
-                self.fields["krige_var"] = krige_var
-            return field, krige_var
-        return field
+        if not want_var:
+            return field
+        variance = np.maximum(self.model.sill - error, 0.0).reshape(shape)
+        if store:
+            self.fields["krige_var"] = variance
+        return field, variance
 
-    def _post(self, field, xyz, shape):
-        """Add mean and trend like ``apply_mean_norm_trend`` without a normalizer."""
-        mean = eval_func(self.mean, xyz, self.dim, broadcast=True).reshape(shape)
-        trend = eval_func(self.trend, xyz, self.dim, broadcast=True).reshape(shape)
+    def _post(self, field, orig, shape):
+        """Add back mean and trend (``apply_mean_norm_trend`` without a normalizer)."""
+        mean = eval_func(self.mean, orig, self.dim, broadcast=True).reshape(shape)
+        trend = eval_func(self.trend, orig, self.dim, broadcast=True).reshape(shape)
         return field + mean + trend
 
+    def _residual(self):
+        """y = [v - trend - mean; 0 per constraint]."""
+        y = np.zeros(self._system.size)
+        y[: self.cond_no] = (self.cond_val - self.cond_trend) - self.cond_mean
+        return y
+
+    def get_mean(self, post_process=True):
+        """Estimated constant mean (``krige/base.py:452-487``): the fitted
+        constant of the unbiasedness constraint (0 without it), plus the
+        given mean when ``post_process``; None when there is no constant
+        mean (drifts, or a callable mean asked for with ``post_process``)."""
+        if self.drift_no > 0 or (post_process and callable(self.mean)):
+            return None
+        fitted = self._system.unit_estimate(self._residual(), 0) if self._unbiased else 0.0
+        if not post_process:
+            return fitted
+        return fitted + (0.0 if self.mean is None else self.mean)
+
+    # -------------------------------------------------- properties --
     def __getitem__(self, name):
         return self.fields[name]
 
@@ -282,192 +458,56 @@ class Krige:
     def field_names(self):
         return list(self.fields)
 
-    def _summate(self, field, krige_var, c_slice, k_vec, return_var):
-        if return_var:
-            f, e = krige_sum_device(self._krige_mat, k_vec, self._krige_cond, variance=True)
-            field[c_slice] = host_array(f)
-            krige_var[c_slice] = host_array(e)
-        else:
-            field[c_slice] = host_array(krige_sum_device(self._krige_mat, k_vec, self._krige_cond, variance=False))
-
-    # ------------------------------------------------------- system ----
-    def _inv(self, mat):
-        if self.pseudo_inv:
-            if callable(self.pseudo_inv_type):
-                return self.pseudo_inv_type(mat)
-            return P_INV[self.pseudo_inv_type](mat)
-        return spl.inv(mat)
-
-    def _get_krige_mat(self):
-        """Inverse of the kriging matrix (``krige/base.py:330-355``)."""
-        res = np.empty((self.krige_size, self.krige_size), dtype=np.double)
-        res[: self.cond_no, : self.cond_no] = self.model.covariance(cdist(self._krige_pos.T, self._krige_pos.T))
-        res[np.diag_indices(self.cond_no)] += self.cond_err
-        if self.unbiased:
-            res[self.cond_no, : self.cond_no] = 1
-            res[: self.cond_no, self.cond_no] = 1
-        for i, f in enumerate(self.drift_functions):
-            drift_tmp = f(*self.cond_pos)
-            res[-self.drift_no + i, : self.cond_no] = drift_tmp
-            res[: self.cond_no, -self.drift_no + i] = drift_tmp
-        if self.ext_drift_no > 0:
-            ext_size = self.krige_size - self.ext_drift_no
-            res[ext_size:, : self.cond_no] = self.cond_ext_drift
-            res[: self.cond_no, ext_size:] = self.cond_ext_drift.T
-        res[self.cond_no:, self.cond_no:] = 0
-        return self._inv(res)
-
-    def _get_krige_vecs(self, pos, chunk_slice=(0, None), ext_drift=None, only_mean=False):
-        """Right-hand sides of a chunk of target points (``krige/base.py:357-388``)."""
-        chunk_size = (pos.shape[1] if chunk_slice[1] is None else chunk_slice[1]) - chunk_slice[0]
-        res = np.empty((self.krige_size, chunk_size), dtype=np.double)
-        if only_mean:
-            res[: self.cond_no, :] = 0
-        else:
-            cf = self.model.cov_nugget if self.exact else self.model.covariance
-            res[: self.cond_no, :] = cf(cdist(self._krige_pos.T, pos.T[slice(*chunk_slice), ...]))
-        if self.unbiased:
-            res[self.cond_no, :] = 1
-        chunk_pos = None
-        if self.int_drift_no > 0:
-            chunk_pos = self.model.anisometrize(pos)[:, slice(*chunk_slice)]
-        for i, f in enumerate(self.drift_functions):
-            res[-self.drift_no + i, :] = f(*chunk_pos)
-        if self.ext_drift_no > 0:
-            ext_size = self.krige_size - self.ext_drift_no
-            res[ext_size:, :] = ext_drift[:, slice(*chunk_slice)]
-        return res
-
-    def _get_krige_vecs_device(self, pos, chunk_slice, ext_drift=None):
-        """``_get_krige_vecs`` with the covariance rows computed on the GPU (a device tensor)."""
-        dev = _device()
-        chunk_pos = np.ascontiguousarray(pos[:, slice(*chunk_slice)])
-        res = torch.empty((self.krige_size, chunk_pos.shape[1]), dtype=torch.float64, device=dev)
-        krige_cov_device(self.model, self._krige_pos, chunk_pos, exact=self.exact, out=res, row0=0)
-        extra = self.krige_size - self.cond_no
-        if extra:
-            rows = np.empty((extra, chunk_pos.shape[1]), dtype=np.double)
-            off = 0
-            if self.unbiased:
-                rows[0] = 1
-                off = 1
-            if self.int_drift_no > 0:
-                drift_pos = self.model.anisometrize(pos)[:, slice(*chunk_slice)]
-                for i, f in enumerate(self.drift_functions):
-                    rows[off + i] = f(*drift_pos)
-            if self.ext_drift_no > 0:
-                rows[extra - self.ext_drift_no:] = ext_drift[:, slice(*chunk_slice)]
-            res[self.cond_no:] = torch.from_numpy(rows).to(dev)
-        return res
-
-    def _pre_ext_drift(self, pnt_cnt, ext_drift=None, set_cond=False):
-        """External drift values checked and shaped (``krige/base.py:390-428``)."""
-        if ext_drift is not None:
-            ext_drift = np.atleast_2d(np.asarray(ext_drift, dtype=np.double))
-            if ext_drift.size == 0:
-                return np.array([])
-            if set_cond:
-                if len(ext_drift.shape) > 2 or ext_drift.shape[1] != pnt_cnt:
-                    raise ValueError("Krige: wrong number of ext. drifts.")
-                return ext_drift
-            ext_shape = np.shape(ext_drift)
-            shape = (self.ext_drift_no, pnt_cnt)
-            if self.drift_no > 1 and ext_shape[0] != self.ext_drift_no:
-                raise ValueError("Krige: wrong number of external drifts.")
-            if np.prod(ext_shape) != np.prod(shape):
-                raise ValueError("Krige: wrong number of ext. drift values.")
-            return np.asarray(ext_drift, dtype=np.double).reshape(shape)
-        if not set_cond and self._cond_ext_drift.size > 0:
-            raise ValueError("Krige: wrong number of ext. drift values.")
-        return np.array([])
-
-    def get_mean(self, post_process=True):
-        """Estimated mean of the detrended field (``krige/base.py:452-487``)."""
-        if not self.has_const_mean and (post_process or self.drift_no > 0):
-            return None
-        res = 0.0
-        mean = 0.0 if self.mean is None else self.mean
-        if self.unbiased:
-            mean_est = np.concatenate((np.full_like(self.cond_val, 0.0), [1]))
-            res = np.einsum("i,ij,j", self._krige_cond, self._krige_mat, mean_est)
-        return res + mean if post_process else res
-
-    def set_condition(self, cond_pos=None, cond_val=None, ext_drift=None, cond_err=None):
-        """(Re)build the kriging system (``krige/base.py:489-586``, without fitting)."""
-        ext_drift = self._cond_ext_drift if (ext_drift is None and cond_pos is None) else ext_drift
-        cond_pos = self._cond_pos if cond_pos is None else cond_pos
-        cond_val = self._cond_val if cond_val is None else cond_val
-        cond_err = self._cond_err if cond_err is None else cond_err
-        cond_err = "nugget" if cond_err is None else cond_err
-        if cond_pos is None or cond_val is None:
-            raise ValueError("Krige.set_condition: missing cond_pos/cond_val.")
-        self._cond_pos, self._cond_val = set_condition(cond_pos, cond_val, self.dim)
-        self.cond_err = cond_err
-        self._cond_ext_drift = self._pre_ext_drift(self.cond_no, ext_drift, set_cond=True)
-        self._krige_pos = self.model.isometrize(self.cond_pos)
-        self._krige_mat = self._get_krige_mat()
-        self.fields = {}
-
-    def set_drift_functions(self, drift_functions=None):
-        """Polynomial order, "linear"/"quadratic" or callables (``krige/base.py:588-624``)."""
-        if drift_functions is None:
-            self._drift_functions = []
-        elif isinstance(drift_functions, (str, int)):
-            self._drift_functions = get_drift_functions(self.dim, drift_functions)
-        else:
-            if callable(drift_functions):
-                drift_functions = [drift_functions]
-            drift_functions = list(drift_functions)
-            for f in drift_functions:
-                if not callable(f):
-                    raise ValueError("Krige: Drift functions not callable")
-            self._drift_functions = drift_functions
-
-    # ----------------------------------------------------- properties ----
     @property
-    def _krige_cond(self):
-        pad_size = self.drift_no + int(self.unbiased)
-        val = self.cond_val - self.cond_trend
-        val = val - self.cond_mean
-        return np.pad(val, (0, pad_size), mode="constant", constant_values=0)
+    def dim(self):
+        return self.model.dim
+
+    @property
+    def name(self):
+        return type(self).__name__
+
+    @property
+    def system(self):
+        """The :class:`KrigingSystem` of the current conditioning data."""
+        return self._system
 
     @property
     def cond_pos(self):
-        return self._cond_pos
+        return self._cond[0]
 
     @property
     def cond_val(self):
-        return self._cond_val
-
-    @property
-    def cond_err(self):
-        if isinstance(self._cond_err, str) and self._cond_err == "nugget":
-            return self.model.nugget
-        return self._cond_err
-
-    @cond_err.setter
-    def cond_err(self, value):
-        if isinstance(value, str) and value == "nugget":
-            self._cond_err = value
-        else:
-            if self.exact:
-                raise ValueError("krige.cond_err: measurement errors can't be given, "
-                                 "when interpolator should be exact.")
-            value = np.asarray(value, dtype=np.double).reshape(-1)
-            if value.size == 1:
-                self._cond_err = value.item()
-            else:
-                if value.size != self.cond_no:
-                    raise ValueError("krige.cond_err: wrong number of measurement errors.")
-                self._cond_err = value
+        return self._cond[1]
 
     @property
     def cond_no(self):
-        return len(self._cond_val)
+        return self._cond[1].size
+
+    @property
+    def cond_err(self):
+        """Measurement-error variances: the model's nugget unless given."""
+        return self.model.nugget if isinstance(self._err_spec, str) else self._err_spec
+
+    @cond_err.setter
+    def cond_err(self, value):
+        if isinstance(value, str):
+            if value != "nugget":
+                raise ValueError(f"Krige: cond_err must be 'nugget' or variances, got {value!r}")
+            self._err_spec = value
+            return
+        if self._exact:
+            raise ValueError("Krige: an exact interpolator takes no measurement errors (cond_err)")
+        arr = np.asarray(value, dtype=np.double).ravel()
+        if arr.size == 1:
+            self._err_spec = float(arr[0])
+        elif arr.size == self.cond_no:
+            self._err_spec = arr
+        else:
+            raise ValueError(f"Krige: cond_err needs 1 or {self.cond_no} values, got {arr.size}")
 
     @property
     def cond_ext_drift(self):
-        return self._cond_ext_drift
+        return self._cond_ext
 
     @property
     def cond_mean(self):
@@ -495,13 +535,13 @@ class Krige:
 
     @pseudo_inv_type.setter
     def pseudo_inv_type(self, val):
-        if val not in P_INV and not callable(val):
-            raise ValueError(f"Krige: pseudo_inv_type not in {sorted(P_INV)}")
+        if not callable(val) and val not in _INVERSES:
+            raise ValueError(f"Krige: pseudo_inv_type must be callable or one of {sorted(_INVERSES)}, got {val!r}")
         self._pseudo_inv_type = val
 
     @property
     def drift_functions(self):
-        return self._drift_functions
+        return self._drifts
 
     @property
     def has_const_mean(self):
@@ -509,19 +549,19 @@ class Krige:
 
     @property
     def krige_size(self):
-        return self.cond_no + self.drift_no + int(self.unbiased)
+        return self.cond_no + self.drift_no + int(self._unbiased)
+
+    @property
+    def int_drift_no(self):
+        return len(self._drifts)
+
+    @property
+    def ext_drift_no(self):
+        return self._cond_ext.shape[0]
 
     @property
     def drift_no(self):
         return self.int_drift_no + self.ext_drift_no
-
-    @property
-    def int_drift_no(self):
-        return len(self.drift_functions)
-
-    @property
-    def ext_drift_no(self):
-        return self.cond_ext_drift.shape[0] if self.cond_ext_drift.size else 0
 
     def structured(self, *args, **kwargs):
         return self(*args, mesh_type="structured", **kwargs)
@@ -533,54 +573,48 @@ class Krige:
         return f"{self.name}(model={self.model.name}, cond_no={self.cond_no})"
 
 
-class Simple(Krige):
-    """Simple kriging: known mean, no unbiasedness row (``krige/methods.py:21-119``)."""
+def _method(name, doc, fixed):
+    """A kriging method: ``Krige`` with some constructor arguments fixed."""
 
-    def __init__(self, model, cond_pos, cond_val, mean=0.0, normalizer=None, trend=None, exact=False,
-                 cond_err="nugget", pseudo_inv=True, pseudo_inv_type="pinv", fit_normalizer=False,
-                 fit_variogram=False):
-        super().__init__(model, cond_pos, cond_val, mean=mean, normalizer=normalizer, trend=trend, unbiased=False,
-                         exact=exact, cond_err=cond_err, pseudo_inv=pseudo_inv, pseudo_inv_type=pseudo_inv_type,
-                         fit_normalizer=fit_normalizer, fit_variogram=fit_variogram)
+    def __init__(self, model, cond_pos, cond_val, *args, **kwargs):
+        names = _METHOD_ARGS[name]
+        if len(args) > len(names):
+            raise TypeError(f"{name}: at most {3 + len(names)} positional arguments")
+        kw = dict(fixed)
+        kw.update(zip(names, args))
+        for key, val in kwargs.items():
+            if key in fixed or key not in names:
+                raise TypeError(f"{name}: unexpected argument '{key}'")
+            if key in kw:
+                raise TypeError(f"{name}: '{key}' given twice")
+            kw[key] = val
+        need = _METHOD_REQUIRED.get(name)
+        if need and need not in kw:
+            raise TypeError(f"{name}: missing argument '{need}'")
+        Krige.__init__(self, model, cond_pos, cond_val, **kw)
 
-
-class Ordinary(Krige):
-    """Ordinary kriging: estimated constant mean (``krige/methods.py:122-214``)."""
-
-    def __init__(self, model, cond_pos, cond_val, normalizer=None, trend=None, exact=False, cond_err="nugget",
-                 pseudo_inv=True, pseudo_inv_type="pinv", fit_normalizer=False, fit_variogram=False):
-        super().__init__(model, cond_pos, cond_val, trend=trend, normalizer=normalizer, exact=exact,
-                         cond_err=cond_err, pseudo_inv=pseudo_inv, pseudo_inv_type=pseudo_inv_type,
-                         fit_normalizer=fit_normalizer, fit_variogram=fit_variogram)
-
-
-class Universal(Krige):
-    """Universal kriging: drift functions (``krige/methods.py:217-324``)."""
-
-    def __init__(self, model, cond_pos, cond_val, drift_functions, normalizer=None, trend=None, exact=False,
-                 cond_err="nugget", pseudo_inv=True, pseudo_inv_type="pinv", fit_normalizer=False,
-                 fit_variogram=False):
-        super().__init__(model, cond_pos, cond_val, drift_functions=drift_functions, normalizer=normalizer,
-                         trend=trend, exact=exact, cond_err=cond_err, pseudo_inv=pseudo_inv,
-                         pseudo_inv_type=pseudo_inv_type, fit_normalizer=fit_normalizer,
-                         fit_variogram=fit_variogram)
+    return type(name, (Krige,), {"__init__": __init__, "__doc__": doc})
 
 
-class ExtDrift(Krige):
-    """External-drift kriging (``krige/methods.py:327-429``)."""
+# positional arguments after (model, cond_pos, cond_val), as in krige/methods.py
+_METHOD_ARGS = {
+    "Simple": ("mean", "normalizer", "trend", "exact", "cond_err", "pseudo_inv", "pseudo_inv_type",
+               "fit_normalizer", "fit_variogram"),
+    "Ordinary": ("normalizer", "trend", "exact", "cond_err", "pseudo_inv", "pseudo_inv_type", "fit_normalizer",
+                 "fit_variogram"),
+    "Universal": ("drift_functions", "normalizer", "trend", "exact", "cond_err", "pseudo_inv", "pseudo_inv_type",
+                  "fit_normalizer", "fit_variogram"),
+    "ExtDrift": ("ext_drift", "normalizer", "trend", "exact", "cond_err", "pseudo_inv", "pseudo_inv_type",
+                 "fit_normalizer", "fit_variogram"),
+    "Detrended": ("trend", "exact", "cond_err", "pseudo_inv", "pseudo_inv_type", "fit_variogram"),
+}
 
-    def __init__(self, model, cond_pos, cond_val, ext_drift, normalizer=None, trend=None, exact=False,
-                 cond_err="nugget", pseudo_inv=True, pseudo_inv_type="pinv", fit_normalizer=False,
-                 fit_variogram=False):
-        super().__init__(model, cond_pos, cond_val, ext_drift=ext_drift, normalizer=normalizer, trend=trend,
-                         exact=exact, cond_err=cond_err, pseudo_inv=pseudo_inv, pseudo_inv_type=pseudo_inv_type,
-                         fit_normalizer=fit_normalizer, fit_variogram=fit_variogram)
+_METHOD_REQUIRED = {"Universal": "drift_functions", "ExtDrift": "ext_drift", "Detrended": "trend"}
 
-
-class Detrended(Krige):
-    """Simple kriging of detrended data (``krige/methods.py:432-519``)."""
-
-    def __init__(self, model, cond_pos, cond_val, trend, exact=False, cond_err="nugget", pseudo_inv=True,
-                 pseudo_inv_type="pinv", fit_variogram=False):
-        super().__init__(model, cond_pos, cond_val, trend=trend, unbiased=False, exact=exact, cond_err=cond_err,
-                         pseudo_inv=pseudo_inv, pseudo_inv_type=pseudo_inv_type, fit_variogram=fit_variogram)
+Simple = _method("Simple", "Simple kriging: known mean, no unbiasedness constraint (``krige/methods.py:21-119``).",
+                 {"unbiased": False})
+Ordinary = _method("Ordinary", "Ordinary kriging: unknown constant mean (``krige/methods.py:122-214``).", {})
+Universal = _method("Universal", "Universal kriging: drift functions (``krige/methods.py:217-324``).", {})
+ExtDrift = _method("ExtDrift", "Kriging with external drifts (``krige/methods.py:327-429``).", {})
+Detrended = _method("Detrended", "Simple kriging of detrended data (``krige/methods.py:432-519``).",
+                    {"unbiased": False})

@@ -74,8 +74,9 @@ class RNG:
     def sample_sphere(self, dim, size=None):
         """Uniform directions on the unit sphere, shape (dim, size).
 
-        Mirrors ``random/rng.py:142-193`` (1D: random sign; 2D: one angle;
-        3D: azimuth plus uniform cosine).
+        Mirrors ``random/rng.py:142-193`` for the dimensions the covariance
+        models here support (1D: random sign; 2D: one angle; 3D: azimuth plus
+        uniform cosine).
         """
         if size is None:
             coord = np.empty((dim, 1), dtype=np.double)
@@ -94,16 +95,7 @@ class RNG:
             coord[1] = np.sqrt(1.0 - ang2**2) * np.sin(ang1)
             coord[2] = ang2
         else:
-            coord = self.random.normal(size=coord.shape)
-            while True:
-                norm = np.linalg.norm(coord, axis=0)
-                zero_norms = np.isclose(norm, 0)
-                if not np.any(zero_norms):
-                    break
-                zero_samples = zero_norms.T.nonzero()
-                new_shape = coord.T[zero_samples].shape
-                coord.T[zero_samples] = self.random.normal(size=new_shape)
-            coord = coord / norm
+            raise ValueError(f"sample_sphere: dimensions 1 to 3 are supported, got {dim}")
         return np.reshape(coord, dim) if size is None else coord
 
     def sample_dist(self, pdf=None, cdf=None, ppf=None, size=None, **kwargs):

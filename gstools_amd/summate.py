@@ -165,7 +165,7 @@ def summate_incompr_device(cov_samples, z_1, z_2, pos, precision="fp64", out=Non
     cov, z1, z2 = _modes(cov_samples, z_1, z_2, dev)
     d, n = cov.shape
     if d < 2:
-        raise ValueError("Only 2D and 3D incompressible fields can be generated.")
+        raise ValueError(f"summate_incompr: incompressible sums need d = 2 or 3, got d = {d}")
     p = _points(pos, d, dev)
     npts = p.shape[1]
     if precision not in PRECISIONS:
@@ -272,7 +272,7 @@ def summate_incompr_grid_device(cov_samples, z_1, z_2, axes, precision="fp64", a
     cov, z1, z2 = _modes(cov_samples, z_1, z_2, dev)
     d, n = cov.shape
     if d < 2:
-        raise ValueError("Only 2D and 3D incompressible fields can be generated.")
+        raise ValueError(f"summate_incompr: incompressible sums need d = 2 or 3, got d = {d}")
     axes = [_as_f64(a, "axes", dev).reshape(-1) for a in axes]
     if len(axes) != d:
         raise ValueError(f"expected {d} axes, got {len(axes)}")

@@ -214,15 +214,14 @@ def ang2dir(angles, dtype=np.double, dim=None):
     return vec
 
 
-def _separate_dirs_test(direction, angles_tol):
-    if direction is None or direction.shape[0] < 2:
+def directions_separated(direction, angles_tol):
+    """Whether every two directions are at least 2 * angles_tol apart, so
+    that a pair can match at most one of them (``variogram.py:568-593``)."""
+    if direction is None or len(direction) < 2:
         return False
-    sep = True
-    for i in range(direction.shape[0] - 1):
-        for j in range(i + 1, direction.shape[0]):
-            s_prod = np.minimum(np.abs(np.dot(direction[i], direction[j])), 1)
-            sep &= np.arccos(s_prod) >= 2 * angles_tol
-    return bool(sep)
+    cosines = np.minimum(np.abs(direction @ direction.T), 1.0)
+    upper = np.triu_indices(len(direction), k=1)
+    return bool(np.all(np.arccos(cosines[upper]) >= 2 * angles_tol))
 
 
 def standard_bins(pos=None, dim=2, latlon=False, mesh_type="unstructured", bin_no=None, max_dist=None,
@@ -249,93 +248,101 @@ def standard_bins(pos=None, dim=2, latlon=False, mesh_type="unstructured", bin_n
 
 
 # ----------------------------------------------------------- front end ----
+def _field_and_points(field, mask, pos, mesh_type):
+    """Stacked fields (F, n) with masked entries as NaN, and their (dim, n)
+    positions; points masked by ``mask`` or in every field are dropped."""
+    data = np.ma.array(field, ndmin=2, dtype=np.double, copy=True)
+    mask_arr = np.asarray(mask, dtype=bool) if mask is not np.ma.nomask else np.zeros((), dtype=bool)
+    if mesh_type == "unstructured":
+        pts, dim = _unstruct_pos_shape(pos, data.shape)
+    else:
+        axes, dim = _struct_pos_shape(pos, data.shape)
+        pts = generate_grid(axes)
+    n = pts.shape[1]
+    data = data.reshape((-1, n))
+    if not (np.ma.is_masked(data) or mask_arr.any()):
+        return np.ma.getdata(data).copy(), pts, dim
+    drop = np.all(np.ma.getmaskarray(data), axis=0)
+    if mask_arr.size > 1:
+        drop |= mask_arr.reshape(n)
+    values = np.ma.getdata(data)[:, ~drop].copy()
+    values[np.ma.getmaskarray(data)[:, ~drop]] = np.nan
+    return values, pts[:, ~drop], dim
+
+
+def _unit_directions(direction, angles, dim):
+    """(D, dim) unit direction vectors from vectors or angles, or None."""
+    if dim < 2 or (direction is None and angles is None):
+        return None
+    if direction is None:
+        vecs = ang2dir(angles=angles, dtype=np.double, dim=dim)
+    else:
+        vecs = np.atleast_2d(np.asarray(direction, dtype=np.double))
+        if vecs.ndim != 2 or vecs.shape[1] != dim:
+            raise ValueError(f"vario_estimate: directions must be vectors of length {dim}, got {direction!r}")
+    lengths = np.linalg.norm(vecs, axis=1)
+    if np.any(np.isclose(lengths, 0)):
+        raise ValueError(f"vario_estimate: directions of zero length: {vecs}")
+    return vecs / lengths[:, None]
+
+
 def vario_estimate(pos, field, bin_edges=None, sampling_size=None, sampling_seed=None, estimator="matheron",
                    latlon=False, direction=None, angles=None, angles_tol=np.pi / 8, bandwidth=None, no_data=np.nan,
                    mask=np.ma.nomask, mesh_type="unstructured", return_counts=False, mean=None, normalizer=None,
                    trend=None, fit_normalizer=False, geo_scale=RADIAN_SCALE, **std_bins):
-    """Empirical variogram (``variogram.py:151-470``); see the module notes for what is not restated."""
+    """Empirical variogram of one or more stacked fields (``variogram.py:151-470``).
+
+    Steps: drop masked points, mark masked / ``no_data`` values NaN (a pair
+    with a NaN value counts in no bin), take directions from vectors or
+    angles, subsample ``sampling_size`` points (``RandomState(sampling_seed)``,
+    without replacement), choose standard bins if none are given, remove
+    trend then mean, and bin all pairs on the GPU (Matheron or Cressie; for
+    lat-lon points great-circle distances in units of ``geo_scale``).
+    Returns bin centers, estimates and, with ``return_counts``, pair counts;
+    with several directions the estimates have one row per direction.
+    """
     if normalizer is not None or fit_normalizer:
         raise NotImplementedError("vario_estimate: normalizers are not part of this build")
-    if bin_edges is not None:
-        bin_edges = np.atleast_1d(np.asarray(bin_edges, dtype=np.double))
-        bin_centers = (bin_edges[:-1] + bin_edges[1:]) / 2.0
-    field = np.ma.array(field, ndmin=2, dtype=np.double, copy=True)
-    masked = np.ma.is_masked(field) or np.any(mask)
-    if masked and np.all(mask):
-        bin_centers = np.empty(0) if bin_edges is None else bin_centers
-        estimates = np.zeros_like(bin_centers)
-        if return_counts:
-            return bin_centers, estimates, np.zeros_like(estimates, dtype=int)
-        return bin_centers, estimates
-    if not masked:
-        field = field.filled()
-    if mesh_type != "unstructured":
-        axes, dim = _struct_pos_shape(pos, field.shape)
-        pos = generate_grid(axes)
-    else:
-        pos, dim = _unstruct_pos_shape(pos, field.shape)
+    kind = str(estimator).lower()
+    if kind not in ("matheron", "cressie"):
+        raise ValueError(f"vario_estimate: estimator must be 'matheron' or 'cressie', got {estimator!r}")
+    edges = None if bin_edges is None else np.atleast_1d(np.asarray(bin_edges, dtype=np.double))
+    fully_masked = mask is not np.ma.nomask and np.size(mask) > 0 and np.all(mask)
+    if fully_masked or (np.ma.is_masked(field) and np.all(np.ma.getmaskarray(field))):
+        centers = np.empty(0) if edges is None else (edges[:-1] + edges[1:]) / 2.0
+        zeros = np.zeros_like(centers)
+        return (centers, zeros, np.zeros_like(zeros, dtype=int)) if return_counts else (centers, zeros)
+    values, pts, dim = _field_and_points(field, mask, pos, mesh_type)
     if latlon and dim != 2:
-        raise ValueError("Variogram: given field needs to be 2D for lat-lon.")
-    pnt_cnt = len(pos[0])
-    field = field.reshape((-1, pnt_cnt))
-    if masked:
-        if np.size(mask) > 1:
-            select = np.invert(np.logical_or(np.reshape(mask, pnt_cnt), np.all(field.mask, axis=0)))
-        else:
-            select = np.invert(np.all(field.mask, axis=0))
-        pos = pos[:, select]
-        field.fill_value = np.nan
-        field = field[:, select].filled()
-        pnt_cnt = len(pos[0])
+        raise ValueError("vario_estimate: lat-lon fields need two coordinates (latitude, longitude)")
     if not np.isnan(no_data):
-        field[np.isclose(field, float(no_data))] = np.nan
-    dir_no = 0
-    if direction is not None and dim > 1:
-        direction = np.atleast_2d(np.asarray(direction, dtype=np.double))
-        if len(direction.shape) > 2 or direction.shape[1] != dim:
-            raise ValueError(f"Can't interpret directions: {direction}")
-        dir_no = direction.shape[0]
-    if angles is not None and direction is None and dim > 1:
-        direction = ang2dir(angles=angles, dtype=np.double, dim=dim)
-        dir_no = direction.shape[0]
-    if dir_no > 0:
-        if latlon:
-            raise ValueError("Directional variogram not allowed for lat-lon.")
-        norms = np.linalg.norm(direction, axis=1)
-        if np.any(np.isclose(norms, 0)):
-            raise ValueError(f"Zero length directions: {direction}")
-        direction = np.divide(direction, norms[:, np.newaxis])
-        bandwidth = float(bandwidth) if bandwidth is not None else -1.0
-        angles_tol = float(angles_tol)
-    if sampling_size is not None and sampling_size < pnt_cnt:
-        idx = np.random.RandomState(sampling_seed).choice(np.arange(pnt_cnt), sampling_size, replace=False)
-        field = field[:, idx]
-        pos = pos[:, idx]
-    if bin_edges is None:
-        bin_edges = standard_bins(pos, dim, latlon, geo_scale=geo_scale, **std_bins)
-        bin_centers = (bin_edges[:-1] + bin_edges[1:]) / 2.0
-    if latlon:
-        # the kernel takes radians (out of place: a caller's edge array is left as it was)
-        bin_edges = bin_edges / geo_scale
-    # trend, then mean (a normalizer would act between them: variogram.py:433-440,
-    # normalizer/tools.py:162-186)
-    for i in range(field.shape[0]):
-        field[i] -= eval_func(trend, pos, dim, broadcast=True)
-        field[i] -= eval_func(mean, pos, dim, broadcast=True)
-    code = "c" if estimator.lower() == "cressie" else "m"
-    if estimator.lower() not in ("matheron", "cressie"):
-        raise ValueError(f"Unknown variogram estimator function: {estimator}")
-    if dir_no == 0:
-        estimates, counts = unstructured(field, bin_edges, pos, estimator_type=code,
-                                         distance_type="h" if latlon else "e")
+        values[np.isclose(values, float(no_data))] = np.nan
+    dirs = _unit_directions(direction, angles, dim)
+    if dirs is not None and latlon:
+        raise ValueError("vario_estimate: directional variograms are not defined for lat-lon data")
+    n = pts.shape[1]
+    if sampling_size is not None and sampling_size < n:
+        keep = np.random.RandomState(sampling_seed).choice(n, sampling_size, replace=False)
+        values, pts = values[:, keep], pts[:, keep]
+    if edges is None:
+        edges = standard_bins(pts, dim, latlon, geo_scale=geo_scale, **std_bins)
+    centers = (edges[:-1] + edges[1:]) / 2.0
+    # trend first, then mean (a normalizer would act in between:
+    # variogram.py:433-440, normalizer/tools.py:162-186)
+    values -= eval_func(trend, pts, dim, broadcast=True)
+    values -= eval_func(mean, pts, dim, broadcast=True)
+    code = "c" if kind == "cressie" else "m"
+    if dirs is None:
+        # the kernel takes great-circle edges in radians (divided out of place)
+        kernel_edges = edges / geo_scale if latlon else edges
+        est, cnt = unstructured(values, kernel_edges, pts, estimator_type=code, distance_type="h" if latlon else "e")
     else:
-        estimates, counts = directional(field, bin_edges, pos, direction, angles_tol, bandwidth,
-                                        separate_dirs=_separate_dirs_test(direction, angles_tol),
-                                        estimator_type=code)
-        if dir_no == 1:
-            estimates, counts = estimates[0], counts[0]
-    est_out = (estimates, counts)
-    return (bin_centers,) + est_out[: 2 if return_counts else 1]
+        est, cnt = directional(values, edges, pts, dirs, float(angles_tol),
+                               -1.0 if bandwidth is None else float(bandwidth),
+                               separate_dirs=directions_separated(dirs, angles_tol), estimator_type=code)
+        if len(dirs) == 1:
+            est, cnt = est[0], cnt[0]
+    return (centers, est, cnt) if return_counts else (centers, est)
 
 
 def vario_estimate_axis(field, direction="x", estimator="matheron", no_data=np.nan):

@@ -82,9 +82,11 @@ def test_kriging_system_sizes():
     krig = gs.krige.Ordinary(model, cond_pos, cond_val)
     pos = rs.uniform(0, 100, size=(2, 20000))
     field, var = krig(pos)
-    vecs = krig._get_krige_vecs(model.isometrize(pos))
-    f_ref, e_ref = oracle.krige_c(krig._krige_mat, vecs, krig._krige_cond)
-    bf, be = _bounds(krig._krige_mat, vecs, krig._krige_cond)
+    system = krig.system
+    vecs = system.rhs_host(model.isometrize(pos), krig._constraint_rows(pos, np.empty((0, pos.shape[1]))),
+                           krig.exact)
+    f_ref, e_ref = oracle.krige_c(system.inverse, vecs, krig._residual())
+    bf, be = _bounds(system.inverse, vecs, krig._residual())
     assert np.all(np.abs(field - f_ref) <= 1e-13 * bf)
     assert np.all(np.abs((model.sill - var) - e_ref) <= 1e-13 * be + 1e-12 * model.sill)
 
