@@ -70,27 +70,61 @@ def time_kernel(fn, steps, warmup, torch):
     return wall / steps, float(np.mean(ev)) / 1e3
 
 
-def cpu_baseline(rm, iso, target_s=10.0):
+def host_cpu():
+    """Model name and clock of the host CPU (what ``lscpu`` reports), from /proc/cpuinfo."""
+    model, mhz = None, []
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                key, _, val = line.partition(":")
+                key = key.strip()
+                if key == "model name" and model is None:
+                    model = val.strip()
+                elif key == "cpu MHz":
+                    mhz.append(float(val))
+    except OSError:
+        pass
+    return {"model": model, "mhz_mean": float(np.mean(mhz)) if mhz else None,
+            "logical_cpus_on_host": os.cpu_count()}
+
+
+def _oracle_rate(cov, z1, z2, iso, nthreads, target_s):
+    """mode.points/s of the C oracle on the first points of ``iso`` that take ~target_s."""
     import oracle
 
+    n_modes = cov.shape[1]
+    probe = 20000 if nthreads > 1 else 2000
+    t0 = time.perf_counter()
+    oracle.summate_c(cov, z1, z2, np.ascontiguousarray(iso[:, :probe]), nthreads=nthreads)
+    dt = time.perf_counter() - t0
+    rate = probe * n_modes / dt
+    npts = int(min(iso.shape[1], max(probe, rate * target_s / n_modes)))
+    t0 = time.perf_counter()
+    oracle.summate_c(cov, z1, z2, np.ascontiguousarray(iso[:, :npts]), nthreads=nthreads)
+    dt = time.perf_counter() - t0
+    return npts * n_modes / dt, npts, dt
+
+
+def cpu_baseline(rm, iso, target_s=10.0, single_core_s=5.0):
+    """The oracle's C/OpenMP restatement of the reference loop on the host:
+    all cores of this process's affinity mask, then one core."""
     cores = len(os.sched_getaffinity(0))
     cov, z1, z2 = rm._cov_sample, rm._z_1, rm._z_2
-    probe = 20000
-    t0 = time.perf_counter()
-    oracle.summate_c(cov, z1, z2, iso[:, :probe], nthreads=cores)
-    dt = time.perf_counter() - t0
-    rate = probe * N_HEADLINE / dt
-    npts = int(min(P_HEADLINE, max(probe, rate * target_s / N_HEADLINE)))
-    t0 = time.perf_counter()
-    oracle.summate_c(cov, z1, z2, np.ascontiguousarray(iso[:, :npts]), nthreads=cores)
-    dt = time.perf_counter() - t0
+    rate, npts, dt = _oracle_rate(cov, z1, z2, iso, cores, target_s)
+    rate1, npts1, dt1 = _oracle_rate(cov, z1, z2, iso, 1, single_core_s)
+    cpu = host_cpu()
     return {
-        "value": npts * N_HEADLINE / dt,
+        "value": rate,
         "unit": "mode·points/s",
         "cores": cores,
         "kind": "port",
         "sample": f"first {npts} of {P_HEADLINE} points x {N_HEADLINE} modes, FP64, "
                   f"C/OpenMP restatement of the reference loop, {cores} threads, {dt:.1f} s",
+        "single_core_value": rate1,
+        "single_core_sample": f"first {npts1} points x {N_HEADLINE} modes, 1 thread, {dt1:.1f} s",
+        "cpu_model": cpu["model"],
+        "cpu_mhz_mean": cpu["mhz_mean"],
+        "logical_cpus_on_host": cpu["logical_cpus_on_host"],
     }
 
 
@@ -634,7 +668,7 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=100)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--extras", type=int, default=1, help="also run the other configurations")
     ap.add_argument("--only", choices=["fp32", "fp64", "mfma", "incompr", "fourier", "krige", "variogram", "adjoint",
@@ -659,7 +693,7 @@ def main():
         sys.exit(1)
     dev = torch.device("cuda", 0)
     k_ext = max(3, args.steps // 4)
-    w_ext = max(1, args.warmup)
+    w_ext = max(1, min(args.warmup, 3))  # the extra sections run long kernels
     if args.only in ("mfma", "incompr", "fourier", "krige", "variogram", "adjoint", "batch", "copy"):
         fn = {"mfma": lambda: bench_mfma_grid(torch, k_ext, w_ext),
               "incompr": lambda: bench_incompr_grid(torch, k_ext, w_ext),
