@@ -584,26 +584,37 @@ __global__ void k_vario_normalize(int64_t n, int cressie, double* __restrict__ g
     }
 }
 
-// structured lags: block (lag k = blockIdx.y, column tile blockIdx.x)
-__global__ __launch_bounds__(256) void k_vario_lags(int64_t n0, int64_t n1, const double* __restrict__ f,
+// structured lags: block b takes lag k = b / tiles and column tile b % tiles.
+// A tile holds C = 2^cols_log2 columns (256, or the next power of two >= n1
+// for narrow fields); the block's 256 / C row phases split the rows i, so a
+// 1D series (n1 = 1) spreads each lag over all 256 lanes.  Lanes of one row
+// phase read consecutive columns (coalesced), and for C = 1 consecutive rows.
+// The block's partial sum is reduced in a fixed order.
+__global__ __launch_bounds__(256) void k_vario_lags(int64_t n0, int64_t n1, int64_t tiles, int cols_log2,
+                                                    const double* __restrict__ f,
                                                     const unsigned char* __restrict__ mask, int cressie,
                                                     double* __restrict__ part_sum,
                                                     unsigned long long* __restrict__ part_cnt) {
     __shared__ double rs[256];
     __shared__ unsigned long long rc[256];
-    const int64_t k = blockIdx.y;
-    const int64_t j = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    const int64_t b = blockIdx.x;
+    const int64_t k = b / tiles;
+    const int64_t tile = b - k * tiles;
+    const int c = (int)threadIdx.x & ((1 << cols_log2) - 1);
+    const int r = (int)threadIdx.x >> cols_log2;
+    const int phases = 256 >> cols_log2;
+    const int64_t j = (tile << cols_log2) + c;
     double s = 0.0;
-    unsigned long long c = 0;
+    unsigned long long cnt = 0;
     if (k > 0 && j < n1) {
-        for (int64_t i = 0; i + k < n0; ++i) {
+        for (int64_t i = r; i + k < n0; i += phases) {
             if (mask && (mask[(i + k) * n1 + j] || mask[i * n1 + j])) continue;
-            ++c;
+            ++cnt;
             s += vario_est(cressie, f[(i + k) * n1 + j] - f[i * n1 + j]);
         }
     }
     rs[threadIdx.x] = s;
-    rc[threadIdx.x] = c;
+    rc[threadIdx.x] = cnt;
     __syncthreads();
     for (int w = 128; w > 0; w >>= 1) {
         if ((int)threadIdx.x < w) {
@@ -613,8 +624,8 @@ __global__ __launch_bounds__(256) void k_vario_lags(int64_t n0, int64_t n1, cons
         __syncthreads();
     }
     if (threadIdx.x == 0) {
-        part_sum[k * gridDim.x + blockIdx.x] = rs[0];
-        part_cnt[k * gridDim.x + blockIdx.x] = rc[0];
+        part_sum[k * tiles + tile] = rs[0];
+        part_cnt[k * tiles + tile] = rc[0];
     }
 }
 
@@ -820,13 +831,16 @@ int gsx_vario_structured_f64(int64_t n0, int64_t n1, const double* f, const uint
     if (n0 == 0) return GSX_OK;
     if (!gamma || !counts || (n1 > 0 && !f)) return GSX_E_INVALID;
     const int64_t tiles = gsx::ceil_div(n1, (int64_t)256) > 0 ? gsx::ceil_div(n1, (int64_t)256) : 1;
-    if (n0 > 65535 || tiles > 0x7fffffffLL) return GSX_E_INVALID;
+    // columns per tile: 256, or the next power of two >= n1 (then one tile)
+    int cols_log2 = 0;
+    while (cols_log2 < 8 && ((int64_t)1 << cols_log2) < n1) ++cols_log2;
+    if (n0 > 0x7fffffffLL / tiles) return GSX_E_UNSUPPORTED;  // one block per (lag, tile)
     if (!workspace || workspace_bytes < n0 * tiles * 16) return GSX_E_WORKSPACE;
     hipStream_t s = gsx::as_stream(stream);
     double* part_sum = static_cast<double*>(workspace);
     unsigned long long* part_cnt = reinterpret_cast<unsigned long long*>(part_sum + n0 * tiles);
-    hipLaunchKernelGGL(gsx::k_vario_lags, dim3((unsigned)tiles, (unsigned)n0), dim3(256), 0, s, n0, n1, f, mask,
-                       estimator, part_sum, part_cnt);
+    hipLaunchKernelGGL(gsx::k_vario_lags, dim3((unsigned)(n0 * tiles)), dim3(256), 0, s, n0, n1, tiles, cols_log2,
+                       f, mask, estimator, part_sum, part_cnt);
     int st = gsx::launch_status();
     if (st != GSX_OK) return st;
     long long* cnt = reinterpret_cast<long long*>(counts);

@@ -313,7 +313,8 @@ int gsx_krige_f64(int64_t K, int64_t P, const double* krig_mat,
  * direction line; separate_dirs != 0 gives each pair to the first matching
  * direction; gamma, counts are (D, B).  Structured: lags 0..n0-1 along the
  * first axis of an (n0, n1) array; mask (n0, n1) bytes, non-zero = masked,
- * or NULL; gamma, counts (n0).  Counts are exact; sums are reduced in a
+ * or NULL; gamma, counts (n0); any n0 with n0 * ceil(n1 / 256) < 2^31
+ * (GSX_E_UNSUPPORTED beyond).  Counts are exact; sums are reduced in a
  * fixed order (bitwise reproducible).  Workspace bytes:
  * gsx_vario_workspace(kind, P, dim, B, D) for kind 0 (unstructured) or 1
  * (directional), gsx_vario_workspace(2, n0, n1, 0, 1) for structured.
@@ -414,7 +415,10 @@ typedef struct gsx_batch_plan gsx_batch_plan;
 int gsx_batch_plan_create(const gsx_batch_item* items, int64_t count,
                           int64_t tile_points, int64_t mode_chunk,
                           gsx_batch_plan** plan);
-/* One persistent launch over every tile of every item. */
+/* One persistent launch over every tile of every item.  A plan has one
+ * device tile counter, reset by every launch: use a plan on one stream at a
+ * time (an eager launch must not overlap a replay of the plan's graph, nor a
+ * launch on another stream); concurrent work needs one plan per stream. */
 int gsx_batch_plan_launch(gsx_batch_plan* plan, void* stream);
 /* Capture the launch into a HIP graph once, then replay it. */
 int gsx_batch_plan_launch_graph(gsx_batch_plan* plan, void* stream);

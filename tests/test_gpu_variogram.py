@@ -148,7 +148,8 @@ def test_directional_slot_chunks():
         _close(g, g_ref)
 
 
-@pytest.mark.parametrize("n0,n1,masked,est", [(10, 1, False, "m"), (300, 5000, False, "c"), (257, 1031, True, "m")])
+@pytest.mark.parametrize("n0,n1,masked,est", [(10, 1, False, "m"), (300, 5000, False, "c"), (257, 1031, True, "m"),
+                                             (1000, 3, True, "c"), (777, 1, True, "m"), (64, 200, False, "m")])
 def test_structured_matches_oracle(n0, n1, masked, est):
     _say(f"vario structured {n0}x{n1} masked={masked}")
     rs = np.random.RandomState(n0)
@@ -159,6 +160,34 @@ def test_structured_matches_oracle(n0, n1, masked, est):
     _close(g, g_ref)
     g2, c2 = v._lags(f, mask, est)
     assert np.array_equal(c2.cpu().numpy(), c_ref)
+
+
+@pytest.mark.parametrize("n0,n1,masked", [(100_000, 1, False), (70_001, 2, True)])
+def test_structured_long_axis(n0, n1, masked):
+    """More lags than a grid dimension's 65535 blocks, on a narrow array:
+    counts by closed form (n0 - k unmasked pairs per column, or the unmasked
+    pairs of the mask), sums against numpy on a spread of lags."""
+    _say(f"vario structured long axis {n0}x{n1} masked={masked}")
+    rs = np.random.RandomState(n0)
+    f = rs.normal(size=(n0, n1))
+    mask = rs.uniform(size=(n0, n1)) < 0.05 if masked else None
+    gamma, counts = v._lags(f, mask, "m")
+    gamma, counts = gamma.cpu().numpy(), counts.cpu().numpy()
+    lags = np.unique(np.concatenate([np.arange(0, 40), rs.randint(1, n0, 60), [65534, 65535, 65536, 65537],
+                                     [n0 - 2, n0 - 1]]))
+    lags = lags[lags < n0]
+    ok = np.ones((n0, n1), dtype=bool) if mask is None else ~mask
+    for k in lags:
+        if k == 0:
+            assert counts[0] == 0
+            continue
+        both = ok[k:] & ok[:-k]
+        diff = (f[k:] - f[:-k])[both]
+        assert counts[k] == both.sum(), k
+        ref = np.sum(diff**2) / (2.0 * max(both.sum(), 1))
+        assert np.isclose(gamma[k], ref, rtol=1e-11, atol=1e-300), (k, gamma[k], ref)
+    if mask is None:
+        assert np.array_equal(counts[1:], (n0 - np.arange(1, n0)) * n1)
 
 
 @pytest.mark.parametrize("nf,est", [(1, "m"), (3, "c"), (9, "m")])
