@@ -40,9 +40,9 @@ class RandMethSum(torch.autograd.Function):
         g = gu.contiguous().to(torch.float64 if ctx.precision == "fp64" else torch.float32)
         gz1, gz2, gk = summate_adjoint_device(cov.detach(), z1.detach(), z2.detach(), pos.detach(), g,
                                               precision=ctx.precision)
-        return (gz1.to(z1.dtype) if ctx.needs_input_grad[0] else None,
-                gz2.to(z2.dtype) if ctx.needs_input_grad[1] else None,
-                gk.to(cov.dtype) if ctx.needs_input_grad[2] else None,
+        return (gz1.to(z1.dtype).reshape(z1.shape) if ctx.needs_input_grad[0] else None,
+                gz2.to(z2.dtype).reshape(z2.shape) if ctx.needs_input_grad[1] else None,
+                gk.to(cov.dtype).reshape(cov.shape) if ctx.needs_input_grad[2] else None,
                 None, None)
 
 

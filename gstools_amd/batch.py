@@ -71,7 +71,12 @@ class BatchItem:
 
 
 class BatchPlan:
-    """A persistent-launch plan over ``items`` (kept alive by the plan)."""
+    """A persistent-launch plan over ``items`` (kept alive by the plan).
+
+    The plan's work queue is one device counter that every launch resets:
+    launch a plan on one stream at a time (never an eager launch overlapping
+    a replay of its graph); build one plan per stream for concurrent work.
+    """
 
     def __init__(self, items, tile_points=0, mode_chunk=0):
         self.items = list(items)

@@ -16,6 +16,7 @@ HIP device or without ``libgsx.so`` every call raises ``RuntimeError``.
 """
 
 import ctypes
+import os
 
 import numpy as np
 import torch
@@ -411,6 +412,11 @@ def summate(cov_samples, z_1, z_2, pos, num_threads=None):
 # the numpy Fourier seam takes the per-axis contraction for meshgrids this large
 TENSOR_MIN_POINTS = 4096
 TENSOR_MIN_MODES = 64
+# The numpy seam ``summate_fourier`` runs meshgrids of modes and points as
+# per-axis FP64 contractions (within ~1e-14 of the per-point sum, not bit for
+# bit).  False, or GSX_FOURIER_TENSOR=0 in the environment, keeps every call
+# on the per-point sum, whose phases follow the reference loop bit for bit.
+FOURIER_TENSOR = os.environ.get("GSX_FOURIER_TENSOR", "1") != "0"
 
 
 def tensor_grid_axes(pts):
@@ -447,18 +453,23 @@ def tensor_grid_axes(pts):
     return axes
 
 
-def summate_fourier(spectrum_factor, modes, z_1, z_2, pos, num_threads=None):
+def summate_fourier(spectrum_factor, modes, z_1, z_2, pos, num_threads=None, tensor=None):
     """Drop-in for the backend ``summate_fourier``: FP64 sum on the GPU -> numpy (P,).
 
     When ``modes`` and ``pos`` are both flattened 'ij' meshgrids in 2D or 3D (the
     Fourier generator on a structured mesh) of at least ``TENSOR_MIN_POINTS`` points
     and ``TENSOR_MIN_MODES`` modes, the sum runs as one FP64 GEMM contraction per
-    axis (``summate_fourier_tensor_device``); otherwise as the per-point sum."""
+    axis (``summate_fourier_tensor_device``; within ~1e-14 of the per-point sum, not
+    bit for bit); otherwise as the per-point sum.  ``tensor=False`` (or
+    ``GSX_FOURIER_TENSOR=0`` in the environment, read at import) keeps every call on
+    the per-point sum, whose phases follow the reference loop bit for bit."""
+    use_tensor = FOURIER_TENSOR if tensor is None else bool(tensor)
     _check_numpy(modes, z_1, z_2, pos)
     if isinstance(spectrum_factor, np.ndarray) and spectrum_factor.dtype != np.float64:
         raise ValueError(f"Buffer dtype mismatch for spectrum_factor: expected 'double' but got "
                          f"'{spectrum_factor.dtype}'")
-    if isinstance(modes, np.ndarray) and isinstance(pos, np.ndarray) and modes.ndim == 2 and pos.ndim == 2 \
+    if use_tensor and isinstance(modes, np.ndarray) and isinstance(pos, np.ndarray) and modes.ndim == 2 \
+            and pos.ndim == 2 \
             and 2 <= modes.shape[0] == pos.shape[0] <= 3 and pos.shape[1] >= TENSOR_MIN_POINTS \
             and modes.shape[1] >= TENSOR_MIN_MODES and np.size(spectrum_factor) == modes.shape[1] \
             and np.size(z_1) == modes.shape[1] and np.size(z_2) == modes.shape[1]:

@@ -163,6 +163,17 @@ def test_numpy_seam_runs_meshgrids_as_contractions():
     scattered = np.random.RandomState(4).uniform(0, 50, size=(2, 300))
     per_point = summate_fourier_device(*args, scattered, "fp64").cpu().numpy()
     assert np.array_equal(summate_fourier(*args, scattered), per_point)
+    # opting out keeps the meshgrid on the per-point sum: bitwise its result
+    direct = summate_fourier_device(*args, pos, "fp64").cpu().numpy()
+    assert np.array_equal(summate_fourier(*args, pos, tensor=False), direct)
+    import importlib
+    smod = importlib.import_module("gstools_amd.summate")
+    saved = smod.FOURIER_TENSOR
+    try:
+        smod.FOURIER_TENSOR = False
+        assert np.array_equal(summate_fourier(*args, pos), direct)
+    finally:
+        smod.FOURIER_TENSOR = saved
 
 
 def test_tensor_path_errors_and_empty():
