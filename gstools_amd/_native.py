@@ -31,6 +31,16 @@ _f64 = ctypes.c_double
 _i64p = ctypes.POINTER(ctypes.c_int64)
 
 
+class StretchModel(ctypes.Structure):
+    """Mirror of ``gsx_stretch_model``."""
+
+    _fields_ = [("kind", ctypes.c_int32), ("dim", ctypes.c_int32), ("len_rescaled", ctypes.c_double),
+                ("c", ctypes.c_double * 6)]
+
+
+STRETCH_GAUSSIAN, STRETCH_EXPONENTIAL, STRETCH_MATERN = 0, 1, 2
+
+
 class BatchItem(ctypes.Structure):
     """Mirror of ``gsx_batch_item``."""
 
@@ -98,6 +108,9 @@ EXPORTS = {
     "gsx_stretch_draws": (_i32, [_i64, _vp, _vp, _i64, _i32, _vp, _vp, _vp, _vp, _i32]),
     "gsx_stretch_propose": (_i32, [_i64, _i32, _vp, _i64, _i64, _i64, _i32, _f64, _vp, _vp, _vp, _vp, _vp, _vp]),
     "gsx_stretch_accept": (_i32, [_i64, _i32, _vp, _i64, _i64, _i64, _i32, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i64]),
+    "gsx_stretch_run_device": (_i32, [_i64, _i32, _vp, _i64, _vp, _i64, _f64, ctypes.POINTER(StretchModel), _vp,
+                                      _vp, _vp, _vp, _vp]),
+    "gsx_stretch_pick_device": (_i32, [_i64, _vp, _vp, _i64, _i32, _vp, _vp, _vp]),
     "gsx_batch_plan_create": (_i32, [ctypes.POINTER(BatchItem), _i64, _i64, _i64, ctypes.POINTER(_vp)]),
     "gsx_batch_plan_launch": (_i32, [_vp, _vp]),
     "gsx_batch_plan_launch_graph": (_i32, [_vp, _vp]),

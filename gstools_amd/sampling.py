@@ -22,16 +22,27 @@ seed=s)`` holds (``_cov_sample``, ``_z_1``, ``_z_2``), bit for bit:
   with the log of the acceptance uniforms taken by numpy as there;
 * the per-realization draws that are not MCMC (the master seeds, normals,
   directions, initial walkers, the final ``choice``) stay on numpy.
+
+For the closed-form models (Gaussian, Exponential, Matern with nu <= 20)
+the MCMC runs on the GPU instead (``device=True``, the default when a HIP
+device is present): ``gsx_stretch_run_device`` advances every chain in one
+launch, one wavefront per chain, drawing from the chain's own MT19937
+streams in the same order and evaluating the closed-form log-pdf on the
+device; the final ``choice`` indices come from numpy and are gathered on the
+device (``gsx_stretch_pick_device``).  Accept decisions compare the same
+quantities up to the last bits of exp / log, so the modes come out bit for
+bit equal to the host sampler's unless a test lands within an ulp.
 """
 
 import ctypes
 
 import numpy as np
+import torch
 
 from . import _native
 from .rng import RNG
 
-__all__ = ["sample_ln_pdf_batch", "sample_modes_batch"]
+__all__ = ["sample_ln_pdf_batch", "sample_modes_batch", "device_model"]
 
 _MT_WORDS = 625  # 624 key words + position
 
@@ -138,14 +149,88 @@ def _lockstep(ln_pdf, coords, log_prob, steps, states, a=2.0, record=False, bloc
     return coords, log_prob, chain
 
 
+def device_model(model):
+    """``gsx_stretch_model`` for this package's Gaussian, Exponential and Matern
+    (nu <= 20) models, with the scalar factors computed exactly as their
+    ``spectral_density`` computes them; None for any other model."""
+    import scipy.special as sps
+
+    from . import covmodel as cm
+
+    kind = type(model)
+    if kind not in (cm.Gaussian, cm.Exponential, cm.Matern) or not 1 <= model.dim <= 3:
+        return None
+    md = _native.StretchModel()
+    md.dim, md.len_rescaled = model.dim, float(model.len_rescaled)
+    if kind is cm.Gaussian:
+        md.kind = _native.STRETCH_GAUSSIAN
+        md.c[0] = (model.len_rescaled / 2.0 / np.sqrt(np.pi)) ** model.dim
+    elif kind is cm.Exponential:
+        md.kind = _native.STRETCH_EXPONENTIAL
+        md.c[0] = model.len_rescaled**model.dim * sps.gamma((model.dim + 1) / 2.0)
+    else:
+        if model.nu > 20.0:
+            return None
+        md.kind = _native.STRETCH_MATERN
+        md.c[0] = (model.len_rescaled / np.sqrt(np.pi)) ** model.dim
+        md.c[1] = -(model.nu + model.dim / 2.0)
+        md.c[2] = model.nu
+        md.c[3] = sps.loggamma(model.nu + model.dim / 2.0)
+        md.c[4] = sps.loggamma(model.nu)
+        md.c[5] = model.dim * np.log(np.sqrt(model.nu))
+    return md
+
+
+_STRETCH_ERRORS = {1: "At least one parameter value was infinite or NaN",
+                   2: "Probability function returned NaN"}
+
+
+def _chains_device(md, init, burn, prod, burn_in, prod_steps, sizes, pickers, a=2.0):
+    """Run every chain on the GPU and return each chain's ``choice`` of samples."""
+    dev = torch.device("cuda", torch.cuda.current_device())
+    nr, nw = init.shape
+    max_steps = max(int(prod_steps.max(initial=0)), 1)
+    states = torch.from_numpy(np.ascontiguousarray(np.stack([burn, prod], axis=1)).view(np.int32)).to(dev)
+    steps_t = torch.from_numpy(np.ascontiguousarray(prod_steps, dtype=np.int64)).to(dev)
+    init_t = torch.from_numpy(np.ascontiguousarray(init, dtype=np.double)).to(dev)
+    chain = torch.empty((nr, max_steps, nw), dtype=torch.float64, device=dev)
+    status = torch.zeros(1, dtype=torch.int32, device=dev)
+    stream = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    lib = _native.lib()
+    _native.check(lib.gsx_stretch_run_device(nr, nw, _tp(states), int(burn_in), _tp(steps_t), max_steps, float(a),
+                                             ctypes.byref(md), _tp(init_t), _tp(chain), None, _tp(status), stream),
+                  "stretch_run_device")
+    # the final choice: indices from each chain's own generator, gathered on the device
+    idx = [pickers[r].choice(int(prod_steps[r]) * nw, sizes[r]) for r in range(nr)]
+    counts = [len(i) for i in idx]
+    owner = np.repeat(np.arange(nr, dtype=np.int64), counts)
+    flat = np.concatenate(idx).astype(np.int64)
+    out = torch.empty(max(flat.size, 1), dtype=torch.float64, device=dev)
+    owner_t, flat_t = torch.from_numpy(owner).to(dev), torch.from_numpy(flat).to(dev)
+    _native.check(lib.gsx_stretch_pick_device(flat.size, _tp(owner_t), _tp(flat_t), max_steps, nw, _tp(chain),
+                                              _tp(out), stream), "stretch_pick_device")
+    code = int(status.item())
+    if code:
+        raise ValueError(_STRETCH_ERRORS.get(code, f"device sampler status {code}"))
+    rad = out[:flat.size].cpu().numpy()
+    return np.split(rad, np.cumsum(counts)[:-1])
+
+
+def _tp(t):
+    return ctypes.c_void_p(t.data_ptr())
+
+
 def sample_ln_pdf_batch(rngs, ln_pdf, sizes, sample_around=1.0, nwalkers=50, burn_in=20, oversampling_factor=10,
-                        block=64, nthreads=1):
+                        block=64, nthreads=1, device_model=None):
+
     """``RNG.sample_ln_pdf`` for many generators sharing one ``ln_pdf``.
 
     ``rngs`` are :class:`gstools_amd.rng.RNG` objects, each consumed exactly
     as ``rng.sample_ln_pdf(ln_pdf, size, sample_around, ...)`` would consume
     it (four master draws: initial walkers, burn-in stream, production stream,
-    final choice).  Returns a list of radius arrays.
+    final choice).  Returns a list of radius arrays.  With ``device_model``
+    (a ``gsx_stretch_model`` of the same distribution as ``ln_pdf``, see
+    :func:`device_model`) and at most 64 walkers the chains run on the GPU.
     """
     nr = len(rngs)
     if nr == 0:
@@ -167,6 +252,8 @@ def sample_ln_pdf_batch(rngs, ln_pdf, sizes, sample_around=1.0, nwalkers=50, bur
     for r in range(nr):
         if np.all(init[r] == init[r, 0]):
             raise ValueError("Initial state has walkers that are not linearly independent")
+    if device_model is not None and nwalkers <= 64:
+        return _chains_device(device_model, init, burn, prod, burn_in, prod_steps, sizes, pickers)
     log_prob = _log_prob(ln_pdf, init)
     if np.any(np.isnan(log_prob)):
         raise ValueError("The initial log_prob was NaN")
@@ -177,11 +264,13 @@ def sample_ln_pdf_batch(rngs, ln_pdf, sizes, sample_around=1.0, nwalkers=50, bur
     return [pickers[r].choice(chain[r, :prod_steps[r]].reshape(-1), sizes[r]) for r in range(nr)]
 
 
-def sample_modes_batch(model, mode_nos, seeds, sampling="auto", nthreads=1):
+def sample_modes_batch(model, mode_nos, seeds, sampling="auto", nthreads=1, device=None):
     """``(cov_sample, z_1, z_2)`` of ``RandMeth(model, mode_no=n, seed=s)`` for every pair.
 
     Same draws, in the same order, as ``RandMeth.reset_seed``; the MCMC runs
-    of all pairs advance together (:func:`sample_ln_pdf_batch`).
+    of all pairs advance together (:func:`sample_ln_pdf_batch`), on the GPU
+    when ``device`` is true (default: when a HIP device is present and the
+    model has a device log-pdf, :func:`device_model`).
     """
     mode_nos = [int(n) for n in mode_nos]
     seeds = list(seeds)
@@ -210,8 +299,15 @@ def sample_modes_batch(model, mode_nos, seeds, sampling="auto", nthreads=1):
             out.append([sph, z1, z2])
             pending.append((len(out) - 1, rng, n))
     if pending:
+        md = None
+        if device is None or device:
+            md = device_model(model)
+            if md is None and device:
+                raise ValueError(f"sample_modes_batch: no device log-pdf for {type(model).__name__}")
+            if md is not None and device is None and not torch.cuda.is_available():
+                md = None
         rads = sample_ln_pdf_batch([p[1] for p in pending], model.ln_spectral_rad_pdf, [p[2] for p in pending],
-                                   sample_around=1.0 / model.len_rescaled, nthreads=nthreads)
+                                   sample_around=1.0 / model.len_rescaled, nthreads=nthreads, device_model=md)
         for (i, _, _), rad in zip(pending, rads):
             out[i][0] = rad * out[i][0]
     return [tuple(o) for o in out]

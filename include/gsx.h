@@ -31,9 +31,9 @@
  *     :44-113, call :438-466, :560-565            gsx_vario_workspace
  *   unstructured(..., distance_type='h')          gsx_vario_unstructured_haversine_f64
  *     (latlon=True, variogram.py:446)
- *   emcee stretch-move draws of                   gsx_stretch_draws (host only)
- *     RNG.sample_ln_pdf for many chains,
- *     src/gstools/random/rng.py:38-104
+ *   emcee stretch-move draws of                   gsx_stretch_draws (host only),
+ *     RNG.sample_ln_pdf for many chains,          gsx_stretch_run_device,
+ *     src/gstools/random/rng.py:38-104            gsx_stretch_pick_device
  *   (no reference counterpart: many realizations  gsx_batch_*
  *    in one launch; each item is one
  *    RandMeth.__call__, generator.py:243-270)
@@ -388,6 +388,50 @@ int gsx_stretch_accept(int64_t R, int32_t nwalkers, const int64_t* steps,
                        const double* log_uacc, const double* q,
                        const double* new_lp, double* coords, double* log_prob,
                        double* chain, int64_t chain_steps);
+
+/* Closed-form radial spectral log-pdf for the device sampler:
+ * log(rad_fac(dim, r) |S(r)|) of covmodel/models.py with the host-computed
+ * constants c[] (the scalar factors exactly as numpy computes them):
+ *   GAUSSIAN     c0 exp(-((r L / 2)^2)),             c0 = (L / 2 / sqrt(pi))^dim
+ *   EXPONENTIAL  c0 / (pi (1 + (r L)^2))^((dim+1)/2), c0 = L^dim Gamma((dim+1)/2)
+ *   MATERN       c0 exp(((c1 log(1 + (r L)^2 / c2) + c3) - c4) - c5),
+ *                c0 = (L / sqrt(pi))^dim, c1 = -(nu + dim/2), c2 = nu,
+ *                c3 = lgamma(nu + dim/2), c4 = lgamma(nu), c5 = dim log(sqrt(nu))
+ * (L = len_rescaled; Matern with nu <= 20 only). */
+#define GSX_STRETCH_GAUSSIAN 0
+#define GSX_STRETCH_EXPONENTIAL 1
+#define GSX_STRETCH_MATERN 2
+typedef struct gsx_stretch_model {
+    int32_t kind;
+    int32_t dim;
+    double len_rescaled;
+    double c[6];
+} gsx_stretch_model;
+
+/* Device form of the stretch-move sampler of RNG.sample_ln_pdf
+ * (src/gstools/random/rng.py:38-104) for R one-dimensional chains of
+ * nwalkers <= 64 walkers: one wavefront per chain runs burn_steps steps on
+ * the chain's first MT19937 stream, then prod_steps[r] steps on its second,
+ * recording the walkers after every production step in chain (R, max_steps,
+ * nwalkers) (may be NULL).  states (R, 2, 625) uint32: the legacy numpy key
+ * (624 words) and position of the burn-in and production streams; init
+ * (R, nwalkers) the initial walkers.  The draws, their order and the move's
+ * arithmetic are those of gsx_stretch_draws / _propose / _accept; the
+ * log-pdf is the model's closed form on the device.  final_coords (R,
+ * nwalkers) may be NULL.  status (device int, 0 on entry) becomes 1 for an
+ * infinite or NaN proposal, 2 for a NaN log-probability. */
+int gsx_stretch_run_device(int64_t R, int32_t nwalkers, const uint32_t* states,
+                           int64_t burn_steps, const int64_t* prod_steps,
+                           int64_t max_steps, double a,
+                           const gsx_stretch_model* model, const double* init,
+                           double* chain, double* final_coords, int* status,
+                           void* stream);
+/* out[i] = chain[owner[i]][idx[i]] (flat index step * nwalkers + walker) for
+ * i < total: the final choice of samples from each chain's recorded walkers. */
+int gsx_stretch_pick_device(int64_t total, const int64_t* owner,
+                            const int64_t* idx, int64_t max_steps,
+                            int32_t nwalkers, const double* chain, double* out,
+                            void* stream);
 
 /* Batched FP32 summation of many realizations in one persistent launch.
  * Each item is one RandMeth / IncomprRandMeth call (generator.py:243-270,
