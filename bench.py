@@ -501,24 +501,43 @@ def _model_of(kind):
 
 
 def batch_modes():
-    """Sample all batch modes on the host before any GPU use.
+    """Sample the modes of all 256 realizations.
 
-    Each kind shares one model, so its realizations are sampled together by
-    the lockstep sampler (identical arrays to one ``RandMeth`` per
-    realization).  The one-at-a-time sampler is timed on a few MCMC
-    realizations and scaled by their share of the MCMC modes.
+    Each kind shares one model, so its realizations are sampled together
+    (``sample_modes_batch``: every numpy draw made natively for all of them,
+    the MCMC chains on the GPU, identical arrays to one ``RandMeth`` per
+    realization).  Timed twice (the first call also loads the kernels); the
+    host lockstep sampler and the one-at-a-time sampler are timed for
+    comparison, the latter on a few MCMC realizations scaled by their share
+    of the MCMC modes.
     """
+    import torch
+
     from gstools_amd.sampling import sample_modes_batch
 
     specs = _batch_specs()
+
+    def sample_all(device):
+        modes = [None] * len(specs)
+        for kind in ("grid2d", "points3d", "incompr3d"):
+            idx = [i for i, sp in enumerate(specs) if sp[0] == kind]
+            got = sample_modes_batch(_model_of(kind), [specs[i][1] for i in idx], [specs[i][2] for i in idx],
+                                     device=device)
+            for i, g in zip(idx, got):
+                modes[i] = g
+        return modes
+
+    times = []
+    for _ in range(2):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        modes = sample_all(None)
+        torch.cuda.synchronize()
+        times.append(time.perf_counter() - t0)
     t0 = time.perf_counter()
-    modes = [None] * len(specs)
-    for kind in ("grid2d", "points3d", "incompr3d"):
-        idx = [i for i, sp in enumerate(specs) if sp[0] == kind]
-        got = sample_modes_batch(_model_of(kind), [specs[i][1] for i in idx], [specs[i][2] for i in idx])
-        for i, g in zip(idx, got):
-            modes[i] = g
-    t_lock = time.perf_counter() - t0
+    host = sample_all(False)
+    t_host = time.perf_counter() - t0
+    same = all(all(np.array_equal(a, b) for a, b in zip(m, h)) for m, h in zip(modes, host))
     mcmc = [sp for sp in specs if sp[0] != "grid2d"]
     sub = mcmc[::16]
     t0 = time.perf_counter()
@@ -527,11 +546,13 @@ def batch_modes():
     t_sub = time.perf_counter() - t0
     share = sum(sp[1] for sp in sub) / sum(sp[1] for sp in mcmc)
     info = {"realizations": len(specs), "mcmc_realizations": len(mcmc),
-            "lockstep_s": t_lock, "one_at_a_time_s_est": t_sub / share,
+            "sampling_s": times[-1], "sampling_s_first_call": times[0],
+            "host_lockstep_s": t_host, "device_equals_host_bitwise": bool(same),
+            "one_at_a_time_s_est": t_sub / share,
             "one_at_a_time_measured_on": f"{len(sub)} of {len(mcmc)} MCMC realizations, scaled by mode count",
-            "speedup": (t_sub / share) / t_lock}
-    log(f"bench: sampled modes of {len(specs)} realizations in {t_lock:.2f} s (lockstep); "
-        f"one at a time ~{t_sub / share:.1f} s")
+            "speedup_vs_one_at_a_time": (t_sub / share) / times[-1]}
+    log(f"bench: sampled modes of {len(specs)} realizations in {times[-1] * 1e3:.1f} ms (device chains); "
+        f"host lockstep {t_host:.2f} s; one at a time ~{t_sub / share:.1f} s")
     return specs, modes, info
 
 
@@ -686,11 +707,11 @@ def main():
     from gstools_amd import summate_device
     import oracle
 
-    want_batch = args.only == "batch" or (args.only is None and args.extras)
-    batch_inputs = batch_modes() if want_batch else None  # host-only, before the GPU is touched
     if not torch.cuda.is_available():
         print("bench.py: no HIP device visible", file=sys.stderr)
         sys.exit(1)
+    want_batch = args.only == "batch" or (args.only is None and args.extras)
+    batch_inputs = batch_modes() if want_batch else None
     dev = torch.device("cuda", 0)
     k_ext = max(3, args.steps // 4)
     w_ext = max(1, min(args.warmup, 3))  # the extra sections run long kernels

@@ -34,6 +34,7 @@
 // into a 53-bit fraction, (a >> 5) * 2^26 + (b >> 6) over 2^53; the masked
 // draw in [0, m] keeps the low bits of one output under the smallest
 // all-ones mask >= m and rejects values above m.
+#include <cmath>
 #include <cstdint>
 #include <thread>
 #include <vector>
@@ -97,6 +98,49 @@ struct Mt {
     }
 };
 
+// A legacy numpy RandomState: the MT19937 stream plus the cached second
+// normal of the polar method (numpy/random legacy distributions).
+struct Legacy {
+    uint32_t words[kMtN + 1];
+    bool has_gauss = false;
+    double gauss = 0.0;
+
+    // RandomState(seed) for an integer seed in [0, 2^32): Knuth's
+    // initialisation of the key (multiplier 1812433253), position at the end
+    void seed(uint32_t s) {
+        words[0] = s;
+        for (int i = 1; i < kMtN; ++i) words[i] = 1812433253u * (words[i - 1] ^ (words[i - 1] >> 30)) + (uint32_t)i;
+        words[kMtN] = kMtN;
+        has_gauss = false;
+        gauss = 0.0;
+    }
+    Mt mt() { return Mt{words}; }
+    double random_sample() { return mt().next_double(); }
+    // randint(low, high) for high - low <= 2^32: low + masked draw in [0, high - low - 1]
+    int64_t randint(int64_t low, int64_t high) { return low + (int64_t)mt().masked((uint32_t)(high - low - 1)); }
+    // standard normal by the polar method, every product and sum rounded
+    double gauss_draw() {
+        if (has_gauss) {
+            has_gauss = false;
+            const double t = gauss;
+            gauss = 0.0;
+            return t;
+        }
+        double x1, x2, r2;
+        do {
+            x1 = 2.0 * random_sample() - 1.0;
+            x2 = 2.0 * random_sample() - 1.0;
+            r2 = x1 * x1 + x2 * x2;
+        } while (r2 >= 1.0 || r2 == 0.0);
+        const double f = std::sqrt(-2.0 * std::log(r2) / r2);
+        gauss = f * x1;
+        has_gauss = true;
+        return f * x2;
+    }
+    double normal() { return 0.0 + 1.0 * gauss_draw(); }            // normal(0, 1)
+    double uniform(double low, double range) { return low + range * random_sample(); }
+};
+
 void chain_draws(Mt mt, int64_t steps, int32_t nw, int8_t* labels, double* uz, int64_t* partner,
                  double* uacc) {
     const int32_t n0 = (nw + 1) / 2;  // walkers labelled 0
@@ -152,6 +196,75 @@ int gsx_stretch_draws(int64_t R, uint32_t* states, const int64_t* steps, int64_t
         Mt mt{states + r * (kMtN + 1)};
         chain_draws(mt, steps[r], nwalkers, labels + r * per_chain, uz + r * per_chain, partner + r * per_chain,
                     uacc + r * per_chain);
+    };
+    int64_t workers = nthreads > 0 ? nthreads : 1;
+    if (workers > R) workers = R;
+    if (workers <= 1) {
+        for (int64_t r = 0; r < R; ++r) run(r);
+        return GSX_OK;
+    }
+    std::vector<std::thread> pool;
+    pool.reserve((size_t)workers);
+    for (int64_t t = 0; t < workers; ++t)
+        pool.emplace_back([&, t] {
+            for (int64_t r = t; r < R; r += workers) run(r);
+        });
+    for (auto& th : pool) th.join();
+    return GSX_OK;
+}
+
+int gsx_mode_draws(int64_t R, const int64_t* seeds, const int64_t* n_modes, int32_t dim, int32_t kind,
+                   int32_t nwalkers, double sample_around, const int64_t* n_flat, double* z1, double* z2,
+                   double* ang1, double* ang2, double* u_inv, double* init, uint32_t* states, int64_t* picks,
+                   int32_t nthreads) {
+    if (R < 0 || dim < 1 || dim > 3 || kind < 0 || kind > 2) return GSX_E_INVALID;
+    if (R == 0) return GSX_OK;
+    if (!seeds || !n_modes || !z1 || !z2) return GSX_E_INVALID;
+    if (kind > 0 && !ang1) return GSX_E_INVALID;
+    if (kind > 0 && dim == 3 && !ang2) return GSX_E_INVALID;
+    if (kind == 1 && !u_inv) return GSX_E_INVALID;
+    if (kind == 2 && (nwalkers < 1 || !n_flat || !init || !states || !picks)) return GSX_E_INVALID;
+    std::vector<int64_t> off((size_t)R + 1, 0);
+    for (int64_t r = 0; r < R; ++r) {
+        if (seeds[r] < 0 || seeds[r] > 0xffffffffLL || n_modes[r] < 0) return GSX_E_INVALID;
+        if (kind == 2 && (n_flat[r] < 1 || n_flat[r] > 0xffffffffLL)) return GSX_E_INVALID;
+        off[(size_t)r + 1] = off[(size_t)r] + n_modes[r];
+    }
+    const double two_pi = 6.283185307179586;  // 2 * pi - 0.0, the range of uniform(0, 2 pi)
+    auto run = [&](int64_t r) {
+        Legacy master, stream;
+        master.seed((uint32_t)seeds[r]);
+        auto next_stream = [&]() { stream.seed((uint32_t)master.randint(1, 65536)); };
+        const int64_t n = n_modes[r], o = off[(size_t)r];
+        next_stream();
+        for (int64_t i = 0; i < n; ++i) z1[o + i] = stream.normal();
+        next_stream();
+        for (int64_t i = 0; i < n; ++i) z2[o + i] = stream.normal();
+        if (kind == 0) return;
+        next_stream();
+        if (dim == 1) {  // choice([-1, 1]): randint(0, 2) picks the entry
+            for (int64_t i = 0; i < n; ++i) ang1[o + i] = stream.randint(0, 2) == 0 ? -1.0 : 1.0;
+        } else {
+            for (int64_t i = 0; i < n; ++i) ang1[o + i] = stream.uniform(0.0, two_pi);
+            if (dim == 3) {
+                next_stream();
+                for (int64_t i = 0; i < n; ++i) ang2[o + i] = stream.uniform(-1.0, 2.0);
+            }
+        }
+        if (kind == 1) {  // inverse-cdf sampling: uniform(size=n)
+            next_stream();
+            for (int64_t i = 0; i < n; ++i) u_inv[o + i] = stream.uniform(0.0, 1.0);
+            return;
+        }
+        next_stream();  // initial walkers: rand(nwalkers) * sample_around
+        for (int32_t w = 0; w < nwalkers; ++w) init[r * nwalkers + w] = stream.random_sample() * sample_around;
+        for (int k = 0; k < 2; ++k) {  // burn-in and production streams, fresh
+            next_stream();
+            uint32_t* dst = states + (r * 2 + k) * (kMtN + 1);
+            for (int i = 0; i <= kMtN; ++i) dst[i] = stream.words[i];
+        }
+        next_stream();  // the final choice(n_flat, n)
+        for (int64_t i = 0; i < n; ++i) picks[o + i] = stream.randint(0, n_flat[r]);
     };
     int64_t workers = nthreads > 0 ? nthreads : 1;
     if (workers > R) workers = R;

@@ -64,10 +64,15 @@ __device__ __forceinline__ uint32_t smear(uint32_t m) {
 
 __device__ __forceinline__ int lane_id() { return (int)threadIdx.x; }
 
-// One chain's MT19937 stream, shared by the 64 lanes of its wave.
+// One chain's MT19937 stream, shared by the 64 lanes of its wave.  With one
+// wave per SIMD every LDS round trip is exposed, so each draw routine reads
+// the key once: lanes load the words they need directly, and the label
+// shuffle works on a register window of 64 tempered words.
 struct WaveMt {
     uint32_t* key;  // LDS, kMtN words
     int pos;        // stream position (uniform across the wave)
+    uint32_t wv;    // window: lane l holds the tempered word wbase + l (l < wlen)
+    int wbase, wlen;
 
     __device__ void twist() {
         const int l = lane_id();
@@ -86,72 +91,88 @@ struct WaveMt {
         pos = 0;
     }
 
-    // lanes 0..n-1 receive the next n words (n <= 64)
-    __device__ uint32_t take(int n) {
-        const int l = lane_id();
+    // advance by n words without reading them
+    __device__ void skip(int n) {
         const int avail = kMtN - pos;
-        uint32_t v = 0;
-        if (l < n && l < avail) v = temper(key[pos + l]);
         if (n > avail) {
             twist();
-            if (l >= avail && l < n) v = temper(key[l - avail]);
             pos = n - avail;
         } else {
             pos += n;
         }
-        return v;
     }
 
-    // random_sample(): lane i < count receives the i-th of `count` doubles (count <= 32)
-    __device__ double doubles(int count) {
-        const uint32_t w = take(2 * count);
-        const int l = lane_id();
-        const int src = 2 * (l < count ? l : 0);
-        const uint32_t a = (uint32_t)__shfl((int)w, src) >> 5;
-        const uint32_t b = (uint32_t)__shfl((int)w, src + 1) >> 6;
+    // random_sample() number i of a batch of `count`: words pos + 2i and
+    // pos + 2i + 1, read by the lanes that `need` one; the stream then
+    // advances by 2 count words (twisting on the way where the batch crosses
+    // the end of the key)
+    __device__ double double_at(int i, bool need, int count) {
+        const int p0 = pos;
+        const int o1 = p0 + 2 * i, o2 = o1 + 1;
+        uint32_t a = 0, b = 0;
+        if (need) {
+            if (o1 < kMtN) a = key[o1];
+            if (o2 < kMtN) b = key[o2];
+        }
+        const int total = 2 * count;
+        if (total > kMtN - p0) {
+            twist();
+            if (need) {
+                if (o1 >= kMtN) a = key[o1 - kMtN];
+                if (o2 >= kMtN) b = key[o2 - kMtN];
+            }
+            pos = p0 + total - kMtN;
+        } else {
+            pos = p0 + total;
+        }
+        a = temper(a) >> 5;
+        b = temper(b) >> 6;
         return ((double)a * 67108864.0 + (double)b) / 9007199254740992.0;
     }
 
-    // one uniform integer in [0, m] by masked rejection; the same value in every lane
+    // register window of the words from pos on (twists at the end of the key)
+    __device__ void load_window() {
+        if (pos >= kMtN) twist();
+        const int l = lane_id();
+        wbase = pos;
+        wlen = kMtN - pos < kWave ? kMtN - pos : kWave;
+        wv = l < wlen ? temper(key[pos + l]) : 0u;
+    }
+
+    // one uniform integer in [0, m] by masked rejection (m > 0), from the
+    // window; the same value in every lane
     __device__ uint32_t masked_one(uint32_t m) {
-        if (m == 0) return 0;
         const uint32_t mask = smear(m);
         const int l = lane_id();
         for (;;) {
-            int avail = kMtN - pos;
-            if (avail == 0) {
-                twist();
-                avail = kMtN;
+            const int off = pos - wbase;
+            if (off >= wlen) {
+                load_window();
+                continue;
             }
-            const int n = avail < kWave ? avail : kWave;
-            const uint32_t v = l < n ? (temper(key[pos + l]) & mask) : 0xffffffffu;
-            const unsigned long long ok = __ballot(l < n && v <= m);
+            const unsigned long long ok = __ballot(l >= off && l < wlen && (wv & mask) <= m);
             if (ok) {
                 const int f = __ffsll((long long)ok) - 1;
-                pos += f + 1;
-                return (uint32_t)__shfl((int)v, f);
+                pos = wbase + f + 1;
+                return (uint32_t)__builtin_amdgcn_readlane((int)wv, f) & mask;
             }
-            pos += n;
+            pos = wbase + wlen;
         }
     }
 
-    // `count` masked draws in [0, m] (count <= 64): lane i receives the i-th;
-    // slot is a 64-entry LDS scratch row
-    __device__ uint32_t masked_batch(uint32_t m, int count, uint32_t* slot) {
-        if (m == 0) return 0;
+    // `count` masked draws in [0, m] (count <= 64): the draws are left in
+    // slot[0 .. count) of LDS (the caller reads them after a barrier); m == 0
+    // draws nothing (every value is 0)
+    __device__ void masked_batch(uint32_t m, int count, uint32_t* slot) {
+        if (m == 0) return;
         const uint32_t mask = smear(m);
         const int l = lane_id();
         const unsigned long long below = (l == 0) ? 0ull : (~0ull >> (64 - l));
         int got = 0;
-        while (got < count) {
-            int avail = kMtN - pos;
-            if (avail == 0) {
-                twist();
-                avail = kMtN;
-            }
-            const int n = avail < kWave ? avail : kWave;
-            const uint32_t v = l < n ? (temper(key[pos + l]) & mask) : 0xffffffffu;
-            const bool ok = l < n && v <= m;
+        load_window();
+        for (;;) {
+            const uint32_t v = wv & mask;
+            const bool ok = l < wlen && v <= m;
             const unsigned long long bal = __ballot(ok);
             const int rank = __popcll(bal & below);
             const int need = count - got;
@@ -159,17 +180,13 @@ struct WaveMt {
             const int total = __popcll(bal);
             if (total >= need) {
                 const unsigned long long last = __ballot(ok && rank == need - 1);
-                pos += __ffsll((long long)last);  // index of that lane + 1
-                got = count;
-            } else {
-                pos += n;
-                got += total;
+                pos = wbase + __ffsll((long long)last);  // that lane + 1
+                return;
             }
+            got += total;
+            pos = wbase + wlen;
+            load_window();
         }
-        __syncthreads();
-        const uint32_t out = l < count ? slot[l] : 0u;
-        __syncthreads();
-        return out;
     }
 };
 
@@ -228,7 +245,7 @@ __global__ __launch_bounds__(kWave) void k_stretch_chains(int64_t R, int nw, con
 #pragma clang fp contract(off)
     __shared__ uint32_t key[kMtN];
     __shared__ uint32_t slot[kWave];
-    __shared__ int other_of[kWave];
+    __shared__ double x_other[kWave];
     const int64_t r = blockIdx.x;
     if (r >= R) return;
     const int l = lane_id();
@@ -246,11 +263,12 @@ __global__ __launch_bounds__(kWave) void k_stretch_chains(int64_t R, int nw, con
         const uint32_t* st = states + (r * 2 + phase) * kStateWords;
         for (int i = l; i < kMtN; i += kWave) key[i] = st[i];
         __syncthreads();
-        WaveMt mt{key, (int)st[kMtN]};
+        WaveMt mt{key, (int)st[kMtN], 0u, 0, 0};
         const int64_t steps = phase == 0 ? burn_steps : prod_steps[r];
         for (int64_t s = 0; s < steps; ++s) {
-            (void)mt.take(2);  // move choice
+            mt.skip(2);  // the move choice (one random_sample, unused)
             unsigned long long labels = 0xaaaaaaaaaaaaaaaaull & all;  // walker w has label w % 2
+            mt.load_window();
             for (int i = nw - 1; i >= 1; --i) {
                 const uint32_t j = mt.masked_one((uint32_t)i);
                 const unsigned long long bi = (labels >> i) & 1ull, bj = (labels >> j) & 1ull;
@@ -263,26 +281,20 @@ __global__ __launch_bounds__(kWave) void k_stretch_chains(int64_t R, int nw, con
                 const int nc = nw - ns;
                 const bool mine = (own >> l) & 1ull;
                 const int rank = __popcll((mine ? own : oth) & below);
-                if (walker && !mine) other_of[rank] = l;
-                const double u_z = mt.doubles(ns);
-                const uint32_t part = mt.masked_batch((uint32_t)(nc - 1), ns, slot);
-                const double u_acc = mt.doubles(ns);
+                if (walker && !mine) x_other[rank] = x;
+                const double uz = mt.double_at(rank, mine, ns);
+                mt.masked_batch((uint32_t)(nc - 1), ns, slot);
+                const double ua = mt.double_at(rank, mine, ns);
                 __syncthreads();
-                // walker of rank i in `own` takes draw i of each batch
-                const int src = mine ? rank : 0;
-                const double uz = __shfl(u_z, src);
-                const int p = (int)__shfl((int)part, src);
-                const double ua = __shfl(u_acc, src);
-                const int pw = other_of[mine ? p : 0];
-                const double cp = __shfl(x, pw);
+                const int p = (mine && nc > 1) ? (int)slot[rank] : 0;
+                const double cp = x_other[mine ? p : 0];
                 const double t = (a - 1.0) * uz + 1.0;
                 const double zz = (t * t) / a;
                 const double q = cp - (cp - x) * zz;
                 bool bad_q = false, bad_lp = false;
-                double nl = 0.0;
                 if (walker && mine) {
                     bad_q = !isfinite(q);
-                    nl = ln_rad_pdf(md, q);
+                    const double nl = ln_rad_pdf(md, q);
                     bad_lp = isnan(nl);
                     const double lnpdiff = (0.0 + nl) - lp;
                     if (!bad_q && !bad_lp && lnpdiff > log(ua)) {

@@ -35,6 +35,7 @@ bit equal to the host sampler's unless a test lands within an ulp.
 """
 
 import ctypes
+import os
 
 import numpy as np
 import torch
@@ -115,6 +116,7 @@ def _lockstep(ln_pdf, coords, log_prob, steps, states, a=2.0, record=False, bloc
     """
     lib = _native.lib()
     nr, nw = coords.shape
+    states = np.ascontiguousarray(states, dtype=np.uint32)
     coords = np.ascontiguousarray(coords, dtype=np.double)
     log_prob = np.ascontiguousarray(log_prob, dtype=np.double)
     steps = np.ascontiguousarray(steps, dtype=np.int64)
@@ -185,44 +187,73 @@ _STRETCH_ERRORS = {1: "At least one parameter value was infinite or NaN",
                    2: "Probability function returned NaN"}
 
 
-def _chains_device(md, init, burn, prod, burn_in, prod_steps, sizes, pickers, a=2.0):
-    """Run every chain on the GPU and return each chain's ``choice`` of samples."""
-    dev = torch.device("cuda", torch.cuda.current_device())
-    nr, nw = init.shape
-    max_steps = max(int(prod_steps.max(initial=0)), 1)
-    states = torch.from_numpy(np.ascontiguousarray(np.stack([burn, prod], axis=1)).view(np.int32)).to(dev)
-    steps_t = torch.from_numpy(np.ascontiguousarray(prod_steps, dtype=np.int64)).to(dev)
-    init_t = torch.from_numpy(np.ascontiguousarray(init, dtype=np.double)).to(dev)
-    chain = torch.empty((nr, max_steps, nw), dtype=torch.float64, device=dev)
-    status = torch.zeros(1, dtype=torch.int32, device=dev)
-    stream = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
-    lib = _native.lib()
-    _native.check(lib.gsx_stretch_run_device(nr, nw, _tp(states), int(burn_in), _tp(steps_t), max_steps, float(a),
-                                             ctypes.byref(md), _tp(init_t), _tp(chain), None, _tp(status), stream),
-                  "stretch_run_device")
-    # the final choice: indices from each chain's own generator, gathered on the device
-    idx = [pickers[r].choice(int(prod_steps[r]) * nw, sizes[r]) for r in range(nr)]
-    counts = [len(i) for i in idx]
-    owner = np.repeat(np.arange(nr, dtype=np.int64), counts)
-    flat = np.concatenate(idx).astype(np.int64)
-    out = torch.empty(max(flat.size, 1), dtype=torch.float64, device=dev)
-    owner_t, flat_t = torch.from_numpy(owner).to(dev), torch.from_numpy(flat).to(dev)
-    _native.check(lib.gsx_stretch_pick_device(flat.size, _tp(owner_t), _tp(flat_t), max_steps, nw, _tp(chain),
-                                              _tp(out), stream), "stretch_pick_device")
-    code = int(status.item())
-    if code:
-        raise ValueError(_STRETCH_ERRORS.get(code, f"device sampler status {code}"))
-    rad = out[:flat.size].cpu().numpy()
-    return np.split(rad, np.cumsum(counts)[:-1])
+class _DeviceChains:
+    """The chains of one ``gsx_stretch_run_device`` launch (running on the
+    current stream; :meth:`pick` waits for it)."""
+
+    def __init__(self, md, init, burn, prod, burn_in, prod_steps, a=2.0):
+        dev = torch.device("cuda", torch.cuda.current_device())
+        nr, self.nw = init.shape
+        self.max_steps = max(int(prod_steps.max(initial=0)), 1)
+        states = np.ascontiguousarray(np.stack([burn, prod], axis=1), dtype=np.uint32)
+        self._keep = [torch.from_numpy(states.view(np.int32)).to(dev),
+                      torch.from_numpy(np.ascontiguousarray(prod_steps, dtype=np.int64)).to(dev),
+                      torch.from_numpy(np.ascontiguousarray(init, dtype=np.double)).to(dev)]
+        self.chain = torch.empty((nr, self.max_steps, self.nw), dtype=torch.float64, device=dev)
+        self.status = torch.zeros(1, dtype=torch.int32, device=dev)
+        st, steps_t, init_t = self._keep
+        _native.check(_native.lib().gsx_stretch_run_device(nr, self.nw, _tp(st), int(burn_in), _tp(steps_t),
+                                                           self.max_steps, float(a), ctypes.byref(md), _tp(init_t),
+                                                           _tp(self.chain), None, _tp(self.status), _stream()),
+                      "stretch_run_device")
+
+    def pick(self, picks):
+        """Chain r's recorded walkers at the flat indices picks[r] -> list of arrays."""
+        dev = self.chain.device
+        counts = [len(p) for p in picks]
+        owner = np.repeat(np.arange(len(picks), dtype=np.int64), counts)
+        flat = np.concatenate(picks).astype(np.int64) if picks else np.zeros(0, dtype=np.int64)
+        out = torch.empty(max(flat.size, 1), dtype=torch.float64, device=dev)
+        owner_t, flat_t = torch.from_numpy(owner).to(dev), torch.from_numpy(flat).to(dev)
+        _native.check(_native.lib().gsx_stretch_pick_device(flat.size, _tp(owner_t), _tp(flat_t), self.max_steps,
+                                                            self.nw, _tp(self.chain), _tp(out), _stream()),
+                      "stretch_pick_device")
+        code = int(self.status.item())
+        if code:
+            raise ValueError(_STRETCH_ERRORS.get(code, f"device sampler status {code}"))
+        rad = out[:flat.size].cpu().numpy()
+        return np.split(rad, np.cumsum(counts)[:-1])
 
 
 def _tp(t):
     return ctypes.c_void_p(t.data_ptr())
 
 
+def _stream():
+    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def _chains_host(ln_pdf, init, burn, prod, burn_in, prod_steps, picks, block=64, nthreads=1):
+    """The host lockstep sampler: chain r's recorded walkers at picks[r]."""
+    nr = init.shape[0]
+    log_prob = _log_prob(ln_pdf, init)
+    if np.any(np.isnan(log_prob)):
+        raise ValueError("The initial log_prob was NaN")
+    coords, log_prob, _ = _lockstep(ln_pdf, init, log_prob, np.full(nr, burn_in, dtype=np.int64), burn,
+                                    block=block, nthreads=nthreads)
+    _, _, chain = _lockstep(ln_pdf, coords, log_prob, prod_steps, prod, record=True, block=block,
+                            nthreads=nthreads)
+    return [chain[r, :prod_steps[r]].reshape(-1)[picks[r]] for r in range(nr)]
+
+
+def _prod_steps(sizes, nwalkers, burn_in, oversampling_factor):
+    """Production steps of each run (``rng.py:88-101``)."""
+    return np.array([int(burn_in if sz is None else max(burn_in, (sz / nwalkers) * oversampling_factor))
+                     for sz in sizes], dtype=np.int64)
+
+
 def sample_ln_pdf_batch(rngs, ln_pdf, sizes, sample_around=1.0, nwalkers=50, burn_in=20, oversampling_factor=10,
                         block=64, nthreads=1, device_model=None):
-
     """``RNG.sample_ln_pdf`` for many generators sharing one ``ln_pdf``.
 
     ``rngs`` are :class:`gstools_amd.rng.RNG` objects, each consumed exactly
@@ -238,39 +269,53 @@ def sample_ln_pdf_batch(rngs, ln_pdf, sizes, sample_around=1.0, nwalkers=50, bur
     sizes = list(sizes)
     if len(sizes) != nr:
         raise ValueError("sample_ln_pdf_batch: one size per generator")
-    prod_steps = np.array([int(burn_in if sz is None else max(burn_in, (sz / nwalkers) * oversampling_factor))
-                           for sz in sizes], dtype=np.int64)
+    prod_steps = _prod_steps(sizes, nwalkers, burn_in, oversampling_factor)
     init = np.empty((nr, nwalkers))
     burn = np.empty((nr, _MT_WORDS), dtype=np.uint32)
     prod = np.empty((nr, _MT_WORDS), dtype=np.uint32)
-    pickers = []
+    picks = []
     for r, rng in enumerate(rngs):
         init[r] = rng.random.rand(nwalkers) * sample_around
         burn[r] = _mt_state(rng.random)
         prod[r] = _mt_state(rng.random)
-        pickers.append(rng.random)
-    for r in range(nr):
+        # choice(chain, size) draws the indices choice(len(chain), size) does
+        picks.append(rng.random.choice(int(prod_steps[r]) * nwalkers, sizes[r]))
+    _check_init(init)
+    if device_model is not None and nwalkers <= 64:
+        return _DeviceChains(device_model, init, burn, prod, burn_in, prod_steps).pick(picks)
+    return _chains_host(ln_pdf, init, burn, prod, burn_in, prod_steps, picks, block, nthreads)
+
+
+def _check_init(init):
+    for r in range(init.shape[0]):
         if np.all(init[r] == init[r, 0]):
             raise ValueError("Initial state has walkers that are not linearly independent")
-    if device_model is not None and nwalkers <= 64:
-        return _chains_device(device_model, init, burn, prod, burn_in, prod_steps, sizes, pickers)
-    log_prob = _log_prob(ln_pdf, init)
-    if np.any(np.isnan(log_prob)):
-        raise ValueError("The initial log_prob was NaN")
-    coords, log_prob, _ = _lockstep(ln_pdf, init, log_prob, np.full(nr, burn_in, dtype=np.int64), burn,
-                                    block=block, nthreads=nthreads)
-    _, _, chain = _lockstep(ln_pdf, coords, log_prob, prod_steps, prod, record=True, block=block,
-                            nthreads=nthreads)
-    return [pickers[r].choice(chain[r, :prod_steps[r]].reshape(-1), sizes[r]) for r in range(nr)]
 
 
-def sample_modes_batch(model, mode_nos, seeds, sampling="auto", nthreads=1, device=None):
+def _sphere(dim, ang1, ang2):
+    """``RNG.sample_sphere`` from its uniform draws (``rng.py:142-193``)."""
+    if dim == 1:
+        return ang1.reshape(1, -1).copy()
+    if dim == 2:
+        return np.array([np.cos(ang1), np.sin(ang1)])
+    return np.array([np.sqrt(1.0 - ang2**2) * np.cos(ang1), np.sqrt(1.0 - ang2**2) * np.sin(ang1), ang2])
+
+
+def _native_seed(seed):
+    return isinstance(seed, (int, np.integer)) and not isinstance(seed, bool) and 0 <= int(seed) < 2**32
+
+
+def sample_modes_batch(model, mode_nos, seeds, sampling="auto", nthreads=None, device=None, nwalkers=50,
+                       burn_in=20, oversampling_factor=10):
     """``(cov_sample, z_1, z_2)`` of ``RandMeth(model, mode_no=n, seed=s)`` for every pair.
 
-    Same draws, in the same order, as ``RandMeth.reset_seed``; the MCMC runs
-    of all pairs advance together (:func:`sample_ln_pdf_batch`), on the GPU
-    when ``device`` is true (default: when a HIP device is present and the
-    model has a device log-pdf, :func:`device_model`).
+    Same draws, in the same order, as ``RandMeth.reset_seed``.  For integer
+    seeds every numpy draw is made natively for all realizations at once
+    (``gsx_mode_draws``, over ``nthreads`` host threads, default all); the
+    MCMC runs of all realizations advance together, on the GPU when
+    ``device`` is true (default: when a HIP device is present and the model
+    has a device log-pdf, :func:`device_model`), while the host finishes the
+    directions.  Other seeds take the one-generator-at-a-time draws.
     """
     mode_nos = [int(n) for n in mode_nos]
     seeds = list(seeds)
@@ -278,20 +323,80 @@ def sample_modes_batch(model, mode_nos, seeds, sampling="auto", nthreads=1, devi
         raise ValueError("sample_modes_batch: one seed per mode_no")
     if sampling not in ("auto", "inversion", "mcmc"):
         raise ValueError("sample_modes_batch: sampling not in ['auto', 'inversion', 'mcmc']")
+    if not mode_nos:
+        return []
     dim = model.dim
     zero_var = bool(np.isclose(model.var, 0))
     inversion = sampling == "inversion" or (sampling == "auto" and model.has_ppf)
-    out = []
-    pending = []
+    kind = 0 if zero_var else (1 if inversion else 2)
+    md = None
+    if kind == 2 and (device is None or device):
+        md = device_model(model)
+        if md is None and device:
+            raise ValueError(f"sample_modes_batch: no device log-pdf for {type(model).__name__}")
+        if md is not None and device is None and not torch.cuda.is_available():
+            md = None
+    threads = len(os.sched_getaffinity(0)) if nthreads is None else int(nthreads)
+    if not all(_native_seed(s) for s in seeds) or dim > 3:
+        return _modes_python(model, mode_nos, seeds, kind, md, nwalkers, burn_in, oversampling_factor, threads)
+    nr = len(mode_nos)
+    n = np.asarray(mode_nos, dtype=np.int64)
+    total = int(n.sum())
+    z1, z2 = np.empty(total), np.empty(total)
+    ang1 = np.empty(total) if kind else None
+    ang2 = np.empty(total) if kind and dim == 3 else None
+    u_inv = np.empty(total) if kind == 1 else None
+    mc = kind == 2
+    prod_steps = _prod_steps(mode_nos, nwalkers, burn_in, oversampling_factor) if mc else None
+    n_flat = prod_steps * nwalkers if mc else None
+    init = np.empty((nr, nwalkers)) if mc else None
+    states = np.empty((nr, 2, _MT_WORDS), dtype=np.uint32) if mc else None
+    picks = np.empty(total, dtype=np.int64) if mc else None
+    around = 1.0 / model.len_rescaled if mc else 0.0
+    seeds_a = np.asarray([int(s) for s in seeds], dtype=np.int64)
+    opt = lambda a: None if a is None else _vp(a)
+    _native.check(_native.lib().gsx_mode_draws(nr, _vp(seeds_a), _vp(n), dim, kind, nwalkers, float(around),
+                                               opt(n_flat), _vp(z1), _vp(z2), opt(ang1), opt(ang2), opt(u_inv),
+                                               opt(init), opt(states), opt(picks), threads), "mode_draws")
+    cuts = np.cumsum(n)[:-1]
+    z1s, z2s = np.split(z1, cuts), np.split(z2, cuts)
+    if kind == 0:
+        return [(np.full((dim, k), 0.0), a, b) for k, a, b in zip(mode_nos, z1s, z2s)]
+    chains = None
+    if mc:
+        _check_init(init)
+        burn, prod = np.ascontiguousarray(states[:, 0]), np.ascontiguousarray(states[:, 1])
+        if md is not None and nwalkers <= 64:
+            chains = _DeviceChains(md, init, burn, prod, burn_in, prod_steps)  # runs while the host continues
+    sph = _sphere(dim, ang1, ang2)
+    if kind == 1:
+        pdf, cdf, ppf = model.dist_func
+        rad = np.asarray(ppf(u_inv), dtype=np.double) * 1.0 + 0.0
+        rads = np.split(rad, cuts)
+    else:
+        pick_list = np.split(picks, cuts)
+        if chains is not None:
+            rads = chains.pick(pick_list)
+        else:
+            rads = _chains_host(model.ln_spectral_rad_pdf, init, burn, prod, burn_in, prod_steps, pick_list,
+                                nthreads=threads)
+    sphs = np.split(sph, cuts, axis=1)
+    return [(r * s, a, b) for r, s, a, b in zip(rads, sphs, z1s, z2s)]
+
+
+def _modes_python(model, mode_nos, seeds, kind, md, nwalkers, burn_in, oversampling_factor, threads):
+    """Per-realization numpy draws (any seed ``RandomState`` accepts)."""
+    dim = model.dim
+    out, pending = [], []
     for n, seed in zip(mode_nos, seeds):
         rng = RNG(seed)
         z1 = rng.random.normal(size=n)
         z2 = rng.random.normal(size=n)
-        if zero_var:
+        if kind == 0:
             out.append([np.full((dim, n), 0.0), z1, z2])
             continue
         sph = rng.sample_sphere(dim, n)
-        if inversion:
+        if kind == 1:
             pdf, cdf, ppf = model.dist_func
             rad = rng.sample_dist(size=n, pdf=pdf, cdf=cdf, ppf=ppf, a=0)
             out.append([rad * sph, z1, z2])
@@ -299,15 +404,9 @@ def sample_modes_batch(model, mode_nos, seeds, sampling="auto", nthreads=1, devi
             out.append([sph, z1, z2])
             pending.append((len(out) - 1, rng, n))
     if pending:
-        md = None
-        if device is None or device:
-            md = device_model(model)
-            if md is None and device:
-                raise ValueError(f"sample_modes_batch: no device log-pdf for {type(model).__name__}")
-            if md is not None and device is None and not torch.cuda.is_available():
-                md = None
         rads = sample_ln_pdf_batch([p[1] for p in pending], model.ln_spectral_rad_pdf, [p[2] for p in pending],
-                                   sample_around=1.0 / model.len_rescaled, nthreads=nthreads, device_model=md)
+                                   sample_around=1.0 / model.len_rescaled, nwalkers=nwalkers, burn_in=burn_in,
+                                   oversampling_factor=oversampling_factor, nthreads=threads, device_model=md)
         for (i, _, _), rad in zip(pending, rads):
             out[i][0] = rad * out[i][0]
     return [tuple(o) for o in out]

@@ -365,6 +365,27 @@ int gsx_stretch_draws(int64_t R, uint32_t* states, const int64_t* steps,
                       double* uz, int64_t* partner, double* uacc,
                       int32_t nthreads);
 
+/* Host-only: every legacy numpy draw RandMeth.reset_seed makes
+ * (src/gstools/field/generator.py:346-387, random/rng.py:38-193) for R
+ * realizations sharing one model: master RandomState(seeds[r]) (a seed in
+ * [0, 2^32)) hands out stream seeds randint(1, 2^16); the streams give z_1,
+ * z_2 = normal(size=n) (polar method, cached second normal), then for
+ * kind >= 1 the sphere draws (dim 1: choice([-1, 1]) as -1.0 / 1.0 in ang1;
+ * dim 2: uniform(0, 2 pi) in ang1; dim 3: that plus uniform(-1, 1) in ang2),
+ * then for kind 1 (inverse cdf) uniform(size=n) in u_inv, for kind 2 (MCMC)
+ * the initial walkers rand(nwalkers) * sample_around (R, nwalkers), the
+ * fresh burn-in and production streams (R, 2, 625) as gsx_stretch_run_device
+ * takes them, and the final choice(n_flat[r], n) indices in picks.  kind 0
+ * (zero variance) stops after z_2.  Per-mode outputs are concatenated over r
+ * (offsets = prefix sums of n_modes).  nthreads > 1 splits the realizations
+ * over host threads. */
+int gsx_mode_draws(int64_t R, const int64_t* seeds, const int64_t* n_modes,
+                   int32_t dim, int32_t kind, int32_t nwalkers,
+                   double sample_around, const int64_t* n_flat, double* z1,
+                   double* z2, double* ang1, double* ang2, double* u_inv,
+                   double* init, uint32_t* states, int64_t* picks,
+                   int32_t nthreads);
+
 /* Host-only: the two halves of one stretch-move half-move for every chain r
  * with steps[r] > step, one-dimensional walkers (emcee's arithmetic, every
  * operation rounded, no fused multiply-add).  Draws are row block_row of

@@ -144,3 +144,45 @@ def test_batch_argument_errors():
     with pytest.raises(ValueError):
         sample_modes_batch(model, [10], [1], sampling="gibbs")
     assert sample_modes_batch(model, [], []) == []
+
+
+@pytest.mark.parametrize("dim,kind", [(1, 1), (2, 1), (3, 2), (2, 2), (1, 2), (3, 0)])
+def test_native_mode_draws_equal_numpy(dim, kind):
+    """``gsx_mode_draws`` against numpy's own legacy calls, realization by
+    realization: master ``RandomState(seed).randint(1, 2**16)`` stream seeds,
+    ``normal``, the sphere's ``choice`` / ``uniform`` draws, ``uniform`` for the
+    inverse cdf, initial walkers, fresh stream states and the final ``choice``."""
+    seeds = np.array([0, 1, 65535, 2**32 - 1, 123456789], dtype=np.int64)
+    n = np.array([5, 1, 0, 333, 64], dtype=np.int64)
+    nw, around = 7, 0.37
+    n_flat = np.array([70, 7, 21, 1400, 140], dtype=np.int64)
+    tot = int(n.sum())
+    z1, z2, a1, a2, ui = (np.full(tot, np.nan) for _ in range(5))
+    init = np.zeros((len(n), nw))
+    states = np.zeros((len(n), 2, 625), dtype=np.uint32)
+    picks = np.full(tot, -1, dtype=np.int64)
+    st = _native.lib().gsx_mode_draws(len(n), _vp(seeds), _vp(n), dim, kind, nw, around, _vp(n_flat), _vp(z1),
+                                      _vp(z2), _vp(a1), _vp(a2), _vp(ui), _vp(init), _vp(states), _vp(picks), 3)
+    assert st == 0
+    off = np.concatenate([[0], np.cumsum(n)])
+    for r, seed in enumerate(seeds):
+        master = np.random.RandomState(int(seed))
+        stream = lambda: np.random.RandomState(master.randint(1, 2**16))
+        sl = slice(off[r], off[r + 1])
+        assert np.array_equal(z1[sl], stream().normal(size=n[r]))
+        assert np.array_equal(z2[sl], stream().normal(size=n[r]))
+        if kind == 0:
+            continue
+        if dim == 1:
+            assert np.array_equal(a1[sl], stream().choice([-1, 1], size=n[r]).astype(float))
+        else:
+            assert np.array_equal(a1[sl], stream().uniform(0.0, 2 * np.pi, n[r]))
+            if dim == 3:
+                assert np.array_equal(a2[sl], stream().uniform(-1.0, 1.0, n[r]))
+        if kind == 1:
+            assert np.array_equal(ui[sl], stream().uniform(size=n[r]))
+            continue
+        assert np.array_equal(init[r], stream().rand(nw) * around)
+        for k in range(2):
+            assert np.array_equal(states[r, k], _mt_state(stream()))
+        assert np.array_equal(picks[sl], stream().choice(int(n_flat[r]), n[r]))
