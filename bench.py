@@ -504,9 +504,9 @@ def batch_modes():
     """Sample the modes of all 256 realizations.
 
     Each kind shares one model, so its realizations are sampled together
-    (``sample_modes_batch``: every numpy draw made natively for all of them,
-    the MCMC chains on the GPU, identical arrays to one ``RandMeth`` per
-    realization).  Timed twice (the first call also loads the kernels); the
+    (``sample_modes_many``: every numpy draw made natively for all of them,
+    the MCMC chains of both MCMC kinds in one GPU launch, identical arrays to
+    one ``RandMeth`` per realization).  Timed twice (the first call also loads the kernels); the
     host lockstep sampler and the one-at-a-time sampler are timed for
     comparison, the latter on a few MCMC realizations scaled by their share
     of the MCMC modes.
@@ -517,12 +517,14 @@ def batch_modes():
 
     specs = _batch_specs()
 
+    kinds = ("grid2d", "points3d", "incompr3d")
+    groups_idx = [[i for i, sp in enumerate(specs) if sp[0] == kind] for kind in kinds]
+
     def sample_all(device):
+        groups = [(_model_of(kind), [specs[i][1] for i in idx], [specs[i][2] for i in idx])
+                  for kind, idx in zip(kinds, groups_idx)]
         modes = [None] * len(specs)
-        for kind in ("grid2d", "points3d", "incompr3d"):
-            idx = [i for i, sp in enumerate(specs) if sp[0] == kind]
-            got = sample_modes_batch(_model_of(kind), [specs[i][1] for i in idx], [specs[i][2] for i in idx],
-                                     device=device)
+        for idx, got in zip(groups_idx, sample_modes_many(groups, device=device)):
             for i, g in zip(idx, got):
                 modes[i] = g
         return modes

@@ -110,8 +110,7 @@ EXPORTS = {
     "gsx_stretch_accept": (_i32, [_i64, _i32, _vp, _i64, _i64, _i64, _i32, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i64]),
     "gsx_mode_draws": (_i32, [_i64, _vp, _vp, _i32, _i32, _i32, _f64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
                               _i32]),
-    "gsx_stretch_run_device": (_i32, [_i64, _i32, _vp, _i64, _vp, _i64, _f64, ctypes.POINTER(StretchModel), _vp,
-                                      _vp, _vp, _vp, _vp]),
+    "gsx_stretch_run_device": (_i32, [_i64, _i32, _vp, _i64, _vp, _i64, _f64, _vp, _vp, _vp, _vp, _vp, _vp]),
     "gsx_stretch_pick_device": (_i32, [_i64, _vp, _vp, _i64, _i32, _vp, _vp, _vp]),
     "gsx_batch_plan_create": (_i32, [ctypes.POINTER(BatchItem), _i64, _i64, _i64, ctypes.POINTER(_vp)]),
     "gsx_batch_plan_launch": (_i32, [_vp, _vp]),

@@ -139,24 +139,38 @@ struct WaveMt {
         wv = l < wlen ? temper(key[pos + l]) : 0u;
     }
 
-    // one uniform integer in [0, m] by masked rejection (m > 0), from the
-    // window; the same value in every lane
-    __device__ uint32_t masked_one(uint32_t m) {
-        const uint32_t mask = smear(m);
+    // Fisher-Yates shuffle of the 1-bit labels of nw walkers: for i = nw-1
+    // down to 1, swap labels i and j = masked draw in [0, i].  Calls that
+    // share a mask are resolved together: call k of the run tries window
+    // word off + k against its own bound; every call before the first
+    // rejection is done (its word accepted), the rejecting call then retries
+    // from the next word -- the sequential consumption, several calls a pass.
+    __device__ void shuffle_labels(unsigned long long& labels, int nw) {
         const int l = lane_id();
-        for (;;) {
+        load_window();
+        int i = nw - 1;
+        while (i >= 1) {
             const int off = pos - wbase;
             if (off >= wlen) {
                 load_window();
                 continue;
             }
-            const unsigned long long ok = __ballot(l >= off && l < wlen && (wv & mask) <= m);
-            if (ok) {
-                const int f = __ffsll((long long)ok) - 1;
-                pos = wbase + f + 1;
-                return (uint32_t)__builtin_amdgcn_readlane((int)wv, f) & mask;
+            const uint32_t mask = smear((uint32_t)i);
+            const int lo = mask > 1u ? (int)((mask + 1u) >> 1) : 1;  // smallest bound with this mask
+            const int k = l - off;
+            const bool cand = k >= 0 && k <= i - lo && l < wlen;
+            const bool acc = cand && (wv & mask) <= (uint32_t)(i - k);
+            const unsigned long long fail = __ballot(cand && !acc);
+            const unsigned long long cands = __ballot(cand);
+            const int done = fail ? (__ffsll((long long)fail) - 1 - off) : __popcll(cands);
+            for (int t = 0; t < done; ++t) {
+                const uint32_t j = (uint32_t)__builtin_amdgcn_readlane((int)wv, off + t) & mask;
+                const int b = i - t;
+                const unsigned long long bi = (labels >> b) & 1ull, bj = (labels >> j) & 1ull;
+                if (bi != bj) labels ^= (1ull << b) | (1ull << j);
             }
-            pos = wbase + wlen;
+            i -= done;
+            pos += done + (fail ? 1 : 0);
         }
     }
 
@@ -209,6 +223,7 @@ __device__ double ln_rad_pdf(const gsx_stretch_model& md, double q) {
     }
     const double L = md.len_rescaled;
     double s;
+    if (md.kind < GSX_STRETCH_GAUSSIAN || md.kind > GSX_STRETCH_MATERN) return NAN;  // flagged as a NaN log-pdf
     if (md.kind == GSX_STRETCH_GAUSSIAN) {
         // c0 exp(-((k L / 2)^2)), c0 = (L / 2 / sqrt(pi))^d
         const double t = (r * L) / 2.0;
@@ -238,7 +253,8 @@ constexpr int kStretchNanLogProb = 2;   // log-probability NaN
 
 __global__ __launch_bounds__(kWave) void k_stretch_chains(int64_t R, int nw, const uint32_t* __restrict__ states,
                                                           int64_t burn_steps, const int64_t* __restrict__ prod_steps,
-                                                          int64_t max_steps, double a, gsx_stretch_model md,
+                                                          int64_t max_steps, double a,
+                                                          const gsx_stretch_model* __restrict__ models,
                                                           const double* __restrict__ init,
                                                           double* __restrict__ chain, double* __restrict__ final_x,
                                                           int* __restrict__ status) {
@@ -248,6 +264,7 @@ __global__ __launch_bounds__(kWave) void k_stretch_chains(int64_t R, int nw, con
     __shared__ double x_other[kWave];
     const int64_t r = blockIdx.x;
     if (r >= R) return;
+    const gsx_stretch_model md = models[r];
     const int l = lane_id();
     const bool walker = l < nw;
     double x = walker ? init[r * nw + l] : 0.0;
@@ -268,12 +285,7 @@ __global__ __launch_bounds__(kWave) void k_stretch_chains(int64_t R, int nw, con
         for (int64_t s = 0; s < steps; ++s) {
             mt.skip(2);  // the move choice (one random_sample, unused)
             unsigned long long labels = 0xaaaaaaaaaaaaaaaaull & all;  // walker w has label w % 2
-            mt.load_window();
-            for (int i = nw - 1; i >= 1; --i) {
-                const uint32_t j = mt.masked_one((uint32_t)i);
-                const unsigned long long bi = (labels >> i) & 1ull, bj = (labels >> j) & 1ull;
-                if (bi != bj) labels ^= (1ull << i) | (1ull << j);
-            }
+            mt.shuffle_labels(labels, nw);
             for (int split = 0; split < 2; ++split) {
                 const unsigned long long own = split == 0 ? (~labels & all) : labels;
                 const unsigned long long oth = all & ~own;
@@ -329,17 +341,15 @@ __global__ void k_stretch_pick(int64_t total, const int64_t* __restrict__ owner,
 extern "C" {
 
 int gsx_stretch_run_device(int64_t R, int32_t nwalkers, const uint32_t* states, int64_t burn_steps,
-                           const int64_t* prod_steps, int64_t max_steps, double a, const gsx_stretch_model* model,
-                           const double* init, double* chain, double* final_coords, int* status, void* stream) {
-    if (R < 0 || nwalkers < 2 || nwalkers > 64 || burn_steps < 0 || max_steps < 0 || !model) return GSX_E_INVALID;
-    if (model->kind < GSX_STRETCH_GAUSSIAN || model->kind > GSX_STRETCH_MATERN || model->dim < 1 ||
-        model->dim > 3)
-        return GSX_E_INVALID;
+                           const int64_t* prod_steps, int64_t max_steps, double a,
+                           const gsx_stretch_model* models, const double* init, double* chain,
+                           double* final_coords, int* status, void* stream) {
+    if (R < 0 || nwalkers < 2 || nwalkers > 64 || burn_steps < 0 || max_steps < 0) return GSX_E_INVALID;
     if (R == 0) return GSX_OK;
-    if (!states || !prod_steps || !init || !status) return GSX_E_INVALID;
+    if (!models || !states || !prod_steps || !init || !status) return GSX_E_INVALID;
     hipStream_t s = gsx::as_stream(stream);
     hipLaunchKernelGGL(gsx::k_stretch_chains, dim3((unsigned)R), dim3(gsx::kWave), 0, s, R, (int)nwalkers, states,
-                       burn_steps, prod_steps, max_steps, a, *model, init, chain, final_coords, status);
+                       burn_steps, prod_steps, max_steps, a, models, init, chain, final_coords, status);
     return gsx::launch_status();
 }
 

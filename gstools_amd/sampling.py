@@ -43,7 +43,7 @@ import torch
 from . import _native
 from .rng import RNG
 
-__all__ = ["sample_ln_pdf_batch", "sample_modes_batch", "device_model"]
+__all__ = ["sample_ln_pdf_batch", "sample_modes_batch", "sample_modes_many", "device_model"]
 
 _MT_WORDS = 625  # 624 key words + position
 
@@ -191,19 +191,24 @@ class _DeviceChains:
     """The chains of one ``gsx_stretch_run_device`` launch (running on the
     current stream; :meth:`pick` waits for it)."""
 
-    def __init__(self, md, init, burn, prod, burn_in, prod_steps, a=2.0):
+    def __init__(self, models, init, burn, prod, burn_in, prod_steps, a=2.0):
+        """``models``: one ``gsx_stretch_model`` for all chains or one per chain."""
         dev = torch.device("cuda", torch.cuda.current_device())
         nr, self.nw = init.shape
+        if isinstance(models, _native.StretchModel):
+            models = [models] * nr
+        table = (_native.StretchModel * nr)(*models)
         self.max_steps = max(int(prod_steps.max(initial=0)), 1)
         states = np.ascontiguousarray(np.stack([burn, prod], axis=1), dtype=np.uint32)
         self._keep = [torch.from_numpy(states.view(np.int32)).to(dev),
                       torch.from_numpy(np.ascontiguousarray(prod_steps, dtype=np.int64)).to(dev),
-                      torch.from_numpy(np.ascontiguousarray(init, dtype=np.double)).to(dev)]
+                      torch.from_numpy(np.ascontiguousarray(init, dtype=np.double)).to(dev),
+                      torch.frombuffer(bytearray(bytes(table)), dtype=torch.uint8).to(dev)]
         self.chain = torch.empty((nr, self.max_steps, self.nw), dtype=torch.float64, device=dev)
         self.status = torch.zeros(1, dtype=torch.int32, device=dev)
-        st, steps_t, init_t = self._keep
+        st, steps_t, init_t, models_t = self._keep
         _native.check(_native.lib().gsx_stretch_run_device(nr, self.nw, _tp(st), int(burn_in), _tp(steps_t),
-                                                           self.max_steps, float(a), ctypes.byref(md), _tp(init_t),
+                                                           self.max_steps, float(a), _tp(models_t), _tp(init_t),
                                                            _tp(self.chain), None, _tp(self.status), _stream()),
                       "stretch_run_device")
 
@@ -292,13 +297,44 @@ def _check_init(init):
             raise ValueError("Initial state has walkers that are not linearly independent")
 
 
-def _sphere(dim, ang1, ang2):
-    """``RNG.sample_sphere`` from its uniform draws (``rng.py:142-193``)."""
+def _sphere_chunk(dim, ang1, ang2, out):
+    """``RNG.sample_sphere`` (``rng.py:142-193``) from its uniform draws, into out (dim, n)."""
     if dim == 1:
-        return ang1.reshape(1, -1).copy()
-    if dim == 2:
-        return np.array([np.cos(ang1), np.sin(ang1)])
-    return np.array([np.sqrt(1.0 - ang2**2) * np.cos(ang1), np.sqrt(1.0 - ang2**2) * np.sin(ang1), ang2])
+        out[0] = ang1
+    elif dim == 2:
+        out[0] = np.cos(ang1)
+        out[1] = np.sin(ang1)
+    else:
+        out[0] = np.sqrt(1.0 - ang2**2) * np.cos(ang1)
+        out[1] = np.sqrt(1.0 - ang2**2) * np.sin(ang1)
+        out[2] = ang2
+
+
+_CHUNK = 1 << 16
+_POOL = None
+
+
+def _chunked(fn, n, threads):
+    """Run fn(lo, hi) over [0, n) in chunks on a thread pool (numpy's elementwise
+    loops release the GIL and give every element the same result in any chunk)."""
+    global _POOL
+    bounds = [(lo, min(n, lo + _CHUNK)) for lo in range(0, n, _CHUNK)]
+    if threads <= 1 or len(bounds) <= 1:
+        for lo, hi in bounds:
+            fn(lo, hi)
+        return
+    if _POOL is None or _POOL._max_workers < threads:
+        from concurrent.futures import ThreadPoolExecutor
+
+        _POOL = ThreadPoolExecutor(max_workers=threads)
+    list(_POOL.map(lambda b: fn(*b), bounds))
+
+
+def _sphere(dim, ang1, ang2, threads=1):
+    out = np.empty((dim, ang1.size))
+    _chunked(lambda lo, hi: _sphere_chunk(dim, ang1[lo:hi], None if ang2 is None else ang2[lo:hi], out[:, lo:hi]),
+             ang1.size, threads)
+    return out
 
 
 def _native_seed(seed):
@@ -317,70 +353,154 @@ def sample_modes_batch(model, mode_nos, seeds, sampling="auto", nthreads=None, d
     has a device log-pdf, :func:`device_model`), while the host finishes the
     directions.  Other seeds take the one-generator-at-a-time draws.
     """
+    return sample_modes_many([(model, mode_nos, seeds)], sampling=sampling, nthreads=nthreads, device=device,
+                             nwalkers=nwalkers, burn_in=burn_in, oversampling_factor=oversampling_factor)[0]
+
+
+def sample_modes_many(groups, sampling="auto", nthreads=None, device=None, nwalkers=50, burn_in=20,
+                      oversampling_factor=10):
+    """:func:`sample_modes_batch` for several ``(model, mode_nos, seeds)`` groups:
+    the MCMC chains of every group on the device run in one launch while the
+    host draws and finishes all groups."""
+    if sampling not in ("auto", "inversion", "mcmc"):
+        raise ValueError("sample_modes_batch: sampling not in ['auto', 'inversion', 'mcmc']")
+    threads = len(os.sched_getaffinity(0)) if nthreads is None else int(nthreads)
+    # groups whose chains run on the device are drawn first, so that the
+    # chains launch before the host draws the others
+    on_device = [_kind(model, sampling) == 2 and _pick_device_model(model, 2, device) is not None
+                 for model, _, _ in groups]
+    preps = [None] * len(groups)
+    for first in (True, False):
+        for i, (model, mode_nos, seeds) in enumerate(groups):
+            if on_device[i] == first:
+                preps[i] = _prepare(model, mode_nos, seeds, sampling, device, nwalkers, burn_in,
+                                    oversampling_factor, threads)
+        if first:
+            chains, dev_groups = _launch(preps, burn_in)
+    out = [None] * len(preps)
+    for i, ((model, mode_nos, seeds), p) in enumerate(zip(groups, preps)):
+        if p is None:
+            continue
+        if not p["n"].size:
+            out[i] = []
+            continue
+        if p["kind"] == 2 and p["md"] is None:
+            p["rads"] = _chains_host(model.ln_spectral_rad_pdf, p["init"], p["burn"], p["prod"], burn_in,
+                                     p["prod_steps"], p["picks"], nthreads=threads)
+        out[i] = _finish(model, p)
+    if chains is not None:
+        rads = chains.pick([pk for p in dev_groups for pk in p["picks"]])
+        k = 0
+        for i, p in enumerate(preps):
+            if p is not None and p.get("md") is not None and p["n"].size:
+                nr = p["init"].shape[0]
+                out[i] = [(r * s, a, b) for r, (s, a, b) in zip(rads[k:k + nr], out[i])]
+                k += nr
+    for i, ((model, mode_nos, seeds), p) in enumerate(zip(groups, preps)):
+        if p is None:
+            out[i] = _modes_python(model, [int(n) for n in mode_nos], list(seeds), _kind(model, sampling),
+                                   _pick_device_model(model, _kind(model, sampling), device), nwalkers, burn_in,
+                                   oversampling_factor, threads)
+    return out
+
+
+def _launch(preps, burn_in):
+    """One chains launch for every prepared device group -> (chains, groups)."""
+    dev_groups = [p for p in preps if p is not None and p.get("md") is not None and p["n"].size]
+    if not dev_groups:
+        return None, []
+    init = np.concatenate([p["init"] for p in dev_groups])
+    burn = np.concatenate([p["burn"] for p in dev_groups])
+    prod = np.concatenate([p["prod"] for p in dev_groups])
+    steps = np.concatenate([p["prod_steps"] for p in dev_groups])
+    models = [p["md"] for p in dev_groups for _ in range(p["init"].shape[0])]
+    return _DeviceChains(models, init, burn, prod, burn_in, steps), dev_groups
+
+
+def _kind(model, sampling):
+    if np.isclose(model.var, 0):
+        return 0
+    return 1 if sampling == "inversion" or (sampling == "auto" and model.has_ppf) else 2
+
+
+def _pick_device_model(model, kind, device):
+    if kind != 2 or device is False:
+        return None
+    md = device_model(model)
+    if md is None and device:
+        raise ValueError(f"sample_modes_batch: no device log-pdf for {type(model).__name__}")
+    if md is not None and device is None and not torch.cuda.is_available():
+        return None
+    return md
+
+
+def _prepare(model, mode_nos, seeds, sampling, device, nwalkers, burn_in, oversampling_factor, threads):
+    """Native draws of one group (None: seeds the native draws do not take)."""
     mode_nos = [int(n) for n in mode_nos]
     seeds = list(seeds)
     if len(mode_nos) != len(seeds):
         raise ValueError("sample_modes_batch: one seed per mode_no")
-    if sampling not in ("auto", "inversion", "mcmc"):
-        raise ValueError("sample_modes_batch: sampling not in ['auto', 'inversion', 'mcmc']")
-    if not mode_nos:
-        return []
     dim = model.dim
-    zero_var = bool(np.isclose(model.var, 0))
-    inversion = sampling == "inversion" or (sampling == "auto" and model.has_ppf)
-    kind = 0 if zero_var else (1 if inversion else 2)
-    md = None
-    if kind == 2 and (device is None or device):
-        md = device_model(model)
-        if md is None and device:
-            raise ValueError(f"sample_modes_batch: no device log-pdf for {type(model).__name__}")
-        if md is not None and device is None and not torch.cuda.is_available():
-            md = None
-    threads = len(os.sched_getaffinity(0)) if nthreads is None else int(nthreads)
-    if not all(_native_seed(s) for s in seeds) or dim > 3:
-        return _modes_python(model, mode_nos, seeds, kind, md, nwalkers, burn_in, oversampling_factor, threads)
+    kind = _kind(model, sampling)
+    md = _pick_device_model(model, kind, device)
+    if not all(_native_seed(x) for x in seeds) or dim > 3:
+        return None
     nr = len(mode_nos)
     n = np.asarray(mode_nos, dtype=np.int64)
     total = int(n.sum())
-    z1, z2 = np.empty(total), np.empty(total)
-    ang1 = np.empty(total) if kind else None
-    ang2 = np.empty(total) if kind and dim == 3 else None
-    u_inv = np.empty(total) if kind == 1 else None
+    p = {"kind": kind, "dim": dim, "n": n, "md": md, "cuts": np.cumsum(n)[:-1], "threads": threads}
+    p["z1"], p["z2"] = np.empty(total), np.empty(total)
+    p["ang1"] = np.empty(total) if kind else None
+    p["ang2"] = np.empty(total) if kind and dim == 3 else None
+    p["u_inv"] = np.empty(total) if kind == 1 else None
     mc = kind == 2
-    prod_steps = _prod_steps(mode_nos, nwalkers, burn_in, oversampling_factor) if mc else None
-    n_flat = prod_steps * nwalkers if mc else None
-    init = np.empty((nr, nwalkers)) if mc else None
-    states = np.empty((nr, 2, _MT_WORDS), dtype=np.uint32) if mc else None
-    picks = np.empty(total, dtype=np.int64) if mc else None
-    around = 1.0 / model.len_rescaled if mc else 0.0
-    seeds_a = np.asarray([int(s) for s in seeds], dtype=np.int64)
-    opt = lambda a: None if a is None else _vp(a)
-    _native.check(_native.lib().gsx_mode_draws(nr, _vp(seeds_a), _vp(n), dim, kind, nwalkers, float(around),
-                                               opt(n_flat), _vp(z1), _vp(z2), opt(ang1), opt(ang2), opt(u_inv),
-                                               opt(init), opt(states), opt(picks), threads), "mode_draws")
-    cuts = np.cumsum(n)[:-1]
-    z1s, z2s = np.split(z1, cuts), np.split(z2, cuts)
-    if kind == 0:
-        return [(np.full((dim, k), 0.0), a, b) for k, a, b in zip(mode_nos, z1s, z2s)]
-    chains = None
     if mc:
-        _check_init(init)
-        burn, prod = np.ascontiguousarray(states[:, 0]), np.ascontiguousarray(states[:, 1])
-        if md is not None and nwalkers <= 64:
-            chains = _DeviceChains(md, init, burn, prod, burn_in, prod_steps)  # runs while the host continues
-    sph = _sphere(dim, ang1, ang2)
+        p["prod_steps"] = _prod_steps(mode_nos, nwalkers, burn_in, oversampling_factor)
+        n_flat = p["prod_steps"] * nwalkers
+        p["init"] = np.empty((nr, nwalkers))
+        states = np.empty((nr, 2, _MT_WORDS), dtype=np.uint32)
+        picks = np.empty(total, dtype=np.int64)
+    else:
+        n_flat = states = picks = None
+    around = 1.0 / model.len_rescaled if mc else 0.0
+    seeds_a = np.asarray([int(x) for x in seeds], dtype=np.int64)
+    opt = lambda a: None if a is None else _vp(a)
+    if nr:
+        _native.check(_native.lib().gsx_mode_draws(nr, _vp(seeds_a), _vp(n), dim, kind, nwalkers, float(around),
+                                                   opt(n_flat), _vp(p["z1"]), _vp(p["z2"]), opt(p["ang1"]),
+                                                   opt(p["ang2"]), opt(p["u_inv"]), opt(p.get("init")), opt(states),
+                                                   opt(picks), threads), "mode_draws")
+    if mc:
+        _check_init(p["init"])
+        p["burn"], p["prod"] = np.ascontiguousarray(states[:, 0]), np.ascontiguousarray(states[:, 1])
+        p["picks"] = np.split(picks, p["cuts"]) if nr else []
+    return p
+
+
+def _finish(model, p):
+    """Modes of a prepared group; MCMC groups on the device still lack their radii
+    (returned as (directions, z1, z2) for the caller to scale)."""
+    kind, dim, cuts = p["kind"], p["dim"], p["cuts"]
+    if not p["n"].size:
+        return []
+    z1s, z2s = np.split(p["z1"], cuts), np.split(p["z2"], cuts)
+    if kind == 0:
+        return [(np.full((dim, k), 0.0), a, b) for k, a, b in zip(p["n"], z1s, z2s)]
+    threads = p["threads"]
+    sphs = np.split(_sphere(dim, p["ang1"], p["ang2"], threads), cuts, axis=1)
     if kind == 1:
         pdf, cdf, ppf = model.dist_func
-        rad = np.asarray(ppf(u_inv), dtype=np.double) * 1.0 + 0.0
+        rad = np.empty(p["u_inv"].size)
+
+        def inv(lo, hi):
+            rad[lo:hi] = np.asarray(ppf(p["u_inv"][lo:hi]), dtype=np.double) * 1.0 + 0.0
+
+        _chunked(inv, rad.size, threads)
         rads = np.split(rad, cuts)
+    elif p["md"] is None:
+        rads = p["rads"]
     else:
-        pick_list = np.split(picks, cuts)
-        if chains is not None:
-            rads = chains.pick(pick_list)
-        else:
-            rads = _chains_host(model.ln_spectral_rad_pdf, init, burn, prod, burn_in, prod_steps, pick_list,
-                                nthreads=threads)
-    sphs = np.split(sph, cuts, axis=1)
+        return list(zip(sphs, z1s, z2s))
     return [(r * s, a, b) for r, s, a, b in zip(rads, sphs, z1s, z2s)]
 
 

@@ -434,7 +434,9 @@ typedef struct gsx_stretch_model {
  * nwalkers <= 64 walkers: one wavefront per chain runs burn_steps steps on
  * the chain's first MT19937 stream, then prod_steps[r] steps on its second,
  * recording the walkers after every production step in chain (R, max_steps,
- * nwalkers) (may be NULL).  states (R, 2, 625) uint32: the legacy numpy key
+ * nwalkers) (may be NULL).  models: DEVICE array of R gsx_stretch_model, the
+ * log-pdf of each chain (an unknown kind reads as a NaN log-probability).
+ * states (R, 2, 625) uint32: the legacy numpy key
  * (624 words) and position of the burn-in and production streams; init
  * (R, nwalkers) the initial walkers.  The draws, their order and the move's
  * arithmetic are those of gsx_stretch_draws / _propose / _accept; the
@@ -444,7 +446,7 @@ typedef struct gsx_stretch_model {
 int gsx_stretch_run_device(int64_t R, int32_t nwalkers, const uint32_t* states,
                            int64_t burn_steps, const int64_t* prod_steps,
                            int64_t max_steps, double a,
-                           const gsx_stretch_model* model, const double* init,
+                           const gsx_stretch_model* models, const double* init,
                            double* chain, double* final_coords, int* status,
                            void* stream);
 /* out[i] = chain[owner[i]][idx[i]] (flat index step * nwalkers + walker) for

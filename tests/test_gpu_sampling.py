@@ -33,6 +33,7 @@ def _say(msg):
 
 
 def _same(a, b):
+    assert len(a) == len(b)
     for x, y in zip(a, b):
         for u, v in zip(x, y):
             assert np.array_equal(u, v)
@@ -126,3 +127,22 @@ def test_device_batch_workload():
         total_host += time.perf_counter() - t0
         _same(host, dev)
     _say(f"  device {total_dev:.3f} s, host lockstep {total_host:.3f} s")
+
+
+def test_device_many_models_one_launch():
+    """Groups of different models in one chains launch: each group's modes
+    equal its own host sampling bit for bit (the batch workload's three kinds)."""
+    from bench import _batch_specs, _model_of
+    from gstools_amd.sampling import sample_modes_many
+
+    specs = _batch_specs()
+    kinds = ("grid2d", "points3d", "incompr3d")
+    groups = []
+    for kind in kinds:
+        idx = [i for i, sp in enumerate(specs) if sp[0] == kind][:20]
+        groups.append((_model_of(kind), [specs[i][1] for i in idx], [specs[i][2] for i in idx]))
+    groups.append((Matern(dim=3, var=1.0, len_scale=8.0, nu=1.5), [300, 50], [1, 2]))
+    _say("device sampling: four groups, one launch")
+    dev = sample_modes_many(groups, device=None)
+    for (model, ns, ss), got in zip(groups, dev):
+        _same(sample_modes_batch(model, ns, ss, device=False), got)
