@@ -23,6 +23,8 @@ NOALLOC_SCRATCH_BYTES = 32 << 20  # GSX_NOALLOC_SCRATCH_BYTES
 MFMA_BF16X3 = 0
 MFMA_FP16X3 = 1
 MFMA_UNIFORM_LAST = 16  # flag: the last axis of a 2D grid is uniformly spaced
+MFMA_FILL_SHIFT = 8     # tuning: log2 of the K-split fill target (8..11) in bits 8-11
+MFMA_NO_SWAP = 0x1000   # tuning: never swap axes 0 and 1 of a 3D grid
 
 _vp = ctypes.c_void_p
 _i64 = ctypes.c_int64
@@ -76,6 +78,7 @@ EXPORTS = {
     "gsx_summate_grid_f64": (_i32, _GRID),
     "gsx_summate_grid_f32": (_i32, _GRID),
     "gsx_summate_grid_mfma_workspace": (_i64, [_i64, _i64, _i64p]),
+    "gsx_summate_grid_mfma_plan": (_i32, [_i64, _i64, _i64p, _i64, _i32, _i64p]),
     "gsx_summate_grid_mfma": (_i32, [_i64, _i64, _i64p, _vp, _vp, _vp, _vp, _i32, _f64, _vp, _vp, _i64, _vp]),
     "gsx_summate_incompr_grid_f64": (_i32, _GRID),
     "gsx_summate_incompr_grid_f32": (_i32, _GRID),

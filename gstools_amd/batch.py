@@ -9,9 +9,6 @@ chunk) on a workload and caches the winner per device and workload shape.
 """
 
 import ctypes
-import json
-import os
-import time
 
 import numpy as np
 import torch
@@ -119,7 +116,9 @@ class MixedBatch:
     :class:`BatchPlan`.  The graph holds kernel launches only (no memset nodes).
     """
 
-    def __init__(self, items, precision="bf16x3", tile_points=0, mode_chunk=0):
+    def __init__(self, items, precision="bf16x3", tile_points=0, mode_chunk=0, tune=True):
+        """``tune``: each grid item runs the matrix-core plan the autotuner chose
+        for its class (timed on first use, then cached; see :mod:`.autotune`)."""
         if precision not in ("bf16x3", "fp16x3"):
             raise ValueError("precision must be 'bf16x3' or 'fp16x3'")
         self.items = list(items)
@@ -140,9 +139,37 @@ class MixedBatch:
             variant = self.variant
             if it.d == 2 and not it.incompr and uniform_axis(it.axes[it.shape[0]:it.shape[0] + it.shape[1]]):
                 variant |= _native.MFMA_UNIFORM_LAST  # factorised last axis
+            if tune:
+                from .autotune import mfma_variant
+
+                variant = mfma_variant(it.d, it.N, it.shape[:it.d], variant, comps=it.d if it.incompr else 1)
             self._calls.append((it, shape, variant))
+        self.tuned = bool(tune)
         self.ws = torch.empty(need, dtype=torch.uint8, device=self.items[0].out.device if self.items else "cuda")
         self.graph = None
+
+    def tuning_summary(self):
+        """Per grid class: the tuned choice, its time and the candidates' spread (ms)."""
+        from .autotune import _lookup, _mfma_key
+
+        out = {}
+        for it, _, variant in self._calls:
+            base = variant & (_native.MFMA_FP16X3 | _native.MFMA_UNIFORM_LAST)
+            key = _mfma_key(it.d, it.N, it.shape[:it.d], it.d if it.incompr else 1, base)
+            entry = _lookup(key)
+            if entry is None or key in out:
+                continue
+            ms = entry["ms"]
+            out[key.split("|", 1)[1]] = {"choice": entry["choice"], "ms": round(min(ms.values()), 4),
+                                         "default_ms": round(entry["default_ms"], 4),
+                                         "spread_ms": [round(min(ms.values()), 4), round(max(ms.values()), 4)],
+                                         "candidates": entry["candidates"], "items": 0}
+        for it, _, variant in self._calls:
+            base = variant & (_native.MFMA_FP16X3 | _native.MFMA_UNIFORM_LAST)
+            k = _mfma_key(it.d, it.N, it.shape[:it.d], it.d if it.incompr else 1, base).split("|", 1)[1]
+            if k in out:
+                out[k]["items"] += 1
+        return out
 
     def _run(self):
         lib = _native.lib()
@@ -170,53 +197,14 @@ class MixedBatch:
         self.graph.replay()
 
 
-def _cache_path():
-    return os.environ.get("GSX_AUTOTUNE_CACHE",
-                          os.path.join(os.path.expanduser("~"), ".cache", "gstools_amd", "autotune.json"))
-
-
-def _workload_key(items):
-    work = sum(it.P * it.N * (it.d if it.incompr else 1) for it in items)
-    dims = sorted({(it.d, int(it.incompr), it.axes is not None) for it in items})
-    name = torch.cuda.get_device_name()
-    return f"{name}|batch|{dims}|work~2^{int(np.log2(max(work, 1)))}|items~2^{int(np.log2(max(len(items), 1)))}"
-
-
 def autotune(items, reps=3, graph=True, use_cache=True, verbose=False):
-    """Time every (tile_points, mode_chunk) variant; return (best, timings_ms).
+    """Time every (tile_points, mode_chunk) variant of the persistent plan on
+    ``items``; return (best, timings_ms).  Cached by :mod:`.autotune` under the
+    device architecture, the item kinds, the mean mode count and the work."""
+    from .autotune import tune_batch
 
-    Results are cached in ``$GSX_AUTOTUNE_CACHE`` (default
-    ``~/.cache/gstools_amd/autotune.json``) keyed by device and workload shape.
-    """
-    key = _workload_key(items)
-    path = _cache_path()
-    cache = {}
-    if use_cache and os.path.exists(path):
-        try:
-            cache = json.load(open(path))
-        except (OSError, ValueError):
-            cache = {}
-        if key in cache:
-            return tuple(cache[key]["best"]), cache[key]["ms"]
-    timings = {}
-    for tp, mc in TILE_VARIANTS:
-        plan = BatchPlan(items, tp, mc)
-        plan.launch(graph)
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        for _ in range(reps):
-            plan.launch(graph)
-        torch.cuda.synchronize()
-        timings[f"{tp}x{mc}"] = (time.perf_counter() - t0) / reps * 1e3
-        plan.close()
-        if verbose:
-            print(f"autotune {tp}x{mc}: {timings[f'{tp}x{mc}']:.3f} ms", flush=True)
-    best = min(TILE_VARIANTS, key=lambda v: timings[f"{v[0]}x{v[1]}"])
-    if use_cache:
-        cache[key] = {"best": list(best), "ms": timings}
-        try:
-            os.makedirs(os.path.dirname(path), exist_ok=True)
-            json.dump(cache, open(path, "w"), indent=1)
-        except OSError:
-            pass
-    return best, timings
+    best, timings = tune_batch(items, TILE_VARIANTS, reps=reps, graph=graph, force=not use_cache)
+    if verbose:
+        for name, ms in timings.items():
+            print(f"autotune {name}: {ms:.3f} ms", flush=True)
+    return tuple(best), timings

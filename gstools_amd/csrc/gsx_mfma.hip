@@ -86,26 +86,28 @@ inline MfmaGeom mfma_geom(int64_t d, int64_t N, const int64_t* shape) {
     return g;
 }
 
-// Blocks resident at once (256 CUs x 2 blocks at 2 waves per SIMD): grids with
-// fewer tiles split the K steps over floor(512 / blocks) blocks (at least 8 steps
-// = 128 modes each), so the launch stays one round with no tail, and add the
-// partial sums afterwards.
+// Blocks resident at once (256 CUs x 2 blocks at 2 waves per SIMD): by default
+// grids with fewer tiles split the K steps over floor(512 / blocks) blocks (at
+// least 8 steps = 128 modes each), so the launch stays one round with no tail,
+// and add the partial sums afterwards.  The fill target is a tuning choice
+// (GSX_MFMA_FILL_LOG2: 256 .. 2048 blocks).
 constexpr int64_t kMFillBlocks = 512;
+constexpr int kMFillLog2Min = 8, kMFillLog2Max = 11;
 
 // (>= 1 also with no K steps or no tiles: those launch nothing)
-inline int64_t mfma_steps_per_split(const MfmaGeom& g, int64_t comps) {
+inline int64_t mfma_steps_per_split(const MfmaGeom& g, int64_t comps, int64_t fill = kMFillBlocks) {
     const int64_t blocks = comps * g.row_tiles * g.col_tiles;
     const int64_t steps = g.Np / kMKM;
     if (steps < 1 || blocks < 1) return 1;
-    int64_t k = blocks >= kMFillBlocks ? 1 : kMFillBlocks / blocks;
+    int64_t k = blocks >= fill ? 1 : fill / blocks;
     if (k > steps / 8) k = steps / 8;
     if (k < 1) k = 1;
     return ceil_div(steps, k);
 }
 
-inline int64_t mfma_ksplit(const MfmaGeom& g, int64_t comps) {
+inline int64_t mfma_ksplit(const MfmaGeom& g, int64_t comps, int64_t fill = kMFillBlocks) {
     const int64_t steps = g.Np / kMKM;
-    return steps < 1 ? 1 : ceil_div(steps, mfma_steps_per_split(g, comps));
+    return steps < 1 ? 1 : ceil_div(steps, mfma_steps_per_split(g, comps, fill));
 }
 
 // A hi / lo (16-bit), E1 and one E2 table per output component (complex FP32)
@@ -114,8 +116,8 @@ inline int64_t mfma_table_bytes(const MfmaGeom& g, int64_t comps) {
 }
 
 // tables, then (K split) the FP32 partial sums, 256-byte aligned
-inline int64_t mfma_ws_bytes(const MfmaGeom& g, int64_t comps = 1) {
-    const int64_t ks = mfma_ksplit(g, comps);
+inline int64_t mfma_ws_bytes(const MfmaGeom& g, int64_t comps = 1, int64_t fill = kMFillBlocks) {
+    const int64_t ks = mfma_ksplit(g, comps, fill);
     const int64_t tables = ceil_div(mfma_table_bytes(g, comps), (int64_t)256) * 256;
     return ks > 1 ? tables + ks * comps * g.n0 * g.out_cols * 4 : mfma_table_bytes(g, comps);
 }
@@ -138,8 +140,8 @@ inline MfmaGeom mfma_split_geom(int64_t N, const int64_t* shape) {
 }
 
 // split workspace: tables and partial sums, then the two virtual axes
-inline int64_t mfma_split_ws_bytes(const MfmaGeom& gv) {
-    return ceil_div(mfma_ws_bytes(gv), (int64_t)256) * 256 + (gv.n1 + 16) * 8;
+inline int64_t mfma_split_ws_bytes(const MfmaGeom& gv, int64_t fill = kMFillBlocks) {
+    return ceil_div(mfma_ws_bytes(gv, 1, fill), (int64_t)256) * 256 + (gv.n1 + 16) * 8;
 }
 
 // The GEMM's row axis is axis 0, in 128-row tiles.  A 3D grid with a short
@@ -153,18 +155,21 @@ struct MfmaPlan {
     MfmaGeom g;
     bool split;    // 2D grid, uniform last axis factorised (k_split_axis)
     bool swapped;  // 3D grid run with axes 0 and 1 swapped
+    int64_t fill;  // K-split fill target (resident blocks)
 };
 
 inline double mfma_tile_cost(const MfmaGeom& g) {
     return (double)g.row_tiles * (double)g.col_tiles * (g.g2 == 1 ? 1.0 : 2.2);
 }
 
-inline MfmaPlan mfma_plan(int64_t d, int64_t N, const int64_t* shape, int64_t comps, bool uniform_last) {
+inline MfmaPlan mfma_plan(int64_t d, int64_t N, const int64_t* shape, int64_t comps, bool uniform_last,
+                          int64_t fill = kMFillBlocks, bool allow_swap = true) {
     MfmaPlan p;
     p.split = uniform_last && d == 2 && comps == 1 && shape[1] >= kSplitMin;
     p.swapped = false;
+    p.fill = fill;
     p.g = p.split ? mfma_split_geom(N, shape) : mfma_geom(d, N, shape);
-    if (d == 3 && shape[0] < shape[1]) {
+    if (allow_swap && d == 3 && shape[0] < shape[1]) {
         const int64_t sw[3] = {shape[1], shape[0], shape[2]};
         MfmaGeom gs = mfma_geom(3, N, sw);
         if (mfma_tile_cost(gs) < 0.8 * mfma_tile_cost(p.g)) {
@@ -178,7 +183,20 @@ inline MfmaPlan mfma_plan(int64_t d, int64_t N, const int64_t* shape, int64_t co
 }
 
 inline int64_t mfma_plan_ws_bytes(const MfmaPlan& p, int64_t comps) {
-    return p.split ? mfma_split_ws_bytes(p.g) : mfma_ws_bytes(p.g, comps);
+    return p.split ? mfma_split_ws_bytes(p.g, p.fill) : mfma_ws_bytes(p.g, comps, p.fill);
+}
+
+// Workspace enough for every tuning choice (fill target, swap) of a grid.
+inline int64_t mfma_ws_any(int64_t d, int64_t N, const int64_t* shape, int64_t comps, bool uniform_last) {
+    int64_t ws = 0;
+    for (int f = kMFillLog2Min; f <= kMFillLog2Max; ++f)
+        for (int sw = 0; sw < 2; ++sw) {
+            const int64_t w = mfma_plan_ws_bytes(mfma_plan(d, N, shape, comps, uniform_last, (int64_t)1 << f,
+                                                           sw == 1),
+                                                 comps);
+            if (w > ws) ws = w;
+        }
+    return ws;
 }
 
 // ---------------------------------------------------------------- splits ---
@@ -686,7 +704,7 @@ __global__ __launch_bounds__(256) void k_split_axis(const double* __restrict__ y
 }
 
 template <bool FP16>
-int mfma_launch(const MfmaGeom& g, int64_t d, int64_t N, int64_t comps, int64_t ka_row, int64_t k1_row,
+int mfma_launch(const MfmaGeom& g, int64_t fill, int64_t d, int64_t N, int64_t comps, int64_t ka_row, int64_t k1_row,
                 int64_t kz_row, const double* ax0,
                 const double* ax1, const double* ax2, const double* cov, const double* z1, const double* z2,
                 const double* fac, double alpha, float* out, void* ws, hipStream_t s) {
@@ -699,8 +717,8 @@ int mfma_launch(const MfmaGeom& g, int64_t d, int64_t N, int64_t comps, int64_t 
                        N, comps, ka_row, k1_row, kz_row, ax0, ax1, ax2, cov, z1, z2, fac, a_hi, a_lo, e1, e2);
     int st = launch_status();
     if (st != GSX_OK) return st;
-    const int64_t steps_per = mfma_steps_per_split(g, comps);
-    const int64_t ksplit = mfma_ksplit(g, comps);
+    const int64_t steps_per = mfma_steps_per_split(g, comps, fill);
+    const int64_t ksplit = mfma_ksplit(g, comps, fill);
     const int64_t blocks = ksplit * comps * g.row_tiles * g.col_tiles;
     const int64_t stride = g.n0 * g.out_cols;  // output values per component
     float* partial = nullptr;
@@ -745,8 +763,12 @@ int mfma_entry(int64_t d, int64_t N, const int64_t* shape, const double* axes, c
     if (d == 1) return GSX_E_UNSUPPORTED;
     if ((d != 2 && d != 3) || N < 0 || !shape) return GSX_E_INVALID;
     const bool uniform_last = (variant & GSX_MFMA_UNIFORM_LAST) != 0;
-    variant &= ~GSX_MFMA_UNIFORM_LAST;
+    const bool no_swap = (variant & GSX_MFMA_NO_SWAP) != 0;
+    const int fill_log2 = (variant >> GSX_MFMA_FILL_SHIFT) & 0xf;
+    variant &= ~(GSX_MFMA_UNIFORM_LAST | GSX_MFMA_NO_SWAP | (0xf << GSX_MFMA_FILL_SHIFT));
     if (variant != GSX_MFMA_BF16X3 && variant != GSX_MFMA_FP16X3) return GSX_E_INVALID;
+    if (fill_log2 != 0 && (fill_log2 < kMFillLog2Min || fill_log2 > kMFillLog2Max)) return GSX_E_INVALID;
+    const int64_t fill = fill_log2 ? ((int64_t)1 << fill_log2) : kMFillBlocks;
     int64_t P = 1;
     for (int i = 0; i < d; ++i) {
         if (shape[i] < 0) return GSX_E_INVALID;
@@ -760,10 +782,10 @@ int mfma_entry(int64_t d, int64_t N, const int64_t* shape, const double* axes, c
                            comps * P);
         return launch_status();
     }
-    const MfmaPlan plan = mfma_plan(d, N, shape, comps, uniform_last);
+    const MfmaPlan plan = mfma_plan(d, N, shape, comps, uniform_last, fill, !no_swap);
     const MfmaGeom& g = plan.g;
     const bool split = plan.split;
-    if (mfma_ksplit(g, comps) * comps * g.row_tiles * g.col_tiles > 0x7fffffffLL ||
+    if (mfma_ksplit(g, comps, fill) * comps * g.row_tiles * g.col_tiles > 0x7fffffffLL ||
         ceil_div(comps * P, (int64_t)256) > 0x7fffffffLL)
         return GSX_E_INVALID;
     if (!workspace || workspace_bytes < mfma_plan_ws_bytes(plan, comps)) return GSX_E_WORKSPACE;
@@ -779,7 +801,7 @@ int mfma_entry(int64_t d, int64_t N, const int64_t* shape, const double* axes, c
     }
     if (split) {
         double* vy = reinterpret_cast<double*>(static_cast<char*>(workspace) +
-                                               ceil_div(mfma_ws_bytes(g), (int64_t)256) * 256);
+                                               ceil_div(mfma_ws_bytes(g, 1, fill), (int64_t)256) * 256);
         double* vz = vy + g.n1;
         hipLaunchKernelGGL(k_split_axis, dim3(1), dim3(256), 0, s, ax1, shape[1], vy, vz);
         const int st = launch_status();
@@ -790,10 +812,10 @@ int mfma_entry(int64_t d, int64_t N, const int64_t* shape, const double* axes, c
         kz_row = 1;  // both virtual axes carry the mode's k_1
     }
     if (variant == GSX_MFMA_FP16X3)
-        return mfma_launch<true>(g, dk, N, comps, ka_row, k1_row, kz_row, ax0, ax1, ax2, cov, z1, z2, fac, alpha,
-                                 out, workspace, s);
-    return mfma_launch<false>(g, dk, N, comps, ka_row, k1_row, kz_row, ax0, ax1, ax2, cov, z1, z2, fac, alpha,
-                              out, workspace, s);
+        return mfma_launch<true>(g, fill, dk, N, comps, ka_row, k1_row, kz_row, ax0, ax1, ax2, cov, z1, z2, fac,
+                                 alpha, out, workspace, s);
+    return mfma_launch<false>(g, fill, dk, N, comps, ka_row, k1_row, kz_row, ax0, ax1, ax2, cov, z1, z2, fac,
+                              alpha, out, workspace, s);
 }
 
 }  // namespace gsx
@@ -804,10 +826,32 @@ int64_t gsx_summate_grid_mfma_workspace(int64_t d, int64_t N, const int64_t* sha
     if ((d != 2 && d != 3) || N < 0 || !shape) return GSX_E_INVALID;
     for (int i = 0; i < d; ++i)
         if (shape[i] < 0) return GSX_E_INVALID;
-    const int64_t ws = gsx::mfma_plan_ws_bytes(gsx::mfma_plan(d, N, shape, 1, false), 1);
-    // enough for GSX_MFMA_UNIFORM_LAST as well
-    const int64_t ws_split = gsx::mfma_plan_ws_bytes(gsx::mfma_plan(d, N, shape, 1, true), 1);
+    // enough for GSX_MFMA_UNIFORM_LAST and every tuning choice as well
+    const int64_t ws = gsx::mfma_ws_any(d, N, shape, 1, false);
+    const int64_t ws_split = gsx::mfma_ws_any(d, N, shape, 1, true);
     return ws > ws_split ? ws : ws_split;
+}
+
+int gsx_summate_grid_mfma_plan(int64_t d, int64_t N, const int64_t* shape, int64_t comps, int variant,
+                               int64_t* info) {
+    if ((d != 2 && d != 3) || N < 0 || !shape || !info || (comps != 1 && comps != d)) return GSX_E_INVALID;
+    for (int i = 0; i < d; ++i)
+        if (shape[i] < 0) return GSX_E_INVALID;
+    const int fill_log2 = (variant >> GSX_MFMA_FILL_SHIFT) & 0xf;
+    if (fill_log2 != 0 && (fill_log2 < gsx::kMFillLog2Min || fill_log2 > gsx::kMFillLog2Max)) return GSX_E_INVALID;
+    const int64_t fill = fill_log2 ? ((int64_t)1 << fill_log2) : gsx::kMFillBlocks;
+    const bool uniform_last = (variant & GSX_MFMA_UNIFORM_LAST) != 0;
+    const gsx::MfmaPlan p = gsx::mfma_plan(d, N, shape, comps, uniform_last, fill, (variant & GSX_MFMA_NO_SWAP) == 0);
+    const int64_t ks = gsx::mfma_ksplit(p.g, comps, fill);
+    info[0] = p.swapped;
+    info[1] = p.split;
+    info[2] = p.g.row_tiles;
+    info[3] = p.g.col_tiles;
+    info[4] = ks;
+    info[5] = ks * comps * p.g.row_tiles * p.g.col_tiles;
+    info[6] = gsx::mfma_plan_ws_bytes(p, comps);
+    info[7] = p.g.g2 == 1 ? 3 : 2;  // 3: the 16 x 16 column-tile kernel, 2: flat columns
+    return GSX_OK;
 }
 
 int gsx_summate_grid_mfma(int64_t d, int64_t N, const int64_t* shape, const double* axes, const double* cov,
@@ -821,7 +865,7 @@ int64_t gsx_summate_incompr_grid_mfma_workspace(int64_t d, int64_t N, const int6
     if ((d != 2 && d != 3) || N < 0 || !shape) return GSX_E_INVALID;
     for (int i = 0; i < d; ++i)
         if (shape[i] < 0) return GSX_E_INVALID;
-    return gsx::mfma_plan_ws_bytes(gsx::mfma_plan(d, N, shape, d, false), d);
+    return gsx::mfma_ws_any(d, N, shape, d, false);
 }
 
 int gsx_summate_incompr_grid_mfma(int64_t d, int64_t N, const int64_t* shape, const double* axes, const double* cov,

@@ -184,7 +184,7 @@ def summate_incompr_device(cov_samples, z_1, z_2, pos, precision="fp64", out=Non
 
 
 def summate_grid_device(cov_samples, z_1, z_2, axes, precision="fp64", alpha=1.0, out=None,
-                        spectrum_factor=None):
+                        spectrum_factor=None, tune=False):
     """Sum on the 'ij' meshgrid of ``axes`` without materialising it.
 
     ``precision``: "fp64" / "fp32" (direct sums; "fp64" reproduces the phases
@@ -195,6 +195,8 @@ def summate_grid_device(cov_samples, z_1, z_2, axes, precision="fp64", alpha=1.0
     order, multiplied by ``alpha`` (split-precision matrix-core path only; the
     other paths return the unscaled sum and require alpha == 1).
     ``spectrum_factor`` (N,) weights each term (the Fourier method's sum).
+    ``tune``: on the matrix cores, run the plan the autotuner chose for this
+    grid class (timed on first use, then cached; :mod:`.autotune`).
     """
     dev = _device()
     cov, z1, z2 = _modes(cov_samples, z_1, z_2, dev)
@@ -245,6 +247,10 @@ def summate_grid_device(cov_samples, z_1, z_2, axes, precision="fp64", alpha=1.0
     variant = _native.MFMA_BF16X3 if precision == "bf16x3" else _native.MFMA_FP16X3
     if uniform_last:  # factorised last axis (no n x N phase table)
         variant |= _native.MFMA_UNIFORM_LAST
+    if tune and fac is None:
+        from .autotune import mfma_variant
+
+        variant = mfma_variant(d, n, shape, variant)
     ws_bytes = int(lib.gsx_summate_grid_mfma_workspace(d, n, shape_c))
     if ws_bytes < 0:
         _native.check(int(ws_bytes), "summate_grid_mfma")
@@ -260,14 +266,14 @@ def summate_grid_device(cov_samples, z_1, z_2, axes, precision="fp64", alpha=1.0
     return out
 
 
-def summate_incompr_grid_device(cov_samples, z_1, z_2, axes, precision="fp64", alpha=1.0, out=None):
+def summate_incompr_grid_device(cov_samples, z_1, z_2, axes, precision="fp64", alpha=1.0, out=None, tune=False):
     """Incompressible sum on the 'ij' meshgrid of ``axes`` -> tensor (d, prod(shape)).
 
     "fp64" / "fp32": the direct sum, one phase per (point, mode) for all d
     components ("fp64" gives the phases of the materialised grid bit for bit;
     the result is unscaled and ``alpha`` must be 1).  "bf16x3" / "fp16x3": one
     matrix-core GEMM per component in one launch, with the projector weight
-    per mode, multiplied by ``alpha``.
+    per mode, multiplied by ``alpha``; ``tune`` as in :func:`summate_grid_device`.
     """
     dev = _device()
     cov, z1, z2 = _modes(cov_samples, z_1, z_2, dev)
@@ -297,6 +303,10 @@ def summate_incompr_grid_device(cov_samples, z_1, z_2, axes, precision="fp64", a
                       "summate_incompr_grid")
         return out
     variant = _native.MFMA_BF16X3 if precision == "bf16x3" else _native.MFMA_FP16X3
+    if tune:
+        from .autotune import mfma_variant
+
+        variant = mfma_variant(d, n, shape, variant, comps=d)
     ws_bytes = int(lib.gsx_summate_incompr_grid_mfma_workspace(d, n, shape_c))
     if ws_bytes < 0:
         _native.check(int(ws_bytes), "summate_incompr_grid_mfma")

@@ -76,6 +76,16 @@ extern "C" {
 /* MFMA split-precision variants for gsx_summate_grid_mfma. */
 #define GSX_MFMA_BF16X3 0 /* bf16 hi/lo split, hi*hi + hi*lo + lo*hi */
 #define GSX_MFMA_FP16X3 1 /* fp16 hi/lo split, hi*hi + hi*lo + lo*hi */
+/* Tuning choices or-ed into the variant (results differ in rounding only):
+ * GSX_MFMA_FILL_LOG2(f), f in 8..11: grids with fewer than 2^f blocks split
+ * the mode (K) steps over about 2^f / blocks blocks (at least 128 modes each)
+ * and sum the partial sums in a fixed order; 0 = the default 2^9.
+ * GSX_MFMA_NO_SWAP: never run a 3D grid with axes 0 and 1 swapped (by
+ * default a short axis 0 is swapped with axis 1 when that needs fewer
+ * tiles).  The workspace sizes below cover every choice. */
+#define GSX_MFMA_FILL_SHIFT 8
+#define GSX_MFMA_FILL_LOG2(f) ((f) << GSX_MFMA_FILL_SHIFT)
+#define GSX_MFMA_NO_SWAP 0x1000
 /* Flag or-ed into the variant: the caller states that the last axis of a 2D
  * grid is uniformly spaced (y_j = y_0 + j h: y_j == y_(16a) + (y_b - y_0),
  * j = 16a + b, to 4 ulp of max|y|).  With at least 256 points on it the
@@ -132,6 +142,14 @@ int gsx_summate_grid_f32(int64_t d, int64_t N, const int64_t* shape,
  * float32, scaled by `alpha` (pass 1.0 for the unscaled sum).  Workspace
  * bytes from gsx_summate_grid_mfma_workspace(). */
 int64_t gsx_summate_grid_mfma_workspace(int64_t d, int64_t N, const int64_t* shape);
+/* The matrix-core plan of a grid for one variant (comps 1, or d for the
+ * incompressible form), host arithmetic only: info[0] axes 0 and 1 swapped,
+ * [1] uniform last axis factorised, [2] row tiles, [3] column tiles, [4] K
+ * split, [5] blocks, [6] workspace bytes of this variant, [7] 3 for the
+ * 16 x 16 column-tile kernel, 2 for flat columns.  For tuning tools: two
+ * variants with equal info launch the same kernels. */
+int gsx_summate_grid_mfma_plan(int64_t d, int64_t N, const int64_t* shape,
+                               int64_t comps, int variant, int64_t* info);
 int gsx_summate_grid_mfma(int64_t d, int64_t N, const int64_t* shape,
                           const double* axes, const double* cov,
                           const double* z1, const double* z2, int variant,

@@ -154,30 +154,65 @@ def test_uniform_axis_check():
     assert q(2, 4096, shp(4096, 4096)) >= q(3, 4096, shp(4096, 256, 16))
 
 
-def test_mfma_plan_through_workspace_sizes():
-    """The matrix-core grid plan is host arithmetic, visible in the workspace size:
-    a 3D grid with a short axis 0 runs with axes 0 and 1 swapped (A table of the
-    long axis, and here a K split that needs partial sums), a cubic grid keeps its
-    order, and a thin slab keeps axis 0 as rows."""
-    q = _native.lib().gsx_summate_grid_mfma_workspace
-    shp = lambda *s: (ctypes.c_int64 * 3)(*(list(s) + [1] * (3 - len(s))))
+def _mfma_plan(d, N, shape, comps=1, variant=0):
+    info = (ctypes.c_int64 * 8)()
+    shp = (ctypes.c_int64 * 3)(*(list(shape) + [1] * (3 - len(shape))))
+    assert _native.lib().gsx_summate_grid_mfma_plan(d, N, shp, comps, variant, info) == 0
+    keys = ("swapped", "split", "row_tiles", "col_tiles", "ksplit", "blocks", "ws", "kernel")
+    return dict(zip(keys, list(info)))
+
+
+def test_mfma_plan_decisions():
+    """The matrix-core grid plan is host arithmetic: a 3D grid with a short axis 0
+    runs with axes 0 and 1 swapped (fewer tiles; here a K split with partial
+    sums), a cubic grid keeps its order, a thin slab keeps axis 0 as rows, and
+    the workspace of each plan is the tables plus the partial sums."""
     a256 = lambda b: -(-b // 256) * 256
 
-    def tables(rows, n1, n2, N=4096):
+    def tables(rows, n1, n2, N=4096, comps=1):
         rows_p = -(-rows // 128) * 128
-        return 2 * (rows_p * 2 * N) * 2 + (n1 * N + n2 * N) * 8
+        return 2 * (rows_p * 2 * N) * 2 + (n1 * N + comps * n2 * N) * 8
 
     # (32, 512, 512): swapped -> rows 512, column axes (32, 512): 4 x 64 = 256 blocks,
     # K split in 2 -> tables + 2 partial sums of P floats
     P = 32 * 512 * 512
-    assert q(3, 4096, shp(32, 512, 512)) == a256(tables(512, 32, 512)) + 2 * P * 4
-    # unswapped it would be rows 32 (one 128-row tile) x 1024 column tiles: no K split
-    assert q(3, 4096, shp(32, 512, 512)) != tables(32, 512, 512)
+    p = _mfma_plan(3, 4096, (32, 512, 512))
+    assert p["swapped"] == 1 and p["row_tiles"] == 4 and p["col_tiles"] == 64 and p["ksplit"] == 2
+    assert p["ws"] == a256(tables(512, 32, 512)) + 2 * P * 4 and p["kernel"] == 3
+    # unswapped: rows 32 (one 128-row tile) x 1024 column tiles, no K split
+    q = _mfma_plan(3, 4096, (32, 512, 512), variant=_native.MFMA_NO_SWAP)
+    assert q["swapped"] == 0 and q["row_tiles"] == 1 and q["col_tiles"] == 1024 and q["ksplit"] == 1
+    assert q["ws"] == tables(32, 512, 512)
     # (512, 512, 512) and a thin slab (512, 512, 8) keep axis 0 as rows
-    assert q(3, 4096, shp(512, 512, 512)) == tables(512, 512, 512)
-    slab = q(3, 4096, shp(512, 512, 8))
+    cube = _mfma_plan(3, 4096, (512, 512, 512))
+    assert cube["swapped"] == 0 and cube["ksplit"] == 1 and cube["ws"] == tables(512, 512, 512)
+    slab = _mfma_plan(3, 4096, (512, 512, 8))
     ks = 512 // (4 * 16)  # 4 row tiles x 16 flat column tiles of 256 -> 64 blocks
-    assert slab == a256(tables(512, 512, 8)) + ks * 512 * 512 * 8 * 4
+    assert slab["kernel"] == 2 and slab["ksplit"] == ks
+    assert slab["ws"] == a256(tables(512, 512, 8)) + ks * 512 * 512 * 8 * 4
+
+
+def test_mfma_fill_targets_and_workspace_bound():
+    """The K-split fill target scales the split (at least 128 modes per split);
+    the generic workspace query covers every tuning choice."""
+    wsq = _native.lib().gsx_summate_grid_mfma_workspace
+    for shape in ((32, 512, 512), (512, 512, 8), (64, 64), (300, 200), (2048, 2048), (16, 16, 16)):
+        d = len(shape)
+        shp = (ctypes.c_int64 * 3)(*(list(shape) + [1] * (3 - d)))
+        bound = wsq(d, 4096, shp)
+        splits = []
+        for f in (8, 9, 10, 11):
+            for extra in (0, _native.MFMA_NO_SWAP):
+                p = _mfma_plan(d, 4096, shape, variant=(f << _native.MFMA_FILL_SHIFT) | extra)
+                assert p["ws"] <= bound
+                assert 4096 // 16 // p["ksplit"] >= 8 or p["ksplit"] == 1
+                if not extra:
+                    splits.append(p["ksplit"])
+        assert splits == sorted(splits)  # more resident blocks -> no smaller split
+        assert _mfma_plan(d, 4096, shape) == _mfma_plan(d, 4096, shape, variant=9 << _native.MFMA_FILL_SHIFT)
+    info = (ctypes.c_int64 * 8)()
+    bad = (ctypes.c_int64 * 3)(64, 64, 1)
+    assert _native.lib().gsx_summate_grid_mfma_plan(2, 4096, bad, 1, 12 << _native.MFMA_FILL_SHIFT, info) == -1
 
 
 def test_fourier_tensor_workspace():
