@@ -311,19 +311,16 @@ def bench_krige(torch, steps, warmup):
     cond_pos = rs.uniform(0, 1000, size=(3, k_cond))
     cond_val = rs.normal(size=k_cond)
     krig = gkrige.Ordinary(model, cond_pos, cond_val)
+    system = krig.system
     dev = torch.device("cuda", 0)
-    mat = torch.from_numpy(krig._krige_mat).to(dev)
-    cond = torch.from_numpy(krig._krige_cond).to(dev)
-    # right-hand sides (krige_size, P) built on the device: covariance rows plus
-    # the unbiasedness row, exactly what the host front end builds per chunk
-    pts = torch.from_numpy(rs.uniform(0, 1000, size=(3, n_pts))).to(dev)
-    cpos = torch.from_numpy(krig._krige_pos).to(dev)
-    vecs = torch.empty((k_cond + 1, n_pts), dtype=torch.float64, device=dev)
-    for c0 in range(0, n_pts, 200_000):
-        d = torch.cdist(cpos.T.contiguous(), pts[:, c0:c0 + 200_000].T.contiguous())
-        vecs[:k_cond, c0:c0 + 200_000] = model.var * torch.exp(-d / model.len_rescaled)
-    vecs[k_cond] = 1.0
-    del d
+    mat_h = system.inverse
+    cond_h = krig._residual()
+    mat = torch.from_numpy(mat_h).to(dev)
+    cond = torch.from_numpy(cond_h).to(dev)
+    # right-hand sides (krige_size, P) built on the device by the front end's own
+    # path: covariance rows from the device covariance kernel plus the unbiased row
+    pts = rs.uniform(0, 1000, size=(3, n_pts))
+    vecs = system.rhs_device(model.isometrize(pts), np.ones((1, n_pts)), krig.exact)
     kk = k_cond + 1
     flops = 2.0 * kk * kk * n_pts
     both = lambda: gkrige.krige_sum_device(mat, vecs, cond, variance=True)
@@ -333,12 +330,14 @@ def bench_krige(torch, steps, warmup):
     wall_f, ker_f = time_kernel(only, steps, warmup, torch)
     sub = np.random.RandomState(4).choice(n_pts, 2000, replace=False)
     vs = vecs[:, torch.from_numpy(sub).to(dev)].cpu().numpy()
+    rhs_diff = float(np.max(np.abs(vs - system.rhs_host(model.isometrize(pts[:, sub]), np.ones((1, sub.size)),
+                                                             krig.exact))))
     t0 = time.perf_counter()
-    f_ref, e_ref = oracle.krige_c(krig._krige_mat, vs, krig._krige_cond)
+    f_ref, e_ref = oracle.krige_c(mat_h, vs, cond_h)
     t_cpu = time.perf_counter() - t0
     fe = f.cpu().numpy()[sub]
     ee = e.cpu().numpy()[sub]
-    scale = np.abs(krig._krige_mat) @ np.abs(vs)
+    scale = np.abs(mat_h) @ np.abs(vs)
     tflops = flops / ker_v / 1e12
     return {
         "workload": "ordinary kriging sums, 3D Exponential var 2 len 50, 1000 conditioning points (+1 unbiased row) "
@@ -348,8 +347,9 @@ def bench_krige(torch, steps, warmup):
         "roofline_frac_fp64_mfma": tflops / FP64_VALU_PEAK_TFLOPS,
         "field_only_ms": wall_f * 1e3,
         "field_only_hbm_gbs": 8.0 * kk * n_pts / ker_f / 1e9,
-        "max_err_over_bound_field": float(np.max(np.abs(fe - f_ref) / (np.abs(krig._krige_cond) @ scale))),
+        "max_err_over_bound_field": float(np.max(np.abs(fe - f_ref) / (np.abs(cond_h) @ scale))),
         "max_err_over_bound_error": float(np.max(np.abs(ee - e_ref) / np.sum(np.abs(vs) * scale, axis=0))),
+        "device_rhs_max_abs_diff": rhs_diff,
         "cpu_oracle_points_per_s": 2000 / t_cpu,
         "gpu_points_per_s": n_pts / wall_v,
     }
@@ -513,7 +513,7 @@ def batch_modes():
     """
     import torch
 
-    from gstools_amd.sampling import sample_modes_batch
+    from gstools_amd.sampling import sample_modes_many
 
     specs = _batch_specs()
 
