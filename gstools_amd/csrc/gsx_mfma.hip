@@ -321,9 +321,13 @@ __global__ __launch_bounds__(kMWaves * 64, 2) void k_mfma_grid(MfmaGeom g, doubl
                                                                float* __restrict__ out, int64_t out_stride,
                                                                int64_t comps, int64_t steps_per,
                                                                float* __restrict__ partial) {
-    // [buffer][hi/lo][row * 16 + chunk * 4]: 64-byte A rows, 16-byte chunk c of row r
-    // stored at c ^ ((r >> 1) & 3) (the 8 lanes of a 16-byte LDS access phase hit 8 bank groups)
-    __shared__ __align__(16) uint32_t lds[2][2][kMBM * 16];
+    // two stages [hi/lo][row * 16 + chunk * 4]: 64-byte A rows, 16-byte chunk c of row r
+    // stored at c ^ ((r >> 1) & 3) (the 8 lanes of a 16-byte LDS access phase hit 8 bank
+    // groups).  The stages are separate LDS objects and the step loop is unrolled by two,
+    // so the compiler sees that a step's LDS reads and the next step's global -> LDS loads
+    // touch different objects and does not wait for those loads before the reads.
+    __shared__ __align__(16) uint32_t lds0[2][kMBM * 16];
+    __shared__ __align__(16) uint32_t lds1[2][kMBM * 16];
 
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
@@ -387,28 +391,29 @@ __global__ __launch_bounds__(kMWaves * 64, 2) void k_mfma_grid(MfmaGeom g, doubl
         const int r = q >> 2;
         aoff[h] = (row0 + r) * g.Np + (((q & 3) ^ ((r >> 1) & 3)) * 4);
     }
-    auto load_a = [&](int64_t step, int buf) {
+    typedef uint32_t AStage[2][kMBM * 16];
+    auto load_a = [&](int64_t step, AStage& S) {
 #pragma unroll
         for (int h = 0; h < 2; ++h) {
             const int slot = (h * 256 + wv * 64) * 4;
             __builtin_amdgcn_global_load_lds((const void*)(a_hi + aoff[h] + step * kMKM),
-                                             (__attribute__((address_space(3))) void*)&lds[buf][0][slot], 16, 0, 0);
+                                             (__attribute__((address_space(3))) void*)&S[0][slot], 16, 0, 0);
             __builtin_amdgcn_global_load_lds((const void*)(a_lo + aoff[h] + step * kMKM),
-                                             (__attribute__((address_space(3))) void*)&lds[buf][1][slot], 16, 0, 0);
+                                             (__attribute__((address_space(3))) void*)&S[1][slot], 16, 0, 0);
         }
     };
 
     const int64_t steps = g.Np / kMKM;
     const int64_t s0 = SPLIT ? split * steps_per : 0;
     const int64_t s1 = SPLIT ? (s0 + steps_per < steps ? s0 + steps_per : steps) : steps;
-    load_a(s0, 0);
+    load_a(s0, lds0);
     asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
     __syncthreads();
 
     const int asw = (l16 >> 1) & 3;
-    for (int64_t s = s0; s < s1; ++s) {
-        const int buf = (int)((s - s0) & 1);
-        if (s + 1 < s1) load_a(s + 1, buf ^ 1);
+    // one K step on stage C while stage Nx receives the next step's A tile
+    auto step = [&](const AStage& C, AStage& Nx, int64_t s) {
+        if (s + 1 < s1) load_a(s + 1, Nx);
 
         // B fragments for this K step, built in registers.
         u32x4 bh[kMCF], bl[kMCF];
@@ -446,8 +451,8 @@ __global__ __launch_bounds__(kMWaves * 64, 2) void k_mfma_grid(MfmaGeom g, doubl
 #pragma unroll
         for (int rf = 0; rf < kMRF; ++rf) {
             const int pos = (rf * 16 + l16) * 16 + ((grp ^ asw) * 4);
-            const u32x4 ah = *reinterpret_cast<const u32x4*>(&lds[buf][0][pos]);
-            const u32x4 al = *reinterpret_cast<const u32x4*>(&lds[buf][1][pos]);
+            const u32x4 ah = *reinterpret_cast<const u32x4*>(&C[0][pos]);
+            const u32x4 al = *reinterpret_cast<const u32x4*>(&C[1][pos]);
 #pragma unroll
             for (int cf = 0; cf < kMCF; ++cf) {
                 acc[rf][cf] = mfma<FP16>(ah, bh[cf], acc[rf][cf]);
@@ -458,6 +463,10 @@ __global__ __launch_bounds__(kMWaves * 64, 2) void k_mfma_grid(MfmaGeom g, doubl
         // the next step's LDS loads have landed (the barrier's own wait covers LDS, not them)
         asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
         __syncthreads();
+    };
+    for (int64_t s = s0; s < s1; s += 2) {
+        step(lds0, lds1, s);
+        if (s + 1 < s1) step(lds1, lds0, s + 1);
     }
 
     const float fa = SPLIT ? 1.0f : (float)alpha;  // partial sums are scaled when added
@@ -511,7 +520,11 @@ __global__ __launch_bounds__(kMWaves * 64, 2) void k_mfma_grid3(MfmaGeom g, doub
                                                                 float* __restrict__ out, int64_t out_stride,
                                                                 int64_t comps, int64_t steps_per,
                                                                 float* __restrict__ partial) {
-    __shared__ Stage3 st[2];
+    // two stages as separate LDS objects, the step loop unrolled by two: the compiler
+    // then sees that a step's LDS reads and the next step's global -> LDS loads touch
+    // different objects and does not wait for those loads before the reads
+    __shared__ Stage3 st0;
+    __shared__ Stage3 st1;
     const int tid = threadIdx.x;
     const int lane = tid & 63;
     const int wave = tid >> 6;
@@ -559,16 +572,16 @@ __global__ __launch_bounds__(kMWaves * 64, 2) void k_mfma_grid3(MfmaGeom g, doub
         aoff[h] = (row0 + r) * g.Np + (((q & 3) ^ ((r >> 1) & 3)) * 4);
     }
     const float2* ebase = (is_e1 ? e1 : e2) + eidx * g.Np + (((tid & 7) ^ (erow & 7)) * 2);
-    auto load = [&](int64_t step, int buf) {
+    auto load = [&](int64_t step, Stage3& S) {
 #pragma unroll
         for (int h = 0; h < 2; ++h) {
             const int slot = (h * 256 + wv * 64) * 4;
             __builtin_amdgcn_global_load_lds((const void*)(a_hi + aoff[h] + step * kMKM),
-                                             (__attribute__((address_space(3))) void*)&st[buf].ah[slot], 16, 0, 0);
+                                             (__attribute__((address_space(3))) void*)&S.ah[slot], 16, 0, 0);
             __builtin_amdgcn_global_load_lds((const void*)(a_lo + aoff[h] + step * kMKM),
-                                             (__attribute__((address_space(3))) void*)&st[buf].al[slot], 16, 0, 0);
+                                             (__attribute__((address_space(3))) void*)&S.al[slot], 16, 0, 0);
         }
-        float* es = (wv < 2 ? st[buf].e1 : st[buf].e2) + (wv & 1) * 256;
+        float* es = (wv < 2 ? S.e1 : S.e2) + (wv & 1) * 256;
         __builtin_amdgcn_global_load_lds((const void*)(ebase + step * kMKM),
                                          (__attribute__((address_space(3))) void*)es, 16, 0, 0);
     };
@@ -582,17 +595,17 @@ __global__ __launch_bounds__(kMWaves * 64, 2) void k_mfma_grid3(MfmaGeom g, doub
     const int64_t steps = g.Np / kMKM;
     const int64_t s0 = SPLIT ? split * steps_per : 0;
     const int64_t s1 = SPLIT ? (s0 + steps_per < steps ? s0 + steps_per : steps) : steps;
-    load(s0, 0);
+    load(s0, st0);
     asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
     __syncthreads();
     const int zsw = l16 & 7;
-    for (int64_t s = s0; s < s1; ++s) {
-        const int buf = (int)((s - s0) & 1);
-        if (s + 1 < s1) load(s + 1, buf ^ 1);
+    // one K step on stage C while stage Nx receives the next step's tiles
+    auto step = [&](const Stage3& C, Stage3& Nx, int64_t s) {
+        if (s + 1 < s1) load(s + 1, Nx);
         // B fragments: E2 of this lane's i2 (row l16) times E1 of each fragment's i1
         float2 zf[4];
         {
-            const float* zr = st[buf].e2 + l16 * 32;
+            const float* zr = C.e2 + l16 * 32;
             const float4 q0 = *reinterpret_cast<const float4*>(zr + (((2 * grp) ^ zsw) * 4));
             const float4 q1 = *reinterpret_cast<const float4*>(zr + (((2 * grp + 1) ^ zsw) * 4));
             zf[0] = {q0.x, q0.y};
@@ -605,7 +618,7 @@ __global__ __launch_bounds__(kMWaves * 64, 2) void k_mfma_grid3(MfmaGeom g, doub
         for (int cf = 0; cf < kMCF; ++cf) {
             const int er = wave * kMCF + cf;
             const int ysw = er & 7;
-            const float* yr = st[buf].e1 + er * 32;
+            const float* yr = C.e1 + er * 32;
             const float4 q0 = *reinterpret_cast<const float4*>(yr + (((2 * grp) ^ ysw) * 4));
             const float4 q1 = *reinterpret_cast<const float4*>(yr + (((2 * grp + 1) ^ ysw) * 4));
             const float2 yf[4] = {{q0.x, q0.y}, {q0.z, q0.w}, {q1.x, q1.y}, {q1.z, q1.w}};
@@ -623,8 +636,8 @@ __global__ __launch_bounds__(kMWaves * 64, 2) void k_mfma_grid3(MfmaGeom g, doub
 #pragma unroll
         for (int rf = 0; rf < kMRF; ++rf) {
             const int pos = (rf * 16 + l16) * 16 + ((grp ^ asw) * 4);
-            const u32x4 ah = *reinterpret_cast<const u32x4*>(&st[buf].ah[pos]);
-            const u32x4 al = *reinterpret_cast<const u32x4*>(&st[buf].al[pos]);
+            const u32x4 ah = *reinterpret_cast<const u32x4*>(&C.ah[pos]);
+            const u32x4 al = *reinterpret_cast<const u32x4*>(&C.al[pos]);
 #pragma unroll
             for (int cf = 0; cf < kMCF; ++cf) {
                 acc[rf][cf] = mfma<FP16>(ah, bh[cf], acc[rf][cf]);
@@ -635,6 +648,10 @@ __global__ __launch_bounds__(kMWaves * 64, 2) void k_mfma_grid3(MfmaGeom g, doub
         // the next step's LDS loads have landed (the barrier's own wait covers LDS, not them)
         asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
         __syncthreads();
+    };
+    for (int64_t s = s0; s < s1; s += 2) {
+        step(st0, st1, s);
+        if (s + 1 < s1) step(st1, st0, s + 1);
     }
 
     const float fa = SPLIT ? 1.0f : (float)alpha;  // partial sums are scaled when added
