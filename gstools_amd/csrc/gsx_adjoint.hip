@@ -19,7 +19,14 @@
 // FP64: exact reference phase and sincos_fast (two-term reduction) with a
 // library fallback for waves whose phases may exceed 2^46.  FP32: phase in
 // revolutions in FP64, v_fract_f64, hardware v_sin_f32 / v_cos_f32, FP32
-// partial sums per LDS chunk folded into FP64 accumulators.
+// partial sums per LDS chunk folded into FP64 accumulators.  The FP32 path
+// sums over the shifted phase psi_ij = t_ij - phi_j, z1 + i z2 = a e^{i phi}
+// (the forward kernel's amplitude-phase form), which needs 2 + D sums per
+// mode instead of 2 + 2D:
+//   C_j = sum_i g_i cos psi_ij,  S_j = sum_i g_i sin psi_ij,
+//   Sx_jd = sum_i g_i x_id sin psi_ij,
+//   dL/dz1_j = cos phi C - sin phi S,   dL/dz2_j = sin phi C + cos phi S,
+//   dL/dk_jd = z2 Cx - z1 Sx = -a_j Sx_jd   (z2 cos t - z1 sin t = -a sin psi).
 #include <mutex>
 
 #include "gsx_internal.h"
@@ -57,14 +64,22 @@ __device__ __noinline__ void sincos_any(double x, double* sn, double* cs) {
     }
 }
 
+// partial sums per (chunk, mode): FP64 2 + 2D (unshifted phase), FP32 2 + D (shifted)
+template <int D, bool F64>
+constexpr int adj_sums() {
+    return F64 ? 2 + 2 * D : 2 + D;
+}
+
 template <int D, typename GT>
 __global__ __launch_bounds__(kThreads) void k_adjoint_stage1(int64_t N, int64_t P, int64_t chunk_pts,
                                                              const double* __restrict__ cov,
+                                                             const double* __restrict__ z1,
+                                                             const double* __restrict__ z2,
                                                              const double* __restrict__ pos,
                                                              const GT* __restrict__ g,
                                                              double* __restrict__ part) {
     constexpr bool F64 = sizeof(GT) == 8;
-    constexpr int Q = 2 + 2 * D;
+    constexpr int Q = adj_sums<D, F64>();
     __shared__ double s_x[kAdjPts * D];
     __shared__ GT s_g[kAdjPts * (D + 1)];  // g, g x_0 .. g x_{D-1}
     __shared__ unsigned long long s_xmax;
@@ -79,6 +94,8 @@ __global__ __launch_bounds__(kThreads) void k_adjoint_stage1(int64_t N, int64_t 
         if (!F64) k[d] *= kInv2Pi;
         k1 += fabs(k[d]);
     }
+    // FP32: -phi / 2pi, the phase shift in revolutions
+    const double mphi = (!F64 && active) ? -atan2(z2[j], z1[j]) * kInv2Pi : 0.0;
     double acc[Q];
 #pragma unroll
     for (int q = 0; q < Q; ++q) acc[q] = 0.0;
@@ -144,7 +161,7 @@ __global__ __launch_bounds__(kThreads) void k_adjoint_stage1(int64_t N, int64_t 
 #pragma unroll
             for (int q = 0; q < Q; ++q) pa[q] = 0.0f;
             for (int m = 0; m < cnt; ++m) {
-                double rev = 0.0;
+                double rev = mphi;
 #pragma unroll
                 for (int d = 0; d < D; ++d) rev = fma_(k[d], s_x[m * D + d], rev);
                 const float f = (float)__builtin_amdgcn_fract(rev);
@@ -154,10 +171,7 @@ __global__ __launch_bounds__(kThreads) void k_adjoint_stage1(int64_t N, int64_t 
                 pa[0] = __builtin_fmaf(gg[0], cs, pa[0]);
                 pa[1] = __builtin_fmaf(gg[0], sn, pa[1]);
 #pragma unroll
-                for (int d = 0; d < D; ++d) {
-                    pa[2 + d] = __builtin_fmaf(gg[1 + d], cs, pa[2 + d]);
-                    pa[2 + D + d] = __builtin_fmaf(gg[1 + d], sn, pa[2 + D + d]);
-                }
+                for (int d = 0; d < D; ++d) pa[2 + d] = __builtin_fmaf(gg[1 + d], sn, pa[2 + d]);
             }
 #pragma unroll
             for (int q = 0; q < Q; ++q) acc[q] += (double)pa[q];
@@ -170,7 +184,7 @@ __global__ __launch_bounds__(kThreads) void k_adjoint_stage1(int64_t N, int64_t 
     }
 }
 
-template <int D>
+template <int D, bool F64>
 __global__ __launch_bounds__(kThreads) void k_adjoint_stage2(int64_t N, int64_t chunks,
                                                              const double* __restrict__ z1,
                                                              const double* __restrict__ z2,
@@ -178,7 +192,7 @@ __global__ __launch_bounds__(kThreads) void k_adjoint_stage2(int64_t N, int64_t 
                                                              double* __restrict__ gz1,
                                                              double* __restrict__ gz2,
                                                              double* __restrict__ gk) {
-    constexpr int Q = 2 + 2 * D;
+    constexpr int Q = adj_sums<D, F64>();
     const int64_t j = (int64_t)blockIdx.x * kThreads + threadIdx.x;
     if (j >= N) return;
     double acc[Q];
@@ -189,11 +203,21 @@ __global__ __launch_bounds__(kThreads) void k_adjoint_stage2(int64_t N, int64_t 
 #pragma unroll
         for (int q = 0; q < Q; ++q) acc[q] += src[q * N + j];
     }
-    gz1[j] = acc[0];
-    gz2[j] = acc[1];
     const double a1 = z1[j], a2 = z2[j];
+    if constexpr (F64) {
+        gz1[j] = acc[0];
+        gz2[j] = acc[1];
 #pragma unroll
-    for (int d = 0; d < D; ++d) gk[d * N + j] = a2 * acc[2 + d] - a1 * acc[2 + D + d];
+        for (int d = 0; d < D; ++d) gk[d * N + j] = a2 * acc[2 + d] - a1 * acc[2 + D + d];
+    } else {
+        // rotate the shifted sums back by phi (a = 0: phi = 0, nothing to rotate)
+        const double a = hypot(a1, a2);
+        const double c = a > 0.0 ? a1 / a : 1.0, sn = a > 0.0 ? a2 / a : 0.0;
+        gz1[j] = c * acc[0] - sn * acc[1];
+        gz2[j] = sn * acc[0] + c * acc[1];
+#pragma unroll
+        for (int d = 0; d < D; ++d) gk[d * N + j] = -a * acc[2 + d];
+    }
 }
 
 template <int D, typename GT>
@@ -202,11 +226,11 @@ int adjoint_launch(const AdjPlan& pl, int64_t N, int64_t P, const double* cov, c
                    double* gk, void* ws, hipStream_t s) {
     double* part = static_cast<double*>(ws);
     hipLaunchKernelGGL((k_adjoint_stage1<D, GT>), dim3((unsigned)pl.mode_tiles, (unsigned)pl.chunks),
-                       dim3(kThreads), 0, s, N, P, pl.chunk_pts, cov, pos, g, part);
+                       dim3(kThreads), 0, s, N, P, pl.chunk_pts, cov, z1, z2, pos, g, part);
     int st = launch_status();
     if (st != GSX_OK) return st;
-    hipLaunchKernelGGL((k_adjoint_stage2<D>), dim3((unsigned)pl.mode_tiles), dim3(kThreads), 0, s, N,
-                       pl.chunks, z1, z2, part, gz1, gz2, gk);
+    hipLaunchKernelGGL((k_adjoint_stage2<D, sizeof(GT) == 8>), dim3((unsigned)pl.mode_tiles), dim3(kThreads), 0, s,
+                       N, pl.chunks, z1, z2, part, gz1, gz2, gk);
     return launch_status();
 }
 
