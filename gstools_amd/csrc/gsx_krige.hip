@@ -8,18 +8,25 @@
 // (src/gstools/krige/base.py:42-61, called at :307-317); the caller turns
 // error into the kriging variance max(sill - error, 0).
 //
-// Variance form: W = M V is an FP64 GEMM (2 K^2 P flops) with an epilogue
-// that never stores W: every block owns a 128-row x 128-point tile of W,
-// accumulates it on the matrix cores (v_mfma_f64_16x16x4, 4 waves in a 2 x 2
-// arrangement, each 64 rows x 64 points, 16 MFMA tiles in 227 registers at
-// 2 waves per SIMD), and reduces V o W and cond o W over its 128 rows into
-// per-row-tile partial sums.  A second kernel adds the partials of the row
-// tiles in a fixed order (bitwise reproducible).  Operand tiles (128 x 16 of
-// M, 16 x 128 of V) are staged through LDS with the next stage prefetched into
-// registers.
+// Field (both forms): field = (M^T cond) . v_k, a weight vector (K) followed
+// by an HBM-bound pass over V (no GEMM).
 //
-// Field-only form: field = (M^T cond) . v_k, a weight vector (K) followed by
-// an HBM-bound pass over V (no GEMM).
+// Variance form: error_k = v_k^T M v_k = v_k^T S v_k with S = (M + M^T) / 2
+// (the antisymmetric part of M adds nothing to a quadratic form), and with S
+// symmetric only the blocks on and right of the diagonal are needed:
+//
+//   error_k = sum over row tiles r of  v_r^T (M2 V)_r,
+//   M2 = S on the diagonal 128 x 128 blocks, 2 S = M + M^T right of them,
+//
+// so the GEMM W = M2 V runs, for row tile r, over the columns from 128 r on:
+// about K^2 P flops instead of 2 K^2 P.  Every block owns a 128-row x
+// 128-point tile of W, accumulates it on the matrix cores (v_mfma_f64_16x16x4,
+// 4 waves in a 2 x 2 arrangement, each 64 rows x 64 points, 16 MFMA tiles at 2
+// waves per SIMD) and reduces V o W over its 128 rows into a per-row-tile
+// partial sum (W is never stored).  A second kernel adds the partials of the
+// row tiles in a fixed order (bitwise reproducible).  Operand tiles (128 x 16
+// of M2, 16 x 128 of V) are staged through LDS with the next stage prefetched
+// into registers.
 //
 // Right-hand sides: k_krige_cov writes the covariance rows of V,
 // C(|c_i - x_p|) for conditioning points c_i and targets x_p, for models
@@ -48,145 +55,154 @@ constexpr int kKrApad = kKrK + 1;
 // (rows 144 doubles apart) fall in disjoint LDS banks
 constexpr int kKrBpad = kKrPts + 16;
 
+// M2 of the variance form (row tile r keeps its columns from 128 r on; the
+// entries left of the diagonal block are never read and not written)
+__global__ __launch_bounds__(256) void k_krige_sym(int64_t K, const double* __restrict__ M,
+                                                   double* __restrict__ M2) {
+    const int64_t idx = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (idx >= K * K) return;
+    const int64_t i = idx / K, l = idx - i * K;
+    const int64_t b0 = (i / kKrRows) * kKrRows;
+    if (l < b0) return;
+    const double s = M[i * K + l] + M[l * K + i];
+    M2[idx] = l < b0 + kKrRows ? 0.5 * s : s;
+}
+
+// W = M2 V over the columns from the row tile on (K steps in descending
+// order); per-row-tile partial sums of V o W.  Row tile r runs about K - 128 r
+// K steps, so each block takes the pair (u, R - 1 - u): equal work per block
+// (about K + 128 steps), and the long tile first.
 __global__ __launch_bounds__(256, 2) void k_krige_gemm(int64_t K, int64_t P, int64_t row_tiles, int64_t col_tiles,
                                                     const double* __restrict__ M,
                                                     const double* __restrict__ V,
-                                                    const double* __restrict__ cond,
-                                                    double* __restrict__ part_f, double* __restrict__ part_e) {
+                                                    double* __restrict__ part_e) {
     __shared__ double As[kKrRows * kKrApad];
     __shared__ double Bs[kKrK * kKrBpad];
-    __shared__ double red[2][2][kKrPts];  // [row half][field / error][point]
+    __shared__ double red[2][kKrPts];  // [row half][point]
 
     const int t = threadIdx.x;
     const int lane = t & 63, wave = t >> 6;
     const int wr = wave >> 1, wc = wave & 1;
     const int g = lane >> 4, l16 = lane & 15;
     // XCD-aware order: dispatch puts block b on XCD b % 8; the blocks of one XCD
-    // take all row tiles of a column tile in turn, so that column's slice of V
-    // (K x 128 doubles) is reused from that XCD's L2
-    int64_t rt_idx, ct_idx;
+    // take all row-tile pairs of a column tile in turn, so that column's slice of
+    // V (K x 128 doubles) is reused from that XCD's L2
+    const int64_t R = row_tiles, U = (R + 1) / 2;
+    int64_t u_idx, ct_idx;
     {
-        // the grid has R * C8 blocks, C8 = col_tiles rounded up to a multiple of
-        // 8: every XCD owns the column tiles ct == xcd (mod 8) with their R row
-        // tiles (a bijection onto row tiles x [0, C8)); padding blocks leave
-        const int64_t R = row_tiles;
+        // the grid has U * C8 blocks, C8 = col_tiles rounded up to a multiple of
+        // 8: every XCD owns the column tiles ct == xcd (mod 8) with their U pairs
+        // (a bijection onto pairs x [0, C8)); padding blocks leave
         const int64_t b = blockIdx.x, xcd = b & 7, slot = b >> 3;
-        rt_idx = slot % R;
-        ct_idx = (slot / R) * 8 + xcd;
+        u_idx = slot % U;
+        ct_idx = (slot / U) * 8 + xcd;
     }
     if (ct_idx >= col_tiles) return;  // whole block: before any barrier
-    const int64_t i0 = rt_idx * kKrRows;
     const int64_t p0 = ct_idx * kKrPts;
+    const int ntiles = (u_idx == R - 1 - u_idx) ? 1 : 2;
+#pragma unroll 1
+    for (int ti = 0; ti < ntiles; ++ti) {
+        const int64_t rt_idx = ti == 0 ? u_idx : R - 1 - u_idx;
+        const int64_t i0 = rt_idx * kKrRows;
 
-    // staging assignment: A kKrRows x kKrK (kAU consecutive per thread); B kKrK x 128
-    // (kKrK / 2 per thread: point t % 128 of rows t / 128 + 2u, so a wave's loads
-    // and LDS stores are unit-stride and the stores conflict-free)
-    constexpr int kAT = 256 / kKrRows;  // threads per A row
-    constexpr int kAU = kKrK / kAT, kBU = kKrK / 2;
-    const int ar = t / kAT, ac = (t % kAT) * kAU;
-    const int br = t >> 7, bc = t & 127;
-    double pa[kAU], pb[kBU];
-    auto load = [&](int64_t j0) {
-        const int64_t row = i0 + ar;
+        // staging assignment: A kKrRows x kKrK (kAU consecutive per thread); B kKrK x 128
+        // (kKrK / 2 per thread: point t % 128 of rows t / 128 + 2u, so a wave's loads
+        // and LDS stores are unit-stride and the stores conflict-free)
+        constexpr int kAT = 256 / kKrRows;  // threads per A row
+        constexpr int kAU = kKrK / kAT, kBU = kKrK / 2;
+        const int ar = t / kAT, ac = (t % kAT) * kAU;
+        const int br = t >> 7, bc = t & 127;
+        double pa[kAU], pb[kBU];
+        auto load = [&](int64_t j0) {
+            const int64_t row = i0 + ar;
 #pragma unroll
-        for (int u = 0; u < kAU; ++u) {
-            const int64_t col = j0 + ac + u;
-            pa[u] = (row < K && col < K) ? M[row * K + col] : 0.0;
-        }
-        const int64_t p = p0 + bc;
+            for (int u = 0; u < kAU; ++u) {
+                const int64_t col = j0 + ac + u;
+                pa[u] = (row < K && col < K) ? M[row * K + col] : 0.0;
+            }
+            const int64_t p = p0 + bc;
 #pragma unroll
-        for (int u = 0; u < kBU; ++u) {
-            const int64_t j = j0 + br + 2 * u;
-            pb[u] = (j < K && p < P) ? V[j * P + p] : 0.0;
-        }
-    };
-    auto store = [&]() {
+            for (int u = 0; u < kBU; ++u) {
+                const int64_t j = j0 + br + 2 * u;
+                pb[u] = (j < K && p < P) ? V[j * P + p] : 0.0;
+            }
+        };
+        auto store = [&]() {
 #pragma unroll
-        for (int u = 0; u < kAU; ++u) As[ar * kKrApad + ac + u] = pa[u];
+            for (int u = 0; u < kAU; ++u) As[ar * kKrApad + ac + u] = pa[u];
 #pragma unroll
-        for (int u = 0; u < kBU; ++u) Bs[(br + 2 * u) * kKrBpad + bc] = pb[u];
-    };
+            for (int u = 0; u < kBU; ++u) Bs[(br + 2 * u) * kKrBpad + bc] = pb[u];
+        };
 
-    f64x4 acc[kKrRF][4];
-#pragma unroll
-    for (int rt = 0; rt < kKrRF; ++rt)
-#pragma unroll
-        for (int ct = 0; ct < 4; ++ct) acc[rt][ct] = f64x4{0.0, 0.0, 0.0, 0.0};
-
-    load(0);
-    for (int64_t j0 = 0; j0 < K; j0 += kKrK) {
-        __syncthreads();
-        store();
-        __syncthreads();
-        if (j0 + kKrK < K) load(j0 + kKrK);
-#pragma unroll
-        for (int kk = 0; kk < kKrK; kk += 4) {
-            double a[kKrRF], b[4];
-#pragma unroll
-            for (int rt = 0; rt < kKrRF; ++rt) a[rt] = As[(wr * (kKrRows / 2) + rt * 16 + l16) * kKrApad + kk + g];
-#pragma unroll
-            for (int ct = 0; ct < 4; ++ct) b[ct] = Bs[(kk + g) * kKrBpad + wc * 64 + ct * 16 + l16];
-#pragma unroll
-            for (int rt = 0; rt < kKrRF; ++rt)
-#pragma unroll
-                for (int ct = 0; ct < 4; ++ct)
-                    acc[rt][ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[rt], b[ct], acc[rt][ct], 0, 0, 0);
-        }
-    }
-
-    // epilogue: per column, sum over this wave's kKrRows / 2 rows of cond o W and V o W
-    double sf[4], se[4];
-#pragma unroll
-    for (int ct = 0; ct < 4; ++ct) {
-        const int64_t p = p0 + wc * 64 + ct * 16 + l16;
-        double f = 0.0, e = 0.0;
+        f64x4 acc[kKrRF][4];
 #pragma unroll
         for (int rt = 0; rt < kKrRF; ++rt)
 #pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                const int64_t i = i0 + wr * (kKrRows / 2) + rt * 16 + g + 4 * r;
-                if (i < K && p < P) {
-                    const double w = acc[rt][ct][r];
-                    f += cond[i] * w;
-                    e += V[i * P + p] * w;
-                }
+            for (int ct = 0; ct < 4; ++ct) acc[rt][ct] = f64x4{0.0, 0.0, 0.0, 0.0};
+
+        // columns left of the diagonal block carry nothing in M2.  The K steps run
+        // from the last one down to the diagonal block: every row tile starts at the
+        // same rows of V, so the row tiles of one column tile (resident together on
+        // one XCD) read each V row at about the same time and share it through L2.
+        const int64_t jlast = ((K - 1) / kKrK) * kKrK;
+        load(jlast);
+        for (int64_t j0 = jlast; j0 >= i0; j0 -= kKrK) {
+            __syncthreads();
+            store();
+            __syncthreads();
+            if (j0 - kKrK >= i0) load(j0 - kKrK);
+#pragma unroll
+            for (int kk = 0; kk < kKrK; kk += 4) {
+                double a[kKrRF], b[4];
+#pragma unroll
+                for (int rt = 0; rt < kKrRF; ++rt) a[rt] = As[(wr * (kKrRows / 2) + rt * 16 + l16) * kKrApad + kk + g];
+#pragma unroll
+                for (int ct = 0; ct < 4; ++ct) b[ct] = Bs[(kk + g) * kKrBpad + wc * 64 + ct * 16 + l16];
+#pragma unroll
+                for (int rt = 0; rt < kKrRF; ++rt)
+#pragma unroll
+                    for (int ct = 0; ct < 4; ++ct)
+                        acc[rt][ct] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[rt], b[ct], acc[rt][ct], 0, 0, 0);
             }
-        // the four lane groups holding this column, in a fixed order
-        f += __shfl_xor(f, 16);
-        e += __shfl_xor(e, 16);
-        f += __shfl_xor(f, 32);
-        e += __shfl_xor(e, 32);
-        sf[ct] = f;
-        se[ct] = e;
-    }
-    if (g == 0) {
+        }
+
+        // epilogue: per column, sum over this wave's kKrRows / 2 rows of V o W
+        double se[4];
 #pragma unroll
         for (int ct = 0; ct < 4; ++ct) {
-            red[wr][0][wc * 64 + ct * 16 + l16] = sf[ct];
-            red[wr][1][wc * 64 + ct * 16 + l16] = se[ct];
+            const int64_t p = p0 + wc * 64 + ct * 16 + l16;
+            double e = 0.0;
+#pragma unroll
+            for (int rt = 0; rt < kKrRF; ++rt)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int64_t i = i0 + wr * (kKrRows / 2) + rt * 16 + g + 4 * r;
+                    if (i < K && p < P) e += V[i * P + p] * acc[rt][ct][r];
+                }
+            // the four lane groups holding this column, in a fixed order
+            e += __shfl_xor(e, 16);
+            e += __shfl_xor(e, 32);
+            se[ct] = e;
         }
-    }
-    __syncthreads();
-    if (t < kKrPts) {
-        const int64_t p = p0 + t;
-        if (p < P) {
-            part_f[rt_idx * P + p] = red[0][0][t] + red[1][0][t];
-            part_e[rt_idx * P + p] = red[0][1][t] + red[1][1][t];
+        if (g == 0) {
+#pragma unroll
+            for (int ct = 0; ct < 4; ++ct) red[wr][wc * 64 + ct * 16 + l16] = se[ct];
         }
+        __syncthreads();
+        if (t < kKrPts) {
+            const int64_t p = p0 + t;
+            if (p < P) part_e[rt_idx * P + p] = red[0][t] + red[1][t];
+        }
+        __syncthreads();  // red is rewritten by the next tile's epilogue
     }
 }
 
-__global__ void k_krige_reduce(int64_t tiles, int64_t P, const double* __restrict__ part_f,
-                               const double* __restrict__ part_e, double* __restrict__ field,
+__global__ void k_krige_reduce(int64_t tiles, int64_t P, const double* __restrict__ part_e,
                                double* __restrict__ error) {
     const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= P) return;
-    double f = 0.0, e = 0.0;
-    for (int64_t it = 0; it < tiles; ++it) {
-        f += part_f[it * P + p];
-        e += part_e[it * P + p];
-    }
-    field[p] = f;
+    double e = 0.0;
+    for (int64_t it = 0; it < tiles; ++it) e += part_e[it * P + p];
     error[p] = e;
 }
 
@@ -294,8 +310,10 @@ __global__ void k_fill_zero_f64(double* __restrict__ p, int64_t n) {
 
 inline int64_t krige_row_tiles(int64_t K) { return ceil_div(K, (int64_t)kKrRows); }
 
+// field: the weight vector (K); variance form: the row-tile partials
+// (tiles x P), the weight vector and M2 (K x K)
 inline int64_t krige_ws_bytes(int64_t K, int64_t P, bool variance) {
-    if (variance) return 2 * krige_row_tiles(K) * P * (int64_t)sizeof(double);
+    if (variance) return (krige_row_tiles(K) * P + K + K * K) * (int64_t)sizeof(double);
     return K * (int64_t)sizeof(double);
 }
 
@@ -324,33 +342,40 @@ int gsx_krige_f64(int64_t K, int64_t P, const double* krig_mat, const double* kr
     const bool variance = error != nullptr;
     if (!workspace || workspace_bytes < gsx::krige_ws_bytes(K, P, variance)) return GSX_E_WORKSPACE;
     double* ws = static_cast<double*>(workspace);
-    if (!variance) {
-        hipLaunchKernelGGL(gsx::k_krige_weights, dim3((unsigned)gsx::ceil_div(K, (int64_t)256)), dim3(256), 0, s,
-                           K, krig_mat, cond, ws);
-        int st = gsx::launch_status();
-        if (st != GSX_OK) return st;
-        const bool vec = P % 2 == 0 && reinterpret_cast<uintptr_t>(krig_vecs) % 16 == 0 &&
-                         reinterpret_cast<uintptr_t>(field) % 16 == 0;
-        if (vec)
-            hipLaunchKernelGGL(gsx::k_krige_field4, dim3((unsigned)gsx::ceil_div(P, (int64_t)1024)), dim3(256), 0, s,
-                               K, P, ws, krig_vecs, field);
-        else
-            hipLaunchKernelGGL(gsx::k_krige_field, dim3((unsigned)gsx::ceil_div(P, (int64_t)512)), dim3(256), 0, s,
-                               K, P, ws, krig_vecs, field);
-        return gsx::launch_status();
-    }
     const int64_t tiles = gsx::krige_row_tiles(K);
     const int64_t col_blocks = gsx::ceil_div(P, (int64_t)gsx::kKrPts);
-    if (tiles * (col_blocks + 8) > 0x7fffffffLL) return GSX_E_INVALID;
-    double* part_f = ws;
-    double* part_e = ws + tiles * P;
-    const int64_t col8 = gsx::ceil_div(col_blocks, (int64_t)8) * 8;
-    hipLaunchKernelGGL(gsx::k_krige_gemm, dim3((unsigned)(col8 * tiles)), dim3(256), 0, s, K, P, tiles, col_blocks,
-                       krig_mat, krig_vecs, cond, part_f, part_e);
+    if (variance && (tiles * (col_blocks + 8) > 0x7fffffffLL || gsx::ceil_div(K * K, (int64_t)256) > 0x7fffffffLL))
+        return GSX_E_INVALID;
+    double* part_e = ws;
+    double* a = variance ? ws + tiles * P : ws;
+    // field = (M^T cond) . v
+    hipLaunchKernelGGL(gsx::k_krige_weights, dim3((unsigned)gsx::ceil_div(K, (int64_t)256)), dim3(256), 0, s, K,
+                       krig_mat, cond, a);
     int st = gsx::launch_status();
     if (st != GSX_OK) return st;
+    const bool vec = P % 2 == 0 && reinterpret_cast<uintptr_t>(krig_vecs) % 16 == 0 &&
+                     reinterpret_cast<uintptr_t>(field) % 16 == 0;
+    if (vec)
+        hipLaunchKernelGGL(gsx::k_krige_field4, dim3((unsigned)gsx::ceil_div(P, (int64_t)1024)), dim3(256), 0, s, K,
+                           P, a, krig_vecs, field);
+    else
+        hipLaunchKernelGGL(gsx::k_krige_field, dim3((unsigned)gsx::ceil_div(P, (int64_t)512)), dim3(256), 0, s, K,
+                           P, a, krig_vecs, field);
+    st = gsx::launch_status();
+    if (st != GSX_OK || !variance) return st;
+    // error = v^T M2 v over the block upper triangle
+    double* m2 = a + K;
+    hipLaunchKernelGGL(gsx::k_krige_sym, dim3((unsigned)gsx::ceil_div(K * K, (int64_t)256)), dim3(256), 0, s, K,
+                       krig_mat, m2);
+    st = gsx::launch_status();
+    if (st != GSX_OK) return st;
+    const int64_t col8 = gsx::ceil_div(col_blocks, (int64_t)8) * 8;
+    hipLaunchKernelGGL(gsx::k_krige_gemm, dim3((unsigned)(col8 * ((tiles + 1) / 2))), dim3(256), 0, s, K, P, tiles,
+                       col_blocks, m2, krig_vecs, part_e);
+    st = gsx::launch_status();
+    if (st != GSX_OK) return st;
     hipLaunchKernelGGL(gsx::k_krige_reduce, dim3((unsigned)gsx::ceil_div(P, (int64_t)256)), dim3(256), 0, s, tiles,
-                       P, part_f, part_e, field, error);
+                       P, part_e, error);
     return gsx::launch_status();
 }
 
