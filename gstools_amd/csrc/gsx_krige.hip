@@ -206,13 +206,34 @@ __global__ void k_krige_reduce(int64_t tiles, int64_t P, const double* __restric
     error[p] = e;
 }
 
-// a_j = sum_i cond_i M[i, j]
-__global__ void k_krige_weights(int64_t K, const double* __restrict__ M, const double* __restrict__ cond,
-                                double* __restrict__ a) {
-    const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+// a_j = sum_i cond_i M[i, j] in two passes: partial sums over kWChunks row
+// chunks (block = 64 columns x 4 waves, each wave every 4th row of the chunk,
+// the waves added in a fixed order), then the chunks in order
+constexpr int kWChunks = 16;
+
+__global__ __launch_bounds__(256) void k_krige_weights_part(int64_t K, const double* __restrict__ M,
+                                                            const double* __restrict__ cond,
+                                                            double* __restrict__ part) {
+    __shared__ double red[4][64];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int64_t j = (int64_t)blockIdx.x * 64 + lane;
+    const int64_t rows = (K + kWChunks - 1) / kWChunks;
+    const int64_t r0 = (int64_t)blockIdx.y * rows;
+    const int64_t r1 = r0 + rows < K ? r0 + rows : K;
+    double s = 0.0;
+    if (j < K)
+        for (int64_t i = r0 + wave; i < r1; i += 4) s += cond[i] * M[i * K + j];
+    red[wave][lane] = s;
+    __syncthreads();
+    if (wave == 0 && j < K) part[blockIdx.y * K + j] = ((red[0][lane] + red[1][lane]) + red[2][lane]) + red[3][lane];
+}
+
+__global__ __launch_bounds__(256) void k_krige_weights_sum(int64_t K, const double* __restrict__ part,
+                                                           double* __restrict__ a) {
+    const int64_t j = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (j >= K) return;
     double s = 0.0;
-    for (int64_t i = 0; i < K; ++i) s += cond[i] * M[i * K + j];
+    for (int c = 0; c < kWChunks; ++c) s += part[c * K + j];
     a[j] = s;
 }
 
@@ -310,11 +331,12 @@ __global__ void k_fill_zero_f64(double* __restrict__ p, int64_t n) {
 
 inline int64_t krige_row_tiles(int64_t K) { return ceil_div(K, (int64_t)kKrRows); }
 
-// field: the weight vector (K); variance form: the row-tile partials
-// (tiles x P), the weight vector and M2 (K x K)
+// field: the weight vector (K) and its row-chunk partials (kWChunks x K);
+// variance form: also the row-tile partials (tiles x P) and M2 (K x K)
 inline int64_t krige_ws_bytes(int64_t K, int64_t P, bool variance) {
-    if (variance) return (krige_row_tiles(K) * P + K + K * K) * (int64_t)sizeof(double);
-    return K * (int64_t)sizeof(double);
+    const int64_t field = (1 + kWChunks) * K;
+    if (variance) return (krige_row_tiles(K) * P + field + K * K) * (int64_t)sizeof(double);
+    return field * (int64_t)sizeof(double);
 }
 
 }  // namespace gsx
@@ -348,10 +370,15 @@ int gsx_krige_f64(int64_t K, int64_t P, const double* krig_mat, const double* kr
         return GSX_E_INVALID;
     double* part_e = ws;
     double* a = variance ? ws + tiles * P : ws;
+    double* wpart = a + K;
     // field = (M^T cond) . v
-    hipLaunchKernelGGL(gsx::k_krige_weights, dim3((unsigned)gsx::ceil_div(K, (int64_t)256)), dim3(256), 0, s, K,
-                       krig_mat, cond, a);
+    hipLaunchKernelGGL(gsx::k_krige_weights_part, dim3((unsigned)gsx::ceil_div(K, (int64_t)64), gsx::kWChunks),
+                       dim3(256), 0, s, K, krig_mat, cond, wpart);
     int st = gsx::launch_status();
+    if (st != GSX_OK) return st;
+    hipLaunchKernelGGL(gsx::k_krige_weights_sum, dim3((unsigned)gsx::ceil_div(K, (int64_t)256)), dim3(256), 0, s, K,
+                       wpart, a);
+    st = gsx::launch_status();
     if (st != GSX_OK) return st;
     const bool vec = P % 2 == 0 && reinterpret_cast<uintptr_t>(krig_vecs) % 16 == 0 &&
                      reinterpret_cast<uintptr_t>(field) % 16 == 0;
@@ -364,7 +391,7 @@ int gsx_krige_f64(int64_t K, int64_t P, const double* krig_mat, const double* kr
     st = gsx::launch_status();
     if (st != GSX_OK || !variance) return st;
     // error = v^T M2 v over the block upper triangle
-    double* m2 = a + K;
+    double* m2 = wpart + gsx::kWChunks * K;
     hipLaunchKernelGGL(gsx::k_krige_sym, dim3((unsigned)gsx::ceil_div(K * K, (int64_t)256)), dim3(256), 0, s, K,
                        krig_mat, m2);
     st = gsx::launch_status();
