@@ -127,6 +127,15 @@ __device__ __forceinline__ int bin_from_guess(const double* e, int B, double x, 
     return g;
 }
 
+// One pair into slot idx of this thread's histogram row: LDS adds without a
+// return value.  The row is the thread's own and adds to one location are
+// ordered as the program issues them, so every slot adds its pairs in pair
+// order (the same roundings as reading, adding and writing back each pair).
+__device__ __forceinline__ void hist_add(double* h_sum, unsigned* h_cnt, int idx, double v) {
+    __hip_atomic_fetch_add(&h_sum[idx], v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    __hip_atomic_fetch_add(&h_cnt[idx], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
 // FAST: one field, no directions; a block whose edges are non-decreasing runs a
 // straight-line pair loop (the general loop covers everything else).
 // CRESSIE selects the estimator of the FAST loop at compile time (a runtime flag
@@ -322,28 +331,7 @@ __global__ __launch_bounds__(kVT) void k_vario_pairs(VarioArgs a, const double* 
                             val[u] = v;
                         }
 #pragma unroll
-                        for (int grp = 0; grp < 8; grp += 4) {
-                            double hs[4];
-                            unsigned hc[4];
-#pragma unroll
-                            for (int u = 0; u < 4; ++u) {
-                                hs[u] = h_sum[slot[grp + u] * kVT + t];
-                                hc[u] = h_cnt[slot[grp + u] * kVT + t];
-                            }
-#pragma unroll
-                            for (int u = 1; u < 4; ++u)
-#pragma unroll
-                                for (int w2 = 0; w2 < u; ++w2) {
-                                    const bool m = slot[grp + u] == slot[grp + w2];
-                                    hs[u] = m ? hs[w2] + val[grp + w2] : hs[u];
-                                    hc[u] = m ? hc[w2] + 1u : hc[u];
-                                }
-#pragma unroll
-                            for (int u = 0; u < 4; ++u) {
-                                h_sum[slot[grp + u] * kVT + t] = hs[u] + val[grp + u];
-                                h_cnt[slot[grp + u] * kVT + t] = hc[u] + 1u;
-                            }
-                        }
+                        for (int u = 0; u < 8; ++u) hist_add(h_sum, h_cnt, slot[u] * kVT + t, val[u]);
                     }
                 };
                 const bool clean = !tile_nan && !__any((xj_nan || !__builtin_isfinite(fj0) || a.F < 1) ? 1 : 0);
@@ -429,26 +417,8 @@ __global__ __launch_bounds__(kVT) void k_vario_pairs(VarioArgs a, const double* 
                             const unsigned sl = (unsigned)(d * a.B + g[u] - a.h0);
                             slot[u] = (in && sl < (unsigned)NHi) ? (int)sl : NHi;
                         }
-                        double hs[4];
-                        unsigned hc[4];
 #pragma unroll
-                        for (int u = 0; u < 4; ++u) {
-                            hs[u] = h_sum[slot[u] * kVT + t];
-                            hc[u] = h_cnt[slot[u] * kVT + t];
-                        }
-#pragma unroll
-                        for (int u = 1; u < 4; ++u)
-#pragma unroll
-                            for (int w2 = 0; w2 < u; ++w2) {
-                                const bool m = slot[u] == slot[w2];
-                                hs[u] = m ? hs[w2] + val[w2] : hs[u];
-                                hc[u] = m ? hc[w2] + 1u : hc[u];
-                            }
-#pragma unroll
-                        for (int u = 0; u < 4; ++u) {
-                            h_sum[slot[u] * kVT + t] = hs[u] + val[u];
-                            h_cnt[slot[u] * kVT + t] = hc[u] + 1u;
-                        }
+                        for (int u = 0; u < 4; ++u) hist_add(h_sum, h_cnt, slot[u] * kVT + t, val[u]);
                     }
                 }
                 continue;
