@@ -15,11 +15,14 @@ subsample, the dominant kernel's roofline from HIP events, and the CPU
 baseline (the oracle's C/OpenMP restatement of the reference loop) timed on
 a bounded sample of the same workload.
 
-Extra sections (``--extras``, on by default): structured 512^3 Matern grid
-on the matrix cores, the periodic Fourier method on a 4096^2 grid, ordinary
-kriging sums (1000 conditioning points onto 1e6 targets), variogram pair
-binning (1e6 points), forward + adjoint
-on the 2D Stable configuration and the 256-realization batch; each prints its own JSON object on stderr.
+Extra sections (``--extras``, on by default): the 100 x 100 2D Gaussian
+grid (N 1000, FP64), structured 512^3 Matern grid on the matrix cores, the
+incompressible 128^3 grid, the periodic Fourier method on a 4096^2 grid,
+ordinary kriging sums (1000 conditioning points onto 1e6 targets), variogram
+pair binning (1e6 points), forward + adjoint on the 2D Stable configuration
+and the 256-realization batch; each prints its own JSON object on stderr, and
+the sections of the reference's configurations carry the CPU restatement
+timed on a bounded sample (scaled linearly: the work is linear in the points).
 """
 
 import argparse
@@ -128,7 +131,74 @@ def cpu_baseline(rm, iso, target_s=10.0, single_core_s=5.0):
     }
 
 
+def _cpu_restatement(fn, cov, z1, z2, pos, *extra, target_s=2.0):
+    """The oracle's C/OpenMP restatement of the reference loop (``fn``: summate_c,
+    summate_incompr_c or summate_adjoint_c) on all cores of this process's
+    affinity mask, over the first points of ``pos`` that take about target_s;
+    returns (mode.points/s, points timed, seconds, threads)."""
+    cores = len(os.sched_getaffinity(0))
+    n_modes = cov.shape[1]
+    cut = lambda n: [np.ascontiguousarray(a[..., :n]) for a in (pos,) + extra]
+    probe = min(pos.shape[1], 4000)
+    t0 = time.perf_counter()
+    fn(cov, z1, z2, *cut(probe), nthreads=cores)
+    rate = probe * n_modes / (time.perf_counter() - t0)
+    npts = int(min(pos.shape[1], max(probe, rate * target_s / n_modes)))
+    args = cut(npts)
+    t0 = time.perf_counter()
+    fn(cov, z1, z2, *args, nthreads=cores)
+    dt = time.perf_counter() - t0
+    return npts * n_modes / dt, npts, dt, cores
+
+
 MFMA_PEAK_TFLOPS = 2500.0  # dense bf16 / fp16
+
+
+def bench_config1(torch, steps, warmup):
+    """2D Gaussian var 1 len 10, mode_no 1000 (inversion sampler, seed 20170519) on the
+    100 x 100 grid x, y = 0..99, FP64: the device sum, the numpy drop-in and the
+    CPU restatement of the reference loop."""
+    import oracle
+    from gstools_amd import Gaussian, RandMeth, summate_device
+
+    log("bench: 100 x 100 grid, N 1000, fp64")
+    model = Gaussian(dim=2, var=1.0, len_scale=10.0)
+    rm = RandMeth(model, mode_no=1000, seed=20170519)
+    x = np.arange(100.0)
+    pos = np.ascontiguousarray(np.asarray(np.meshgrid(x, x, indexing="ij")).reshape(2, -1))
+    dev = torch.device("cuda", 0)
+    cov, z1, z2, post = (torch.from_numpy(a).to(dev) for a in (rm._cov_sample, rm._z_1, rm._z_2, pos))
+    out = torch.empty(pos.shape[1], dtype=torch.float64, device=dev)
+    work = float(pos.shape[1]) * 1000
+    wall, ker = time_kernel(lambda: summate_device(cov, z1, z2, post, "fp64", out=out), steps, warmup, torch)
+    rm(pos)  # numpy in, numpy out (the reference's call), after a warm-up
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        field = rm(pos)
+    wall_np = (time.perf_counter() - t0) / steps
+    cores = len(os.sched_getaffinity(0))
+    c = (rm._cov_sample, rm._z_1, rm._z_2, pos)
+    ref = oracle.summate_c(*c, nthreads=cores)
+    t0 = time.perf_counter()
+    for _ in range(3):
+        oracle.summate_c(*c, nthreads=cores)
+    t_all = (time.perf_counter() - t0) / 3
+    t0 = time.perf_counter()
+    oracle.summate_c(*c, nthreads=1)
+    t_one = time.perf_counter() - t0
+    scale = float(np.sqrt(model.var / 1000))
+    return {
+        "workload": "2D Gaussian var 1 len 10 (len_rescaled 11.284), mode_no 1000, inversion sampler, seed "
+                    "20170519, structured 100 x 100 (x, y = 0..99), fp64",
+        "mode_points": work,
+        "device_ms": wall * 1e3, "device_value": work / wall,
+        "numpy_dropin_ms": wall_np * 1e3,
+        "parity_vs_oracle_rel_max": oracle.rel_errors(out.cpu().numpy(), ref)[0],
+        "dropin_vs_oracle_rel_max": oracle.rel_errors(field, scale * ref)[0],
+        "cpu_restatement": {"value": work / t_all, "unit": "mode·points/s", "cores": cores,
+                            "ms": t_all * 1e3, "single_core_value": work / t_one, "single_core_ms": t_one * 1e3,
+                            "sample": "the whole grid, FP64"},
+    }
 
 
 def bench_mfma_grid(torch, steps, warmup):
@@ -169,6 +239,13 @@ def bench_mfma_grid(torch, steps, warmup):
     res["fp64_direct"] = {"value": 64 * 512 * 512 * 4096 / wall64, "measured_on": "64 of 512 x-planes"}
     res["speedup_bf16x3_vs_fp64"] = res["bf16x3"]["value"] / res["fp64_direct"]["value"]
     res["speedup_fp16x3_vs_fp64"] = res["fp16x3"]["value"] / res["fp64_direct"]["value"]
+    # CPU restatement on the x = 0 plane (the work is exactly linear in the points)
+    plane = np.asarray(np.meshgrid(ax[:1], ax, ax, indexing="ij")).reshape(3, -1)
+    rate, npts, dt, cores = _cpu_restatement(oracle.summate_c, rm._cov_sample, rm._z_1, rm._z_2, plane)
+    res["cpu_restatement"] = {"value": rate, "unit": "mode·points/s", "cores": cores,
+                              "sample": f"first {npts} points of the x = 0 plane x 4096 modes, FP64, {dt:.1f} s",
+                              "full_field_estimate_s": work / rate,
+                              "speedup_bf16x3_vs_cpu": res["bf16x3"]["value"] / rate}
     # FP64 GEMM form (precision "fp64gemm"): 32 column chunks of 8192 columns
     out64 = torch.empty(n, dtype=torch.float64, device=dev)
     fg = lambda: summate_grid_device(cov, z1, z2, axes, "fp64gemm", out=out64)
@@ -459,6 +536,15 @@ def bench_adjoint(torch, steps, warmup):
     a64 = [t.cpu().numpy() for t in summate_adjoint_device(cov, z1, z2, pobs, torch.tensor(gobs, device=dev), "fp64")]
     ora = oracle.summate_adjoint_c(rm._cov_sample, rm._z_1, rm._z_2, pobs.cpu().numpy(), gobs)
     rel = lambda a, b: float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
+    # CPU restatement (forward and adjoint loops) on a prefix of the points
+    pos_h, g_h = pos.cpu().numpy(), g.cpu().numpy().astype(np.float64)
+    c = (rm._cov_sample, rm._z_1, rm._z_2)
+    rate_f, n_f, dt_f, cores = _cpu_restatement(oracle.summate_c, *c, pos_h)
+    rate_a, n_a, dt_a, _ = _cpu_restatement(oracle.summate_adjoint_c, *c, pos_h, g_h)
+    cpu = {"forward_value": rate_f, "adjoint_value": rate_a, "unit": "mode·points/s", "cores": cores,
+           "sample": f"forward: first {n_f} points x 2048 modes ({dt_f:.1f} s); adjoint: first {n_a} points "
+                     f"({dt_a:.1f} s); FP64",
+           "forward_plus_adjoint_estimate_ms": (work / rate_f + work / rate_a) * 1e3}
     return {
         "workload": "unstructured 2D Stable alpha 1.5 len 15, mode_no 2048 (MCMC modes), 1e6 points in [0,500]^2, fp32",
         "stable_density": C4_DENSITY_NOTE,
@@ -468,6 +554,8 @@ def bench_adjoint(torch, steps, warmup):
         "forward_value": work / wall_f,
         "grad_rel_err_fp32_vs_fp64": [rel(a32[i], a64[i]) for i in range(3)],
         "grad_rel_err_fp64_vs_oracle": [rel(a64[i], ora[i]) for i in range(3)],
+        "cpu_restatement": cpu,
+        "speedup_fwd_adj_vs_cpu": cpu["forward_plus_adjoint_estimate_ms"] / (wall_fa * 1e3),
     }
 
 
@@ -651,6 +739,29 @@ def bench_batch(torch, steps, warmup, specs, modes, sampling_info=None):
         ref = oracle.summate_incompr_c(cov, z1, z2, p) if it.incompr else oracle.summate_c(cov, z1, z2, p)[None]
         got = it.out[:, idx].cpu().numpy().astype(np.float64)
         mixed_errs.append(max(oracle.rel_errors(got[c], ref[c])[1] for c in range(got.shape[0])))
+    # CPU restatement on a stratified sample: every 16th realization, its first 20000 points
+    cores = len(os.sched_getaffinity(0))
+    sample_work, sample_s = 0.0, 0.0
+    for r in range(0, len(items), 16):
+        cov, z1, z2 = modes[r]
+        it = items[r]
+        idx = np.arange(min(20000, it.P))
+        if it.axes is not None:
+            ax = [a.cpu().numpy() for a in torch.split(it.axes, list(it.shape[:it.d]))]
+            p = np.ascontiguousarray(np.asarray(np.meshgrid(*ax, indexing="ij")).reshape(it.d, -1)[:, idx])
+        else:
+            p = np.ascontiguousarray(it.pos.cpu().numpy()[:, idx])
+        fn = oracle.summate_incompr_c if it.incompr else oracle.summate_c
+        t0 = time.perf_counter()
+        fn(cov, z1, z2, p, nthreads=cores)
+        sample_s += time.perf_counter() - t0
+        sample_work += float(idx.size) * cov.shape[1] * (it.d if it.incompr else 1)
+    cpu_rate = sample_work / sample_s
+    work_weighted = sum(float(it.P) * it.N * (it.d if it.incompr else 1) for it in items)
+    cpu_batch = {"value": cpu_rate, "unit": "mode·points/s (incompressible: per component)", "cores": cores,
+                 "sample": f"every 16th realization ({len(range(0, len(items), 16))}), first 20000 points of each, "
+                           f"FP64, {sample_s:.1f} s",
+                 "full_batch_estimate_s": work_weighted / cpu_rate}
     mixed = {"precision": "bf16x3 grids (128 2D + 32 incompressible 3D), fp32 point sets, one HIP graph",
              "graph_ms": mixed_wall * 1e3, "value": work / mixed_wall, "speedup_vs_fp32_graph": graph_wall / mixed_wall,
              "default_plans_graph_ms": default_wall * 1e3, "rel_rms_vs_oracle_spot": mixed_errs,
@@ -672,6 +783,8 @@ def bench_batch(torch, steps, warmup, specs, modes, sampling_info=None):
         "batched_equals_single_launches": bool(same),
         "rel_max_vs_oracle_spot": errs,
         "mixed_matrix_cores": mixed,
+        "cpu_restatement": cpu_batch,
+        "speedup_mixed_vs_cpu": cpu_batch["full_batch_estimate_s"] / mixed_wall,
         "mode_sampling": sampling_info,
     }
 
@@ -708,8 +821,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=100)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--extras", type=int, default=1, help="also run the other configurations")
-    ap.add_argument("--only", choices=["fp32", "fp64", "mfma", "incompr", "fourier", "krige", "variogram", "adjoint",
-                                       "batch", "copy"],
+    ap.add_argument("--only", choices=["fp32", "fp64", "config1", "mfma", "incompr", "fourier", "krige", "variogram",
+                                       "adjoint", "batch", "copy"],
                     help="run one section only (for profiling) and print its JSON")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -731,8 +844,9 @@ def main():
     dev = torch.device("cuda", 0)
     k_ext = max(3, args.steps // 4)
     w_ext = max(1, min(args.warmup, 3))  # the extra sections run long kernels
-    if args.only in ("mfma", "incompr", "fourier", "krige", "variogram", "adjoint", "batch", "copy"):
-        fn = {"mfma": lambda: bench_mfma_grid(torch, k_ext, w_ext),
+    if args.only in ("config1", "mfma", "incompr", "fourier", "krige", "variogram", "adjoint", "batch", "copy"):
+        fn = {"config1": lambda: bench_config1(torch, k_ext, w_ext),
+              "mfma": lambda: bench_mfma_grid(torch, k_ext, w_ext),
               "incompr": lambda: bench_incompr_grid(torch, k_ext, w_ext),
               "variogram": lambda: bench_variogram(torch, k_ext, w_ext),
               "krige": lambda: bench_krige(torch, k_ext, w_ext),
@@ -819,7 +933,8 @@ def main():
     }
     if args.extras:
         extras = {}
-        sections = (("mfma_grid", bench_mfma_grid), ("incompr_grid", bench_incompr_grid), ("fourier", bench_fourier),
+        sections = (("config1", bench_config1),
+                    ("mfma_grid", bench_mfma_grid), ("incompr_grid", bench_incompr_grid), ("fourier", bench_fourier),
                     ("krige", bench_krige),
                     ("variogram", bench_variogram),
                     ("adjoint", bench_adjoint),
