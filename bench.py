@@ -472,9 +472,18 @@ def bench_variogram(torch, steps, warmup):
     dt = torch.from_numpy(dirs).to(dev)
     drun = lambda: gv._pairs(ft[:, :200_000], et, pt[:, :200_000], "m", dt, np.pi / 8, 20.0, True)
     dwall, _ = time_kernel(drun, 2, 1, torch)
-    lag_f = torch.from_numpy(rs.normal(size=(1024, 100_000))).to(dev)
+    lag_np = rs.normal(size=(1024, 100_000))
+    lag_f = torch.from_numpy(lag_np).to(dev)
     lrun = lambda: gv._lags(lag_f, None, "m")
     lwall, _ = time_kernel(lrun, 3, 1, torch)
+    # spot lags against numpy (sum of squared differences / (2 count), counts closed-form)
+    lg, lc = (t.cpu().numpy() for t in lrun())
+    lag_err = 0.0
+    for k in (1, 2, 7, 31, 32, 33, 500, 1023):
+        d = lag_np[k:] - lag_np[:-k]
+        ref = np.sum(d * d) / (2.0 * d.size)
+        lag_err = max(lag_err, abs(lg[k] - ref) / ref)
+    lag_counts_ok = bool(np.array_equal(lc[1:], (1024 - np.arange(1, 1024)) * 100_000))
     sub_pairs = 3000 * 2999 / 2
     return {
         "workload": "unstructured 3D variogram, 1e6 points uniform in [0,1000]^3, one field, 20 bins to 577, "
@@ -490,6 +499,8 @@ def bench_variogram(torch, steps, warmup):
         "directional_pairs_per_s": 200_000 * 199_999 / 2 / dwall,
         "axis_lags_1024x100000_ms": lwall * 1e3,
         "axis_lag_pairs_per_s": (1024 * 1023 / 2) * 100_000 / lwall,
+        "axis_lags_rel_max_vs_numpy_8_lags": lag_err,
+        "axis_lag_counts_exact": lag_counts_ok,
     }
 
 

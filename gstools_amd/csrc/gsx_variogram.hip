@@ -599,6 +599,150 @@ __global__ __launch_bounds__(256) void k_vario_lags(int64_t n0, int64_t n1, int6
     }
 }
 
+// Wide arrays (n1 >= 64): block = 64 columns x 4 KW lags over one chunk of rows;
+// wave w takes lags kb = k0 + KW w .. kb + KW - 1 of the block's group, lane c
+// column 64 t + c.  A thread walks its column's rows KW at a time with the KW
+// values f[i .. i + KW - 1] and the 2 KW - 1 values f[i + kb .. i + kb + 2 KW - 2]
+// in registers (3 KW - 1 loads serve KW^2 pairs, each pair the same difference
+// as in k_vario_lags; the rows are addressed from wave-uniform row pointers with
+// 32-bit lane offsets).  KW = 16 without a mask, 8 with one (its flags take
+// registers too).  Sums run over rows in order per thread, then over the 64
+// columns (fixed shuffle order), then over the blocks
+// (k_vario_lag_collect_wave).  Unmasked counts are closed-form: (rows with a
+// pair) x (columns) per block.
+constexpr int kLagWMax = 16;
+
+template <int kLagW, bool MASK, bool CRESSIE>
+__global__ __launch_bounds__(256) void k_vario_lags_win(int64_t n0, int64_t n1, int64_t ctiles, int64_t rchunks,
+                                                        int64_t rows_per, const double* __restrict__ f,
+                                                        const unsigned char* __restrict__ mask,
+                                                        double* __restrict__ part_sum,
+                                                        unsigned long long* __restrict__ part_cnt) {
+    const int lane = (int)threadIdx.x & 63, wave = (int)threadIdx.x >> 6;
+    const int64_t b = blockIdx.x;
+    const int64_t rc = b % rchunks, rest = b / rchunks;
+    const int64_t ct = rest % ctiles, lg = rest / ctiles;
+    const int64_t kb = lg * 4 * kLagW + __builtin_amdgcn_readfirstlane(wave) * kLagW;  // this wave's first lag
+    const int64_t j = ct * 64 + lane;
+    const bool col_ok = j < n1;
+    const int jc = (int)(col_ok ? j : n1 - 1);  // loads of lanes past the last column stay in bounds
+    const int n1i = (int)n1;
+    const int64_t i_lo = rc * rows_per;
+    const int64_t i_hi = i_lo + rows_per < n0 ? i_lo + rows_per : n0;
+    const int64_t i_end = i_hi < n0 - kb ? i_hi : n0 - kb;  // rows with a pair at lag kb
+    double sum[kLagW];
+    unsigned cnt[kLagW];
+#pragma unroll
+    for (int t = 0; t < kLagW; ++t) {
+        sum[t] = 0.0;
+        cnt[t] = 0u;
+    }
+    for (int64_t i0 = i_lo; i0 < i_end; i0 += kLagW) {
+        double a[kLagW], w[2 * kLagW - 1];
+        bool ma[kLagW], mw[2 * kLagW - 1];
+        // rows i0 .. and i0 + kb .. (clamped at the last row: those pairs are skipped)
+        const int64_t ra = i0, rw = i0 + kb;
+        const int na = (int)(n0 - ra < kLagW ? n0 - ra : kLagW);                // rows of a in the array
+        const int nw = (int)(n0 - rw < 2 * kLagW - 1 ? n0 - rw : 2 * kLagW - 1);  // rows of w in the array
+        const double* fa = f + ra * n1;
+        const double* fw = f + rw * n1;
+#pragma unroll
+        for (int u = 0; u < kLagW; ++u) {
+            const int o = (u < na ? u : na - 1) * n1i + jc;
+            a[u] = fa[o];
+            if constexpr (MASK) ma[u] = mask[ra * n1 + o] != 0;
+        }
+#pragma unroll
+        for (int v = 0; v < 2 * kLagW - 1; ++v) {
+            const int o = (v < nw ? v : nw - 1) * n1i + jc;
+            w[v] = fw[o];
+            if constexpr (MASK) mw[v] = mask[rw * n1 + o] != 0;
+        }
+        // every pair (i0 + u, kb + t) exists: rows in the chunk, i + k < n0, k > 0
+        const bool full = kb > 0 && i0 + kLagW <= i_hi && nw == 2 * kLagW - 1;
+#pragma unroll
+        for (int u = 0; u < kLagW; ++u)
+#pragma unroll
+            for (int t = 0; t < kLagW; ++t) {
+                const double d = w[u + t] - a[u];
+                const double e = CRESSIE ? sqrt(fabs(d)) : d * d;
+                bool ok = full || (i0 + u < i_hi && u + t < nw && kb + t > 0);
+                if constexpr (MASK) ok = ok && !ma[u] && !mw[u + t];
+                if (MASK || !full) {
+                    sum[t] += ok ? e : 0.0;
+                    cnt[t] += ok ? 1u : 0u;
+                } else {
+                    sum[t] += e;
+                }
+            }
+    }
+    const int64_t cols = n1 - ct * 64 < 64 ? n1 - ct * 64 : 64;
+    const int64_t T = ctiles * rchunks;
+#pragma unroll
+    for (int t = 0; t < kLagW; ++t) {
+        double v = col_ok ? sum[t] : 0.0;
+        unsigned c = col_ok ? cnt[t] : 0u;
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+            v += __shfl_xor(v, o);
+            if constexpr (MASK) c += __shfl_xor(c, o);
+        }
+        const int64_t k = kb + t;
+        if (lane == 0 && k < n0) {
+            part_sum[k * T + ct * rchunks + rc] = v;
+            if constexpr (MASK) {
+                part_cnt[k * T + ct * rchunks + rc] = c;
+            } else {
+                const int64_t top = i_hi < n0 - k ? i_hi : n0 - k;
+                const int64_t rows = (k > 0 && top > i_lo) ? top - i_lo : 0;
+                part_cnt[k * T + ct * rchunks + rc] = (unsigned long long)(rows * cols);
+            }
+        }
+    }
+}
+
+// k_vario_lag_collect for many partials per lag: one wave per lag, lane l adding
+// partials l, l + 64, ... in order, then the 64 lanes in a fixed shuffle order
+__global__ __launch_bounds__(256) void k_vario_lag_collect_wave(int64_t n0, int64_t tiles,
+                                                                const double* __restrict__ part_sum,
+                                                                const unsigned long long* __restrict__ part_cnt,
+                                                                double* __restrict__ gamma,
+                                                                long long* __restrict__ counts) {
+    const int lane = (int)threadIdx.x & 63;
+    const int64_t k = (int64_t)blockIdx.x * 4 + ((int)threadIdx.x >> 6);
+    if (k >= n0) return;  // whole wave
+    double s = 0.0;
+    unsigned long long c = 0;
+    for (int64_t b = lane; b < tiles; b += 64) {
+        s += part_sum[k * tiles + b];
+        c += part_cnt[k * tiles + b];
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        s += __shfl_xor(s, o);
+        c += __shfl_xor(c, o);
+    }
+    if (lane == 0) {
+        gamma[k] = s;
+        counts[k] = (long long)c;
+    }
+}
+
+// row chunks of the wide-array lag kernel: enough blocks to fill the chip
+// (from 64-lag groups for either lag width, so that both share one workspace size)
+inline int64_t lag_win_rchunks(int64_t n0, int64_t n1) {
+    const int64_t groups = ceil_div(n0, (int64_t)64) * ceil_div(n1, (int64_t)64);
+    int64_t rc = ceil_div((int64_t)4096, groups);
+    const int64_t max_rc = ceil_div(n0, (int64_t)256);  // at least 256 rows per chunk
+    if (rc > max_rc) rc = max_rc;
+    if (rc > 64) rc = 64;
+    return rc < 1 ? 1 : rc;
+}
+
+inline int64_t lag_win_rows(int64_t n0, int64_t rchunks) {
+    return ceil_div(ceil_div(n0, rchunks), (int64_t)kLagWMax) * kLagWMax;
+}
+
 __global__ void k_vario_lag_collect(int64_t n0, int64_t tiles, const double* __restrict__ part_sum,
                                     const unsigned long long* __restrict__ part_cnt, double* __restrict__ gamma,
                                     long long* __restrict__ counts) {
@@ -756,14 +900,59 @@ int vario_pairs(int64_t F, int64_t P, int64_t dim, int64_t B, int64_t D, const d
     return launch_status();
 }
 
+// structured lags of an (n0, n1 >= 64) array with the register-window kernel
+inline int lags_wide(int64_t n0, int64_t n1, const double* f, const uint8_t* mask, int estimator, double* gamma,
+                     int64_t* counts, void* workspace, int64_t workspace_bytes, void* stream) {
+    const int64_t ctiles = ceil_div(n1, (int64_t)64), rchunks = lag_win_rchunks(n0, n1);
+    const int64_t rows_per = lag_win_rows(n0, rchunks);
+    const int64_t T = ctiles * rchunks;
+    const int kw = mask ? 8 : 16;  // lags per wave
+    const int64_t blocks = ceil_div(n0, (int64_t)(4 * kw)) * T;
+    // 32-bit lane offsets within 2 kw rows
+    if (blocks > 0x7fffffffLL || n1 > 0x7fffffffLL / (2 * kLagWMax)) return GSX_E_UNSUPPORTED;
+    if (!workspace || workspace_bytes < n0 * T * 16) return GSX_E_WORKSPACE;
+    hipStream_t s = as_stream(stream);
+    double* part_sum = static_cast<double*>(workspace);
+    unsigned long long* part_cnt = reinterpret_cast<unsigned long long*>(part_sum + n0 * T);
+    const unsigned char* m = mask;
+    const dim3 grid((unsigned)blocks), block(256);
+    if (mask) {
+        if (estimator)
+            hipLaunchKernelGGL((k_vario_lags_win<8, true, true>), grid, block, 0, s, n0, n1, ctiles, rchunks, rows_per,
+                               f, m, part_sum, part_cnt);
+        else
+            hipLaunchKernelGGL((k_vario_lags_win<8, true, false>), grid, block, 0, s, n0, n1, ctiles, rchunks, rows_per,
+                               f, m, part_sum, part_cnt);
+    } else {
+        if (estimator)
+            hipLaunchKernelGGL((k_vario_lags_win<16, false, true>), grid, block, 0, s, n0, n1, ctiles, rchunks,
+                               rows_per, f, m, part_sum, part_cnt);
+        else
+            hipLaunchKernelGGL((k_vario_lags_win<16, false, false>), grid, block, 0, s, n0, n1, ctiles, rchunks,
+                               rows_per, f, m, part_sum, part_cnt);
+    }
+    int st = launch_status();
+    if (st != GSX_OK) return st;
+    long long* cnt = reinterpret_cast<long long*>(counts);
+    hipLaunchKernelGGL(k_vario_lag_collect_wave, dim3((unsigned)ceil_div(n0, (int64_t)4)), dim3(256), 0, s, n0, T,
+                       part_sum, part_cnt, gamma, cnt);
+    st = launch_status();
+    if (st != GSX_OK) return st;
+    hipLaunchKernelGGL(k_vario_normalize, dim3((unsigned)ceil_div(n0, (int64_t)256)), dim3(256), 0, s, n0, estimator,
+                       gamma, cnt);
+    return launch_status();
+}
+
 }  // namespace gsx
 
 extern "C" {
 
 int64_t gsx_vario_workspace(int64_t kind, int64_t P_or_n0, int64_t dim_or_n1, int64_t B, int64_t D) {
-    if (kind == 2) {  // structured: n0 lags x column tiles
+    if (kind == 2) {  // structured: n0 lags x column tiles (x row chunks for wide arrays)
         if (P_or_n0 < 0 || dim_or_n1 < 0) return GSX_E_INVALID;
         const int64_t tiles = gsx::ceil_div(dim_or_n1, (int64_t)256);
+        if (dim_or_n1 >= 64)
+            return P_or_n0 * gsx::ceil_div(dim_or_n1, (int64_t)64) * gsx::lag_win_rchunks(P_or_n0, dim_or_n1) * 16;
         return P_or_n0 * (tiles > 0 ? tiles : 1) * 16;
     }
     if (P_or_n0 < 0 || dim_or_n1 < 1 || dim_or_n1 > gsx::kVMaxDim || B < 0 || D < 1) return GSX_E_INVALID;
@@ -800,6 +989,7 @@ int gsx_vario_structured_f64(int64_t n0, int64_t n1, const double* f, const uint
     if (n0 < 0 || n1 < 0 || (estimator != 0 && estimator != 1)) return GSX_E_INVALID;
     if (n0 == 0) return GSX_OK;
     if (!gamma || !counts || (n1 > 0 && !f)) return GSX_E_INVALID;
+    if (n1 >= 64) return gsx::lags_wide(n0, n1, f, mask, estimator, gamma, counts, workspace, workspace_bytes, stream);
     const int64_t tiles = gsx::ceil_div(n1, (int64_t)256) > 0 ? gsx::ceil_div(n1, (int64_t)256) : 1;
     // columns per tile: 256, or the next power of two >= n1 (then one tile)
     int cols_log2 = 0;

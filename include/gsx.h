@@ -331,8 +331,9 @@ int gsx_krige_f64(int64_t K, int64_t P, const double* krig_mat,
  * direction line; separate_dirs != 0 gives each pair to the first matching
  * direction; gamma, counts are (D, B).  Structured: lags 0..n0-1 along the
  * first axis of an (n0, n1) array; mask (n0, n1) bytes, non-zero = masked,
- * or NULL; gamma, counts (n0); any n0 with n0 * ceil(n1 / 256) < 2^31
- * (GSX_E_UNSUPPORTED beyond).  Counts are exact; sums are reduced in a
+ * or NULL; gamma, counts (n0); n1 < 64: any n0 with n0 * ceil(n1 / 256) <
+ * 2^31, n1 >= 64: fewer than 2^31 blocks of 64 columns x 32 or 64 lags and
+ * n1 < 2^26 (GSX_E_UNSUPPORTED beyond).  Counts are exact; sums are reduced in a
  * fixed order (bitwise reproducible).  Workspace bytes:
  * gsx_vario_workspace(kind, P, dim, B, D) for kind 0 (unstructured) or 1
  * (directional), gsx_vario_workspace(2, n0, n1, 0, 1) for structured.

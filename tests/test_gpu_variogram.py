@@ -149,7 +149,8 @@ def test_directional_slot_chunks():
 
 
 @pytest.mark.parametrize("n0,n1,masked,est", [(10, 1, False, "m"), (300, 5000, False, "c"), (257, 1031, True, "m"),
-                                             (1000, 3, True, "c"), (777, 1, True, "m"), (64, 200, False, "m")])
+                                             (1000, 3, True, "c"), (777, 1, True, "m"), (64, 200, False, "m"),
+                                             (2000, 64, False, "m"), (1100, 130, True, "c"), (33, 65, True, "m")])
 def test_structured_matches_oracle(n0, n1, masked, est):
     _say(f"vario structured {n0}x{n1} masked={masked}")
     rs = np.random.RandomState(n0)
