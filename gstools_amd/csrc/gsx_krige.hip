@@ -305,7 +305,11 @@ __device__ __forceinline__ double cov_of(const CovModelArgs& m, double r) {
     return m.var * c;
 }
 
-// out[i * ld + p] = C(|c_i - x_p|), i < K, p < P
+// out[i * ld + p] = C(|c_i - x_p|), i < K, p < P.  A thread keeps its target in
+// registers and writes kCovRows rows (conditioning points i0 .. i0 + kCovRows - 1,
+// read as wave-uniform values), so the targets are read once per kCovRows rows.
+constexpr int kCovRows = 16;
+
 template <int DIM>
 __global__ __launch_bounds__(256) void k_krige_cov(CovModelArgs m, int64_t K, int64_t P,
                                                    const double* __restrict__ cpos,
@@ -313,15 +317,22 @@ __global__ __launch_bounds__(256) void k_krige_cov(CovModelArgs m, int64_t K, in
                                                    int64_t ld) {
 #pragma clang fp contract(off)
     const int64_t p = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    const int64_t i = blockIdx.y;
+    const int64_t i0 = (int64_t)blockIdx.y * kCovRows;
     if (p >= P) return;
-    double d2 = 0.0;
+    double x[DIM];
 #pragma unroll
-    for (int d = 0; d < DIM; ++d) {
-        const double t = cpos[d * K + i] - pos[d * P + p];
-        d2 += t * t;
+    for (int d = 0; d < DIM; ++d) x[d] = pos[d * P + p];
+    const int rows = (int)(K - i0 < kCovRows ? K - i0 : kCovRows);
+    for (int r = 0; r < rows; ++r) {
+        const int64_t i = i0 + r;
+        double d2 = 0.0;
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) {
+            const double t = cpos[d * K + i] - x[d];
+            d2 += t * t;
+        }
+        out[i * ld + p] = cov_of(m, sqrt(d2));
     }
-    out[i * ld + p] = cov_of(m, sqrt(d2));
 }
 
 __global__ void k_fill_zero_f64(double* __restrict__ p, int64_t n) {
@@ -416,7 +427,7 @@ int gsx_krige_cov_f64(int model_kind, double var, double len_rescaled, double al
     if (!cond_pos || !pos || !out) return GSX_E_INVALID;
     gsx::CovModelArgs m{model_kind, var, len_rescaled, alpha, sill_at_zero};
     hipStream_t s = gsx::as_stream(stream);
-    const dim3 grid((unsigned)gsx::ceil_div(P, (int64_t)256), (unsigned)K);
+    const dim3 grid((unsigned)gsx::ceil_div(P, (int64_t)256), (unsigned)gsx::ceil_div(K, (int64_t)gsx::kCovRows));
     switch (dim) {
         case 1: hipLaunchKernelGGL((gsx::k_krige_cov<1>), grid, dim3(256), 0, s, m, K, P, cond_pos, pos, out, ld_out); break;
         case 2: hipLaunchKernelGGL((gsx::k_krige_cov<2>), grid, dim3(256), 0, s, m, K, P, cond_pos, pos, out, ld_out); break;
