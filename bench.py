@@ -773,10 +773,18 @@ def bench_batch(torch, steps, warmup, specs, modes, sampling_info=None):
                  "sample": f"every 16th realization ({len(range(0, len(items), 16))}), first 20000 points of each, "
                            f"FP64, {sample_s:.1f} s",
                  "full_batch_estimate_s": work_weighted / cpu_rate}
-    mixed = {"precision": "bf16x3 grids (128 2D + 32 incompressible 3D), fp32 point sets, one HIP graph",
+    # the same launches on one stream (no graph branches): the same outputs bit for bit
+    out4 = [it.out.clone() for it in items]
+    mb1 = MixedBatch(items, "bf16x3", *best, tune=True, branches=1)
+    one_wall, _ = time_kernel(lambda: mb1.launch(graph=True), steps, warmup, torch)
+    branches_equal = all(torch.equal(it.out, o) for it, o in zip(items, out4))
+    mixed = {"precision": "bf16x3 grids (128 2D + 32 incompressible 3D) on 4 graph branches beside the fp32 point "
+                          "sets' persistent launch, one HIP graph",
              "graph_ms": mixed_wall * 1e3, "value": work / mixed_wall, "speedup_vs_fp32_graph": graph_wall / mixed_wall,
              "default_plans_graph_ms": default_wall * 1e3, "rel_rms_vs_oracle_spot": mixed_errs,
-             "mfma_tuning": mb.tuning_summary(), "mfma_tuning_s_first_use": tune_s}
+             "mfma_tuning": mb.tuning_summary(), "mfma_tuning_s_first_use": tune_s,
+             "branches": mb.branches, "one_stream_graph_ms": one_wall * 1e3,
+             "branches_equal_one_stream": bool(branches_equal)}
     return {
         "workload": "256 realizations: 128 2D Gaussian grids (side log-U[64,2048]), 96 3D Exponential point "
                     "sets (log-U[1e4,1e6]), 32 incompressible 3D Gaussian 128^3 grids; mode_no in "

@@ -111,12 +111,18 @@ class MixedBatch:
     HIP graph.
 
     Grid items run ``gsx_summate_grid_mfma`` / ``gsx_summate_incompr_grid_mfma`` in
-    split bf16 ("bf16x3") or fp16 ("fp16x3") with one shared workspace (the launches
-    are ordered on one stream); outputs are the unscaled FP32 sums, as in
+    split bf16 ("bf16x3") or fp16 ("fp16x3"), one workspace per branch (the launches
+    of a branch are ordered on its stream); outputs are the unscaled FP32 sums, as in
     :class:`BatchPlan`.  The graph holds kernel launches only (no memset nodes).
+
+    ``branches`` > 1 (default 4): the grid items are split over that many side streams
+    (longest work first, each branch with its own workspace), forked from the
+    caller's stream with an event and joined back, while the persistent launch
+    runs on the caller's stream; captured, the graph keeps the branches
+    parallel.  Each item's output is the same as with one branch, bit for bit.
     """
 
-    def __init__(self, items, precision="bf16x3", tile_points=0, mode_chunk=0, tune=True):
+    def __init__(self, items, precision="bf16x3", tile_points=0, mode_chunk=0, tune=True, branches=4):
         """``tune``: each grid item runs the matrix-core plan the autotuner chose
         for its class (timed on first use, then cached; see :mod:`.autotune`)."""
         if precision not in ("bf16x3", "fp16x3"):
@@ -128,14 +134,26 @@ class MixedBatch:
         self.plan = BatchPlan(rest, tile_points, mode_chunk) if rest else None
         lib = _native.lib()
         self._calls = []
-        need = 1
+        if int(branches) < 1:
+            raise ValueError("branches must be >= 1")
+        self.branches = int(branches)
+        # grid items over the branches: longest work first, each to the least loaded
+        load = [0.0] * self.branches
+        need = [1] * self.branches
+        order = sorted(self.grid_items, key=lambda it: -float(it.P) * it.N * (it.d if it.incompr else 1))
+        branch_of = {}
+        for it in order:
+            b = min(range(self.branches), key=lambda i: load[i])
+            load[b] += float(it.P) * it.N * (it.d if it.incompr else 1)
+            branch_of[id(it)] = b
         for it in self.grid_items:
             shape = (ctypes.c_int64 * 3)(*it.shape)
             q = lib.gsx_summate_incompr_grid_mfma_workspace if it.incompr else lib.gsx_summate_grid_mfma_workspace
             nbytes = int(q(it.d, it.N, shape))
             if nbytes < 0:
                 _native.check(nbytes, "mixed batch workspace")
-            need = max(need, nbytes)
+            b = branch_of[id(it)]
+            need[b] = max(need[b], nbytes)
             variant = self.variant
             if it.d == 2 and not it.incompr and uniform_axis(it.axes[it.shape[0]:it.shape[0] + it.shape[1]]):
                 variant |= _native.MFMA_UNIFORM_LAST  # factorised last axis
@@ -143,9 +161,12 @@ class MixedBatch:
                 from .autotune import mfma_variant
 
                 variant = mfma_variant(it.d, it.N, it.shape[:it.d], variant, comps=it.d if it.incompr else 1)
-            self._calls.append((it, shape, variant))
+            self._calls.append((it, shape, variant, branch_of[id(it)]))
         self.tuned = bool(tune)
-        self.ws = torch.empty(need, dtype=torch.uint8, device=self.items[0].out.device if self.items else "cuda")
+        dev = self.items[0].out.device if self.items else "cuda"
+        self.ws = [torch.empty(n, dtype=torch.uint8, device=dev) for n in need]
+        # side streams at the default priority, one per branch when branching
+        self.streams = [torch.cuda.Stream(device=dev) for _ in range(self.branches)] if self.branches > 1 else []
         self.graph = None
 
     def tuning_summary(self):
@@ -153,7 +174,7 @@ class MixedBatch:
         from .autotune import _lookup, _mfma_key
 
         out = {}
-        for it, _, variant in self._calls:
+        for it, _, variant, _ in self._calls:
             base = variant & (_native.MFMA_FP16X3 | _native.MFMA_UNIFORM_LAST)
             key = _mfma_key(it.d, it.N, it.shape[:it.d], it.d if it.incompr else 1, base)
             entry = _lookup(key)
@@ -164,24 +185,47 @@ class MixedBatch:
                                          "default_ms": round(entry["default_ms"], 4),
                                          "spread_ms": [round(min(ms.values()), 4), round(max(ms.values()), 4)],
                                          "candidates": entry["candidates"], "items": 0}
-        for it, _, variant in self._calls:
+        for it, _, variant, _ in self._calls:
             base = variant & (_native.MFMA_FP16X3 | _native.MFMA_UNIFORM_LAST)
             k = _mfma_key(it.d, it.N, it.shape[:it.d], it.d if it.incompr else 1, base).split("|", 1)[1]
             if k in out:
                 out[k]["items"] += 1
         return out
 
-    def _run(self):
+    def _grid_calls(self, branch, stream):
         lib = _native.lib()
-        s = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
-        ws, nws = ctypes.c_void_p(self.ws.data_ptr()), self.ws.numel()
-        for it, shape, variant in self._calls:
+        s = ctypes.c_void_p(stream.cuda_stream)
+        ws = self.ws[branch]
+        for it, shape, variant, b in self._calls:
+            if b != branch:
+                continue
             fn = lib.gsx_summate_incompr_grid_mfma if it.incompr else lib.gsx_summate_grid_mfma
             _native.check(fn(it.d, it.N, shape, it.axes.data_ptr(), it.cov.data_ptr(), it.z1.data_ptr(),
-                             it.z2.data_ptr(), variant, 1.0, it.out.data_ptr(), ws, nws, s),
+                             it.z2.data_ptr(), variant, 1.0, it.out.data_ptr(), ctypes.c_void_p(ws.data_ptr()),
+                             ws.numel(), s),
                           "mixed batch grid item")
+
+    def _run(self):
+        main = torch.cuda.current_stream()
+        if self.branches == 1:
+            self._grid_calls(0, main)
+            if self.plan is not None:
+                self.plan.launch(graph=False)
+            return
+        # fork: every branch waits for the work already on the caller's stream
+        fork = torch.cuda.Event()
+        fork.record(main)
+        joins = []
+        for b, st in enumerate(self.streams):
+            st.wait_event(fork)
+            self._grid_calls(b, st)
+            ev = torch.cuda.Event()
+            ev.record(st)
+            joins.append(ev)
         if self.plan is not None:
-            self.plan.launch(graph=False)
+            self.plan.launch(graph=False)  # on the caller's stream, beside the branches
+        for ev in joins:
+            main.wait_event(ev)  # join
 
     def launch(self, graph=False):
         if not graph:

@@ -122,6 +122,47 @@ def test_mixed_batch_grids_on_matrix_cores(precision):
             assert torch.equal(it.out, e), f"replay {rep} differs"
 
 
+def test_mixed_batch_branches_equal_one_branch():
+    """Grid items split over side streams (forked and joined with events) give
+    each item the same output as one branch, bit for bit, eager and in graph
+    replays (with eager work and synchronization between the replays)."""
+    _say("mixed batch branches")
+    rs = np.random.RandomState(7)
+    items = []
+    for k, shape in enumerate([(64, 70), (200, 256), (333, 17), (48, 40, 36), (128, 130), (20, 21, 22)]):
+        d, n = len(shape), int(rs.choice([300, 1000, 2048]))
+        cov = rs.normal(size=(d, n)) * 0.3
+        axes = [np.arange(m, dtype=np.float64) * 0.7 for m in shape]
+        items.append(BatchItem(cov, rs.normal(size=n), rs.normal(size=n), axes=axes, incompr=(k == 5)))
+    for k in range(3):
+        cov = rs.normal(size=(3, 500)) * 0.3
+        items.append(BatchItem(cov, rs.normal(size=500), rs.normal(size=500),
+                               pos=rs.uniform(0, 100, size=(3, 4000 + 777 * k))))
+    one = MixedBatch(items, "bf16x3", tune=False, branches=1)
+    one.launch()
+    torch.cuda.synchronize()
+    ref = [it.out.clone() for it in items]
+    mb = MixedBatch(items, "bf16x3", tune=False, branches=3)
+    assert len({b for *_, b in mb._calls}) == 3
+    for it in items:
+        it.out.fill_(float("nan"))
+    mb.launch()
+    torch.cuda.synchronize()
+    for it, r in zip(items, ref):
+        assert torch.equal(it.out, r)
+    for rep in range(3):
+        for it in items:
+            it.out.fill_(float("nan"))
+        _ = summate_device(items[6].cov, items[6].z1, items[6].z2, items[6].pos, "fp32")
+        torch.cuda.synchronize()
+        mb.launch(graph=True)
+        torch.cuda.synchronize()
+        for it, r in zip(items, ref):
+            assert torch.equal(it.out, r), f"replay {rep} differs"
+    with pytest.raises(ValueError):
+        MixedBatch(items, "bf16x3", tune=False, branches=0)
+
+
 def test_batch_empty_and_invalid():
     plan = BatchPlan([], 1024, 256)
     assert plan.tiles == 0
