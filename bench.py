@@ -805,7 +805,76 @@ def bench_batch(torch, steps, warmup, specs, modes, sampling_info=None):
         "cpu_restatement": cpu_batch,
         "speedup_mixed_vs_cpu": cpu_batch["full_batch_estimate_s"] / mixed_wall,
         "mode_sampling": sampling_info,
+        "sampling_and_sums": _end_to_end(torch, specs),
     }
+
+
+def _end_to_end(torch, specs, reps=3):
+    """Sampling and sums of the 256 realizations, end to end: sampling then the
+    sums, against :func:`gstools_amd.batch.sample_and_sum` (each group's sums
+    launched as soon as its modes are final, beside the device chains).
+    Positions and axes are built on the device beforehand; both runs build
+    their items and batches inside the timing and launch eagerly."""
+    from gstools_amd import Exponential
+    from gstools_amd.batch import BatchItem, MixedBatch, sample_and_sum
+    from gstools_amd.sampling import sample_modes_many
+
+    dev = torch.device("cuda", 0)
+    expo = Exponential(dim=3, var=2.0, len_scale=[50, 20, 5], angles=[0.5, 0.2, 0.1])
+    kinds = ("grid2d", "points3d", "incompr3d")
+    idx = {k: [i for i, sp in enumerate(specs) if sp[0] == k] for k in kinds}
+    groups = [(_model_of(k), [specs[i][1] for i in idx[k]], [specs[i][2] for i in idx[k]]) for k in kinds]
+    geo = {}
+    for i, (kind, n, seed, size) in enumerate(specs):
+        if kind == "points3d":
+            pos = np.random.RandomState(seed).uniform(0.0, 1000.0, size=(3, size))
+            geo[i] = torch.from_numpy(np.ascontiguousarray(expo.isometrize(pos))).to(dev)
+        else:
+            geo[i] = [np.arange(size, dtype=np.float64)] * (2 if kind == "grid2d" else 3)
+
+    def items_of(g, modes):
+        # one upload per group: each realization's cov, z1, z2 as contiguous views of three flat buffers
+        k = kinds[g]
+        flat = [torch.from_numpy(np.concatenate([np.ravel(m[c]) for m in modes])).to(dev) for c in range(3)]
+        out, o, oz = [], 0, 0
+        for i, (cov, z1, z2) in zip(idx[k], modes):
+            d, n = cov.shape
+            args = (flat[0][o:o + d * n].view(d, n), flat[1][oz:oz + n], flat[2][oz:oz + n])
+            o, oz = o + d * n, oz + n
+            if k == "points3d":
+                out.append(BatchItem(*args, pos=geo[i]))
+            else:
+                out.append(BatchItem(*args, axes=geo[i], incompr=k == "incompr3d"))
+        return out
+
+    def sequential():
+        modes = sample_modes_many(groups)
+        its = [it for g in range(len(kinds)) for it in items_of(g, modes[g])]
+        mb = MixedBatch(its, "bf16x3", tune=True)
+        mb.launch(graph=False)
+        torch.cuda.synchronize()
+        return its
+
+    def overlapped():
+        _, its, keep = sample_and_sum(groups, items_of)
+        torch.cuda.synchronize()
+        return [it for g in its for it in g]
+
+    seq_s, ovl_s = [], []
+    for rep in range(reps + 1):  # the first round warms the sampler and the tuning cache
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        a = sequential()
+        seq_s.append(time.perf_counter() - t0)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        b = overlapped()
+        ovl_s.append(time.perf_counter() - t0)
+    same = len(a) == len(b) and all(torch.equal(x.out, y.out) for x, y in zip(a, b))
+    return {"sequential_ms": 1e3 * float(np.median(seq_s[1:])), "overlapped_ms": 1e3 * float(np.median(ovl_s[1:])),
+            "outputs_equal": bool(same),
+            "note": "sampling (native draws, MCMC chains on the device) and the MixedBatch sums, eager launches; "
+                    "overlapped: the 2D grids (inversion sampling, drawn on the host) sum beside the chains"}
 
 
 def _grid_pos(torch, axes):

@@ -16,7 +16,7 @@ import torch
 from . import _native
 from .summate import uniform_axis
 
-__all__ = ["BatchItem", "BatchPlan", "MixedBatch", "autotune", "TILE_VARIANTS"]
+__all__ = ["BatchItem", "BatchPlan", "MixedBatch", "sample_and_sum", "autotune", "TILE_VARIANTS"]
 
 TILE_VARIANTS = [(512, 128), (512, 256), (1024, 128), (1024, 256), (2048, 128), (2048, 256)]
 
@@ -43,6 +43,9 @@ class BatchItem:
             self.shape = (int(self.pos.shape[1]), 1, 1)
             self.P = int(self.pos.shape[1])
         else:
+            # host copies of axes given as host arrays (checked without a device round trip)
+            self.axes_host = None if any(isinstance(a, torch.Tensor) for a in axes) else [
+                np.asarray(a, dtype=np.float64).reshape(-1) for a in axes]
             ax = [as_dev(a).reshape(-1) for a in axes]
             if len(ax) != self.d:
                 raise ValueError("one axis per dimension")
@@ -155,7 +158,8 @@ class MixedBatch:
             b = branch_of[id(it)]
             need[b] = max(need[b], nbytes)
             variant = self.variant
-            if it.d == 2 and not it.incompr and uniform_axis(it.axes[it.shape[0]:it.shape[0] + it.shape[1]]):
+            last = it.axes_host[1] if getattr(it, "axes_host", None) else it.axes[it.shape[0]:it.shape[0] + it.shape[1]]
+            if it.d == 2 and not it.incompr and uniform_axis(last):
                 variant |= _native.MFMA_UNIFORM_LAST  # factorised last axis
             if tune:
                 from .autotune import mfma_variant
@@ -239,6 +243,46 @@ class MixedBatch:
                 self._run()
             self.graph = g
         self.graph.replay()
+
+
+def sample_and_sum(groups, items_of, precision="bf16x3", sampling="auto", nthreads=None, device=None,
+                   tune=True):
+    """Mode sampling and the batched sums of several groups of realizations,
+    overlapped.  ``groups`` are ``(model, mode_nos, seeds)`` as for
+    :func:`~gstools_amd.sampling.sample_modes_many`; ``items_of(i, modes)``
+    returns the :class:`BatchItem` list of group i for its sampled modes.  Each
+    group's sums (grids on the matrix cores, the rest in one persistent FP32
+    launch) are launched on a stream of their own as soon as the group's modes
+    are final, so the groups drawn on the host run while the MCMC chains of the
+    others are still on the device.  Returns ``(modes, items, batches)``:
+    modes and items per group, and the :class:`MixedBatch` objects whose
+    launches fill the items' outputs (keep them until the outputs are read).
+    The outputs are complete once the caller's stream has synchronized (the
+    group streams are joined back into it)."""
+    from .sampling import sample_modes_many
+
+    main = torch.cuda.current_stream()
+    start = torch.cuda.Event()
+    start.record(main)  # the work before this call; the device chains come after it on `main`
+    items = [None] * len(groups)
+    streams, keep = [], []
+
+    def ready(i, modes):
+        st = torch.cuda.Stream()
+        st.wait_event(start)
+        with torch.cuda.stream(st):
+            its = items_of(i, modes)
+            if its:
+                mb = MixedBatch(its, precision, tune=tune)
+                mb.launch(graph=False)
+                keep.append(mb)
+        items[i] = its
+        streams.append(st)
+
+    modes = sample_modes_many(groups, sampling=sampling, nthreads=nthreads, device=device, on_ready=ready)
+    for st in streams:
+        main.wait_stream(st)  # join
+    return modes, items, keep
 
 
 def autotune(items, reps=3, graph=True, use_cache=True, verbose=False):

@@ -358,10 +358,13 @@ def sample_modes_batch(model, mode_nos, seeds, sampling="auto", nthreads=None, d
 
 
 def sample_modes_many(groups, sampling="auto", nthreads=None, device=None, nwalkers=50, burn_in=20,
-                      oversampling_factor=10):
+                      oversampling_factor=10, on_ready=None):
     """:func:`sample_modes_batch` for several ``(model, mode_nos, seeds)`` groups:
     the MCMC chains of every group on the device run in one launch while the
-    host draws and finishes all groups."""
+    host draws and finishes all groups.  ``on_ready(i, modes)``, if given, is
+    called for each group as soon as its modes are final: the groups drawn on
+    the host before the device chains are read back, so the caller can start
+    GPU work on them while the chains still run."""
     if sampling not in ("auto", "inversion", "mcmc"):
         raise ValueError("sample_modes_batch: sampling not in ['auto', 'inversion', 'mcmc']")
     threads = len(os.sched_getaffinity(0)) if nthreads is None else int(nthreads)
@@ -383,11 +386,15 @@ def sample_modes_many(groups, sampling="auto", nthreads=None, device=None, nwalk
             continue
         if not p["n"].size:
             out[i] = []
+            if on_ready is not None:
+                on_ready(i, out[i])
             continue
         if p["kind"] == 2 and p["md"] is None:
             p["rads"] = _chains_host(model.ln_spectral_rad_pdf, p["init"], p["burn"], p["prod"], burn_in,
                                      p["prod_steps"], p["picks"], nthreads=threads)
         out[i] = _finish(model, p)
+        if on_ready is not None and p.get("md") is None:
+            on_ready(i, out[i])
     if chains is not None:
         rads = chains.pick([pk for p in dev_groups for pk in p["picks"]])
         k = 0
@@ -396,11 +403,15 @@ def sample_modes_many(groups, sampling="auto", nthreads=None, device=None, nwalk
                 nr = p["init"].shape[0]
                 out[i] = [(r * s, a, b) for r, (s, a, b) in zip(rads[k:k + nr], out[i])]
                 k += nr
+                if on_ready is not None:
+                    on_ready(i, out[i])
     for i, ((model, mode_nos, seeds), p) in enumerate(zip(groups, preps)):
         if p is None:
             out[i] = _modes_python(model, [int(n) for n in mode_nos], list(seeds), _kind(model, sampling),
                                    _pick_device_model(model, _kind(model, sampling), device), nwalkers, burn_in,
                                    oversampling_factor, threads)
+            if on_ready is not None:
+                on_ready(i, out[i])
     return out
 
 

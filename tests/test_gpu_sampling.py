@@ -146,3 +146,48 @@ def test_device_many_models_one_launch():
     dev = sample_modes_many(groups, device=None)
     for (model, ns, ss), got in zip(groups, dev):
         _same(sample_modes_batch(model, ns, ss, device=False), got)
+
+
+def test_sample_and_sum_overlapped_equals_sequential():
+    """``batch.sample_and_sum`` (each group's sums launched as soon as its modes
+    are final, beside the device chains): the same modes as ``sample_modes_many``
+    and the same outputs, bit for bit, as one MixedBatch after the sampling."""
+    from bench import _batch_specs, _model_of
+    from gstools_amd.batch import BatchItem, MixedBatch, sample_and_sum
+    from gstools_amd.sampling import sample_modes_many
+
+    _say("sample_and_sum: overlapped vs sequential")
+    dev = torch.device("cuda", 0)
+    specs = _batch_specs()
+    kinds = ("grid2d", "points3d", "incompr3d")
+    idx = {k: [i for i, sp in enumerate(specs) if sp[0] == k][:12] for k in kinds}
+    groups = [(_model_of(k), [specs[i][1] for i in idx[k]], [specs[i][2] for i in idx[k]]) for k in kinds]
+    rs = np.random.RandomState(3)
+    geo = {}
+    for k in kinds:
+        for i in idx[k]:
+            size = min(specs[i][3], 40 if k == "incompr3d" else (300 if k == "grid2d" else 20000))
+            if k == "points3d":
+                geo[i] = torch.from_numpy(rs.uniform(0, 100, size=(3, size))).to(dev)
+            else:
+                geo[i] = [torch.arange(size, dtype=torch.float64, device=dev)] * (2 if k == "grid2d" else 3)
+
+    def items_of(g, modes):
+        k = kinds[g]
+        if k == "points3d":
+            return [BatchItem(*m, pos=geo[i]) for i, m in zip(idx[k], modes)]
+        return [BatchItem(*m, axes=geo[i], incompr=k == "incompr3d") for i, m in zip(idx[k], modes)]
+
+    seen = []
+    ref_modes = sample_modes_many(groups, on_ready=lambda i, m: seen.append(i))
+    assert sorted(seen) == [0, 1, 2]
+    seq = [it for g in range(3) for it in items_of(g, ref_modes[g])]
+    MixedBatch(seq, "bf16x3", tune=False).launch()
+    torch.cuda.synchronize()
+    modes, items, keep = sample_and_sum(groups, items_of, tune=False)
+    torch.cuda.synchronize()
+    _same([m for g in modes for m in g], [m for g in ref_modes for m in g])
+    got = [it for g in items for it in g]
+    assert len(got) == len(seq)
+    for a, b in zip(got, seq):
+        assert torch.equal(a.out, b.out)
