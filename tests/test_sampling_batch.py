@@ -186,3 +186,32 @@ def test_native_mode_draws_equal_numpy(dim, kind):
         for k in range(2):
             assert np.array_equal(states[r, k], _mt_state(stream()))
         assert np.array_equal(picks[sl], stream().choice(int(n_flat[r]), n[r]))
+
+
+def test_many_groups_on_ready_host():
+    """``sample_modes_many(..., on_ready)`` on the host sampler: the callback sees
+    every group once (empty groups included), with the modes the call returns."""
+    from gstools_amd.sampling import sample_modes_many
+
+    groups = [(Gaussian(dim=2, var=1.0, len_scale=10.0), [300, 50], [1, 2]),  # inversion
+              (Exponential(dim=3, var=2.0, len_scale=5.0), [], []),           # empty
+              (Matern(dim=3, var=1.0, len_scale=8.0, nu=1.5), [64, 80], [3, 4])]  # MCMC
+    seen = {}
+
+    def ready(i, modes):
+        assert i not in seen
+        seen[i] = [tuple(np.copy(a) for a in m) for m in modes]
+
+    out = sample_modes_many(groups, device=False, on_ready=ready)
+    assert sorted(seen) == [0, 1, 2]
+    for i, modes in enumerate(out):
+        assert len(seen[i]) == len(modes)
+        for a, b in zip(seen[i], modes):
+            for u, v in zip(a, b):
+                assert np.array_equal(u, v)
+    # the same modes as one group at a time
+    for (model, ns, ss), modes in zip(groups, out):
+        ref = sample_modes_batch(model, ns, ss, device=False)
+        for a, b in zip(ref, modes):
+            for u, v in zip(a, b):
+                assert np.array_equal(u, v)
