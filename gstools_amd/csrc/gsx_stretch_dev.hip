@@ -20,10 +20,10 @@
 //              accept iff (0 + lp(q)) - lp(x) > log(u_acc).
 //
 // The MT19937 key (624 words) of the chain sits in LDS.  The twist runs in
-// 64-word chunks in ascending order: word i reads words i + 1 (the next
-// chunk's first word is still old) and (i + 397) mod 624 (old for i < 227,
-// already new for i >= 227, at least 227 > 64 positions back), exactly the
-// sequential recurrence.  Draws read the key at the stream position, each
+// three dependency levels: word i reads words i, i + 1 (old, except word 0
+// for i = 623) and (i + 397) mod 624 (old for i < 227, new for i >= 227), so
+// [0, 227) needs only the old key, [227, 454) the first level and [454, 624)
+// the second: exactly the sequential recurrence.  Draws read the key at the stream position, each
 // lane tempering one word; the rejection loops of the masked draws test 64
 // candidate words at once and take the first acceptable ones (ballot and
 // rank), so the stream advances exactly as the sequential code's does.
@@ -74,20 +74,36 @@ struct WaveMt {
     uint32_t wv;    // window: lane l holds the tempered word wbase + l (l < wlen)
     int wbase, wlen;
 
-    __device__ void twist() {
+    // One dependency level of the twist, words [lo, hi): every lane reads its
+    // words into registers, then writes them after a barrier.
+    __device__ void twist_level(int lo, int hi) {
+        constexpr int kPer = (kMtN - kMtM + kWave - 1) / kWave;  // words per lane and level
         const int l = lane_id();
-        for (int i0 = 0; i0 < kMtN; i0 += kWave) {
-            const int i = i0 + l;
-            uint32_t nv = 0;
-            if (i < kMtN) {
+        uint32_t nv[kPer];
+#pragma unroll
+        for (int k = 0; k < kPer; ++k) {
+            const int i = lo + l + k * kWave;
+            nv[k] = 0;
+            if (i < hi) {
                 const uint32_t y = (key[i] & kUpper) | (key[i + 1 < kMtN ? i + 1 : 0] & kLower);
                 const int far = i + kMtM < kMtN ? i + kMtM : i + kMtM - kMtN;
-                nv = key[far] ^ (y >> 1) ^ ((y & 1u) ? kMatrixA : 0u);
+                nv[k] = key[far] ^ (y >> 1) ^ ((y & 1u) ? kMatrixA : 0u);
             }
-            __syncthreads();
-            if (i < kMtN) key[i] = nv;
-            __syncthreads();
         }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < kPer; ++k) {
+            const int i = lo + l + k * kWave;
+            if (i < hi) key[i] = nv[k];
+        }
+        __syncthreads();
+    }
+
+    __device__ void twist() {
+        constexpr int kLevel = kMtN - kMtM;  // 227
+        twist_level(0, kLevel);
+        twist_level(kLevel, 2 * kLevel);
+        twist_level(2 * kLevel, kMtN);
         pos = 0;
     }
 
