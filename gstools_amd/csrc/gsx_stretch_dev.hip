@@ -1,12 +1,13 @@
-// Ensemble MCMC of spectral radii on the GPU: one wavefront per chain.
+// Ensemble MCMC of spectral radii on the GPU: one block of two wavefronts per
+// chain.
 //
 // GSTools samples the radii of its spectral modes with emcee's stretch move
 // (src/gstools/random/rng.py:38-104, called from generator.py:380-385).  The
 // host restatement (gsx_stretch.cpp) draws every random number of a step
 // from the chain's legacy numpy MT19937 stream and advances the chains in
 // lockstep with one log-probability evaluation per half-move.  This file
-// runs the whole chain on the device instead: block r = one 64-lane wave =
-// chain r, lane w = walker w (nwalkers <= 64), for every step of burn-in and
+// runs the whole chain on the device instead: block r = chain r, lane w of
+// its walker wave = walker w (nwalkers <= 64), for every step of burn-in and
 // production, with the same draws in the same order:
 //
 //   per step:  one random_sample (the move choice, discarded);
@@ -23,10 +24,12 @@
 // three dependency levels: word i reads words i, i + 1 (old, except word 0
 // for i = 623) and (i + 397) mod 624 (old for i < 227, new for i >= 227), so
 // [0, 227) needs only the old key, [227, 454) the first level and [454, 624)
-// the second: exactly the sequential recurrence.  Draws read the key at the stream position, each
-// lane tempering one word; the rejection loops of the masked draws test 64
-// candidate words at once and take the first acceptable ones (ballot and
-// rank), so the stream advances exactly as the sequential code's does.
+// the second: exactly the sequential recurrence.  Draws read the key at the
+// stream position, each lane tempering one word; the rejection loops of the
+// masked draws test 64 candidate words at once and take the first acceptable
+// ones (ballot and rank), so the stream advances exactly as the sequential
+// code's does.  No draw depends on the walkers' positions, so the block's
+// stream wave draws step s + 1 while its walker wave applies step s.
 // Every arithmetic step of the move is rounded as the host's (no fused
 // multiply-add); the log-probability is the model's closed form in the
 // order numpy evaluates it, with the device's exp / log / pow (an accept
@@ -62,7 +65,20 @@ __device__ __forceinline__ uint32_t smear(uint32_t m) {
     return m;
 }
 
-__device__ __forceinline__ int lane_id() { return (int)threadIdx.x; }
+// A chain runs on two wavefronts: wave 0 holds the walkers, wave 1 the
+// MT19937 stream.
+constexpr int kChainThreads = 2 * kWave;
+
+__device__ __forceinline__ int lane_id() { return (int)(threadIdx.x & (kWave - 1)); }
+
+// Orders the LDS accesses of the lanes of one wavefront (a wave's LDS
+// operations execute in issue order; the fences keep the compiler from
+// moving accesses across this point).
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
 
 // One chain's MT19937 stream, shared by the 64 lanes of its wave.  With one
 // wave per SIMD every LDS round trip is exposed, so each draw routine reads
@@ -90,13 +106,13 @@ struct WaveMt {
                 nv[k] = key[far] ^ (y >> 1) ^ ((y & 1u) ? kMatrixA : 0u);
             }
         }
-        __syncthreads();
+        wave_sync();
 #pragma unroll
         for (int k = 0; k < kPer; ++k) {
             const int i = lo + l + k * kWave;
             if (i < hi) key[i] = nv[k];
         }
-        __syncthreads();
+        wave_sync();
     }
 
     __device__ void twist() {
@@ -155,43 +171,75 @@ struct WaveMt {
         wv = l < wlen ? temper(key[pos + l]) : 0u;
     }
 
-    // Fisher-Yates shuffle of the 1-bit labels of nw walkers: for i = nw-1
-    // down to 1, swap labels i and j = masked draw in [0, i].  Calls that
-    // share a mask are resolved together: call k of the run tries window
-    // word off + k against its own bound; every call before the first
-    // rejection is done (its word accepted), the rejecting call then retries
-    // from the next word -- the sequential consumption, several calls a pass.
-    __device__ void shuffle_labels(unsigned long long& labels, int nw) {
+    // Fisher-Yates shuffle of the labels w % 2 of nw walkers: for i = nw-1
+    // down to 1, swap positions i and J[i] = masked draw in [0, i] (words
+    // until (word & smear(i)) <= i).
+    //
+    // The draws: a window of 64 words is resolved with one ballot per
+    // rejection.  With r rejections before it, window word l is tried by
+    // the call i0 - (l - r), so the first lane from m on that fails under r
+    // is the next rejection, or the end of the shuffle where that call index
+    // drops below 1; each accepted word then writes J of its call.
+    //
+    // The swaps, in parallel: position i is final after step i and holds
+    // what position J[i] held then, which is position J[i]'s starting
+    // element unless a step k > i drew J[k] = J[i] (then, for the smallest
+    // such k, what position k held just before step k).  What position k
+    // holds just before step k follows the same rule from k, so it is the
+    // end of the chain k -> succ(k) -> ... with succ(k) the smallest k' > k
+    // drawing J[k'] = k; six rounds of pointer doubling find the chain ends
+    // (chains are shorter than 64), and label i is that element's label.
+    __device__ unsigned long long shuffle_labels(int nw, uint32_t* jv, unsigned long long* occ) {
         const int l = lane_id();
-        load_window();
+        const unsigned long long below = (l == 0) ? 0ull : (~0ull >> (64 - l));
         int i = nw - 1;
         while (i >= 1) {
-            const int off = pos - wbase;
-            if (off >= wlen) {
-                load_window();
-                continue;
+            load_window();
+            const int i0 = i;
+            unsigned long long rej = 0;
+            int r = 0, m = 0, used = wlen;
+            while (m < wlen) {
+                const int c = i0 - l + r;  // the call that tries word l after r rejections
+                const uint32_t mask = c >= 1 ? (0xffffffffu >> __clz(c)) : 0u;
+                const bool ok = l < wlen && c >= 1 && (wv & mask) <= (uint32_t)c;
+                const unsigned long long ev = ~__ballot(ok) & (~0ull << m);
+                const int e = ev ? __ffsll((long long)ev) - 1 : kWave;
+                if (e >= wlen) break;  // the rest of the window accepted
+                if (i0 - e + r < 1) {  // every call done before word e
+                    used = e;
+                    break;
+                }
+                rej |= 1ull << e;
+                ++r;
+                m = e + 1;
             }
-            const uint32_t mask = smear((uint32_t)i);
-            const int lo = mask > 1u ? (int)((mask + 1u) >> 1) : 1;  // smallest bound with this mask
-            const int k = l - off;
-            const bool cand = k >= 0 && k <= i - lo && l < wlen;
-            const bool acc = cand && (wv & mask) <= (uint32_t)(i - k);
-            const unsigned long long fail = __ballot(cand && !acc);
-            const unsigned long long cands = __ballot(cand);
-            const int done = fail ? (__ffsll((long long)fail) - 1 - off) : __popcll(cands);
-            for (int t = 0; t < done; ++t) {
-                const uint32_t j = (uint32_t)__builtin_amdgcn_readlane((int)wv, off + t) & mask;
-                const int b = i - t;
-                const unsigned long long bi = (labels >> b) & 1ull, bj = (labels >> j) & 1ull;
-                if (bi != bj) labels ^= (1ull << b) | (1ull << j);
+            if (l < used && !((rej >> l) & 1ull)) {
+                const int call = i0 - l + __popcll(rej & below);
+                jv[call] = wv & (0xffffffffu >> __clz(call));
             }
-            i -= done;
-            pos += done + (fail ? 1 : 0);
+            i = i0 - (used - __popcll(rej));
+            pos = wbase + used;
         }
+        occ[l] = 0ull;
+        wave_sync();
+        // J[0] = 0 closes position 0's rule; J[l] <= l < 64 (the mask only bounds the index)
+        const int j = (l >= 1 && l < nw) ? (int)(jv[l] & (kWave - 1)) : 0;
+        if (l < nw) __hip_atomic_fetch_or(&occ[j], 1ull << l, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+        wave_sync();
+        const unsigned long long above = l == kWave - 1 ? 0ull : (~0ull << (l + 1));
+        const unsigned long long same = occ[j] & above;  // later steps drawing position J[l]
+        const unsigned long long onto = occ[l] & above;  // later steps drawing position l
+        int end = onto ? __ffsll((long long)onto) - 1 : l;
+#pragma unroll
+        for (int round = 0; round < 6; ++round) end = __shfl(end, end);
+        const int via = __shfl(end, same ? __ffsll((long long)same) - 1 : l);
+        const int src = same ? via : j;
+        wave_sync();  // the next call rewrites jv and occ
+        return __ballot(l < nw && (src & 1));
     }
 
     // `count` masked draws in [0, m] (count <= 64): the draws are left in
-    // slot[0 .. count) of LDS (the caller reads them after a barrier); m == 0
+    // slot[0 .. count) of LDS (read by the other wave after a barrier); m == 0
     // draws nothing (every value is 0)
     __device__ void masked_batch(uint32_t m, int count, uint32_t* slot) {
         if (m == 0) return;
@@ -267,79 +315,150 @@ constexpr int kStretchOk = 0;
 constexpr int kStretchBadProposal = 1;  // infinite or NaN proposal
 constexpr int kStretchNanLogProb = 2;   // log-probability NaN
 
-__global__ __launch_bounds__(kWave) void k_stretch_chains(int64_t R, int nw, const uint32_t* __restrict__ states,
-                                                          int64_t burn_steps, const int64_t* __restrict__ prod_steps,
-                                                          int64_t max_steps, double a,
-                                                          const gsx_stretch_model* __restrict__ models,
-                                                          const double* __restrict__ init,
-                                                          double* __restrict__ chain, double* __restrict__ final_x,
-                                                          int* __restrict__ status) {
+// The random variates of one step.  They depend on the stream alone, never
+// on the walkers' positions: the shuffled labels and, per half-move, the
+// stretch variate, the partner (rank among the other half) and the
+// acceptance variate of the moving walker of each rank.
+struct StepDraws {
+    double uz[2][kWave];
+    double ua[2][kWave];
+    uint32_t partner[2][kWave];
+    unsigned long long labels;
+};
+
+// Stream wave: every draw of one step in the sequential order (the move
+// choice, the label shuffle, then per half-move ns stretch doubles, ns
+// partner draws and ns acceptance doubles).
+__device__ void draw_step(WaveMt& mt, int nw, int n0, StepDraws& d, uint32_t* jv, unsigned long long* occ) {
+#pragma clang fp contract(off)
+    const int l = lane_id();
+    mt.skip(2);  // the move choice (one random_sample, unused)
+    const unsigned long long labels = mt.shuffle_labels(nw, jv, occ);
+    if (l == 0) d.labels = labels;
+    for (int split = 0; split < 2; ++split) {
+        const int ns = split == 0 ? n0 : nw - n0;  // walkers with label `split`
+        const int nc = nw - ns;
+        const bool need = l < ns;
+        const double uz = mt.double_at(l, need, ns);
+        mt.masked_batch((uint32_t)(nc - 1), ns, d.partner[split]);
+        const double ua = mt.double_at(l, need, ns);
+        if (need) {
+            d.uz[split][l] = uz;
+            d.ua[split][l] = ua;
+        }
+    }
+}
+
+// Walker wave: the two half-moves of one step from its draws.  Returns false
+// (status set) on an infinite or NaN proposal or a NaN log-probability.
+__device__ bool apply_step(const StepDraws& d, const gsx_stretch_model& md, int nw, int n0,
+                           unsigned long long all, double a, double& x, double& lp, double* x_other,
+                           int* status) {
+#pragma clang fp contract(off)
+    const int l = lane_id();
+    const bool walker = l < nw;
+    const unsigned long long below = (l == 0) ? 0ull : (~0ull >> (64 - l));
+    const unsigned long long labels = d.labels;
+    for (int split = 0; split < 2; ++split) {
+        const unsigned long long own = split == 0 ? (~labels & all) : labels;
+        const unsigned long long oth = all & ~own;
+        const int ns = split == 0 ? n0 : nw - n0;
+        const int nc = nw - ns;
+        const bool mine = (own >> l) & 1ull;
+        const int rank = __popcll((mine ? own : oth) & below);
+        if (walker && !mine) x_other[rank] = x;
+        wave_sync();
+        const int p = (mine && nc > 1) ? (int)d.partner[split][rank] : 0;
+        const double cp = x_other[mine ? p : 0];
+        const double uz = mine ? d.uz[split][rank] : 0.0;
+        const double t = (a - 1.0) * uz + 1.0;
+        const double zz = (t * t) / a;
+        const double q = cp - (cp - x) * zz;
+        bool bad_q = false, bad_lp = false;
+        if (walker && mine) {
+            bad_q = !isfinite(q);
+            const double nl = ln_rad_pdf(md, q);
+            bad_lp = isnan(nl);
+            const double lnpdiff = (0.0 + nl) - lp;
+            if (!bad_q && !bad_lp && lnpdiff > log(d.ua[split][rank])) {
+                x = q;
+                lp = nl;
+            }
+        }
+        const unsigned long long bq = __ballot(bad_q), bl = __ballot(bad_lp);
+        if (bq | bl) {
+            if (l == 0) atomicCAS(status, kStretchOk, bq ? kStretchBadProposal : kStretchNanLogProb);
+            return false;
+        }
+        wave_sync();  // the next half-move rewrites x_other
+    }
+    return true;
+}
+
+// One block of two waves per chain.  The stream wave draws step s while the
+// walker wave applies step s - 1, double-buffered in LDS with one barrier
+// per step; both waves run the same loop, so every barrier is reached by
+// both, and a stop flag raised by the walker wave ends both after the
+// barrier.
+__global__ __launch_bounds__(kChainThreads) void k_stretch_chains(int64_t R, int nw,
+                                                                  const uint32_t* __restrict__ states,
+                                                                  int64_t burn_steps,
+                                                                  const int64_t* __restrict__ prod_steps,
+                                                                  int64_t max_steps, double a,
+                                                                  const gsx_stretch_model* __restrict__ models,
+                                                                  const double* __restrict__ init,
+                                                                  double* __restrict__ chain,
+                                                                  double* __restrict__ final_x,
+                                                                  int* __restrict__ status) {
 #pragma clang fp contract(off)
     __shared__ uint32_t key[kMtN];
-    __shared__ uint32_t slot[kWave];
     __shared__ double x_other[kWave];
+    __shared__ StepDraws draws[2];
+    __shared__ uint32_t shuffle_j[kWave];
+    __shared__ unsigned long long shuffle_occ[kWave];
+    __shared__ int stop;
     const int64_t r = blockIdx.x;
     if (r >= R) return;
     const gsx_stretch_model md = models[r];
+    // the wave's role, uniform across its lanes (scalar branches and loops)
+    const bool stream_wave = __builtin_amdgcn_readfirstlane((int)threadIdx.x) >= kWave;
     const int l = lane_id();
-    const bool walker = l < nw;
+    const bool walker = !stream_wave && l < nw;
     double x = walker ? init[r * nw + l] : 0.0;
     double lp = walker ? ln_rad_pdf(md, x) : 0.0;
-    if (__ballot(walker && isnan(lp))) {
-        if (l == 0) atomicCAS(status, kStretchOk, kStretchNanLogProb);
-        return;
+    if (threadIdx.x == 0) stop = 0;
+    if (!stream_wave && __ballot(walker && isnan(lp))) {
+        if (l == 0) {
+            atomicCAS(status, kStretchOk, kStretchNanLogProb);
+            stop = 1;
+        }
     }
-    const unsigned long long below = (l == 0) ? 0ull : (~0ull >> (64 - l));
+    __syncthreads();
+    if (stop) return;
     const unsigned long long all = nw == 64 ? ~0ull : ((1ull << nw) - 1ull);
     const int n0 = (nw + 1) / 2;
     for (int phase = 0; phase < 2; ++phase) {
-        const uint32_t* st = states + (r * 2 + phase) * kStateWords;
-        for (int i = l; i < kMtN; i += kWave) key[i] = st[i];
-        __syncthreads();
-        WaveMt mt{key, (int)st[kMtN], 0u, 0, 0};
         const int64_t steps = phase == 0 ? burn_steps : prod_steps[r];
-        for (int64_t s = 0; s < steps; ++s) {
-            mt.skip(2);  // the move choice (one random_sample, unused)
-            unsigned long long labels = 0xaaaaaaaaaaaaaaaaull & all;  // walker w has label w % 2
-            mt.shuffle_labels(labels, nw);
-            for (int split = 0; split < 2; ++split) {
-                const unsigned long long own = split == 0 ? (~labels & all) : labels;
-                const unsigned long long oth = all & ~own;
-                const int ns = split == 0 ? n0 : nw - n0;
-                const int nc = nw - ns;
-                const bool mine = (own >> l) & 1ull;
-                const int rank = __popcll((mine ? own : oth) & below);
-                if (walker && !mine) x_other[rank] = x;
-                const double uz = mt.double_at(rank, mine, ns);
-                mt.masked_batch((uint32_t)(nc - 1), ns, slot);
-                const double ua = mt.double_at(rank, mine, ns);
-                __syncthreads();
-                const int p = (mine && nc > 1) ? (int)slot[rank] : 0;
-                const double cp = x_other[mine ? p : 0];
-                const double t = (a - 1.0) * uz + 1.0;
-                const double zz = (t * t) / a;
-                const double q = cp - (cp - x) * zz;
-                bool bad_q = false, bad_lp = false;
-                if (walker && mine) {
-                    bad_q = !isfinite(q);
-                    const double nl = ln_rad_pdf(md, q);
-                    bad_lp = isnan(nl);
-                    const double lnpdiff = (0.0 + nl) - lp;
-                    if (!bad_q && !bad_lp && lnpdiff > log(ua)) {
-                        x = q;
-                        lp = nl;
-                    }
-                }
-                const unsigned long long bq = __ballot(bad_q), bl = __ballot(bad_lp);
-                if (bq | bl) {
-                    if (l == 0) atomicCAS(status, kStretchOk, bq ? kStretchBadProposal : kStretchNanLogProb);
-                    return;
-                }
-                __syncthreads();
-            }
-            if (phase == 1 && chain && walker) chain[(r * max_steps + s) * nw + l] = x;
+        WaveMt mt{key, 0, 0u, 0, 0};
+        if (stream_wave) {
+            const uint32_t* st = states + (r * 2 + phase) * kStateWords;
+            for (int i = l; i < kMtN; i += kWave) key[i] = st[i];
+            mt.pos = (int)st[kMtN];
+            wave_sync();
         }
-        __syncthreads();
+        for (int64_t s = 0; s <= steps; ++s) {
+            if (stream_wave) {
+                if (s < steps) draw_step(mt, nw, n0, draws[s & 1], shuffle_j, shuffle_occ);
+            } else if (s > 0) {
+                if (!apply_step(draws[(s - 1) & 1], md, nw, n0, all, a, x, lp, x_other, status)) {
+                    if (l == 0) stop = 1;
+                } else if (phase == 1 && chain && walker) {
+                    chain[(r * max_steps + s - 1) * nw + l] = x;
+                }
+            }
+            __syncthreads();
+            if (stop) return;
+        }
     }
     if (final_x && walker) final_x[r * nw + l] = x;
 }
@@ -364,7 +483,7 @@ int gsx_stretch_run_device(int64_t R, int32_t nwalkers, const uint32_t* states, 
     if (R == 0) return GSX_OK;
     if (!models || !states || !prod_steps || !init || !status) return GSX_E_INVALID;
     hipStream_t s = gsx::as_stream(stream);
-    hipLaunchKernelGGL(gsx::k_stretch_chains, dim3((unsigned)R), dim3(gsx::kWave), 0, s, R, (int)nwalkers, states,
+    hipLaunchKernelGGL(gsx::k_stretch_chains, dim3((unsigned)R), dim3(gsx::kChainThreads), 0, s, R, (int)nwalkers, states,
                        burn_steps, prod_steps, max_steps, a, models, init, chain, final_coords, status);
     return gsx::launch_status();
 }
