@@ -68,6 +68,8 @@ __device__ __forceinline__ uint32_t smear(uint32_t m) {
 // A chain runs on two wavefronts: wave 0 holds the walkers, wave 1 the
 // MT19937 stream.
 constexpr int kChainThreads = 2 * kWave;
+// rejection hypotheses tested per ballot round of the label shuffle
+constexpr int kHyp = 4;
 
 __device__ __forceinline__ int lane_id() { return (int)(threadIdx.x & (kWave - 1)); }
 
@@ -179,7 +181,8 @@ struct WaveMt {
     // rejection.  With r rejections before it, window word l is tried by
     // the call i0 - (l - r), so the first lane from m on that fails under r
     // is the next rejection, or the end of the shuffle where that call index
-    // drops below 1; each accepted word then writes J of its call.
+    // drops below 1; the ballots for r .. r + kHyp - 1 are taken together.
+    // Each accepted word then writes J of its call.
     //
     // The swaps, in parallel: position i is final after step i and holds
     // what position J[i] held then, which is position J[i]'s starting
@@ -198,20 +201,32 @@ struct WaveMt {
             const int i0 = i;
             unsigned long long rej = 0;
             int r = 0, m = 0, used = wlen;
-            while (m < wlen) {
-                const int c = i0 - l + r;  // the call that tries word l after r rejections
-                const uint32_t mask = c >= 1 ? (0xffffffffu >> __clz(c)) : 0u;
-                const bool ok = l < wlen && c >= 1 && (wv & mask) <= (uint32_t)c;
-                const unsigned long long ev = ~__ballot(ok) & (~0ull << m);
-                const int e = ev ? __ffsll((long long)ev) - 1 : kWave;
-                if (e >= wlen) break;  // the rest of the window accepted
-                if (i0 - e + r < 1) {  // every call done before word e
-                    used = e;
-                    break;
+            bool open = true;
+            while (open) {
+                // acceptance of every word after r + h rejections, h < kHyp
+                unsigned long long acc[kHyp];
+#pragma unroll
+                for (int h = 0; h < kHyp; ++h) {
+                    const int c = i0 - l + r + h;  // the call that tries word l
+                    const uint32_t mask = c >= 1 ? (0xffffffffu >> __clz(c)) : 0u;
+                    acc[h] = __ballot(l < wlen && c >= 1 && (wv & mask) <= (uint32_t)c);
                 }
-                rej |= 1ull << e;
-                ++r;
-                m = e + 1;
+#pragma unroll
+                for (int h = 0; h < kHyp && open; ++h) {
+                    const unsigned long long ev = ~acc[h] & (~0ull << m);
+                    const int e = ev ? __ffsll((long long)ev) - 1 : kWave;
+                    if (e >= wlen) {  // the rest of the window accepted
+                        open = false;
+                    } else if (i0 - e + r < 1) {  // every call done before word e
+                        used = e;
+                        open = false;
+                    } else {
+                        rej |= 1ull << e;
+                        ++r;
+                        m = e + 1;
+                        open = m < wlen;
+                    }
+                }
             }
             if (l < used && !((rej >> l) & 1ull)) {
                 const int call = i0 - l + __popcll(rej & below);
