@@ -302,15 +302,16 @@ def _sphere_chunk(dim, ang1, ang2, out):
     if dim == 1:
         out[0] = ang1
     elif dim == 2:
-        out[0] = np.cos(ang1)
-        out[1] = np.sin(ang1)
+        np.cos(ang1, out=out[0])
+        np.sin(ang1, out=out[1])
     else:
-        out[0] = np.sqrt(1.0 - ang2**2) * np.cos(ang1)
-        out[1] = np.sqrt(1.0 - ang2**2) * np.sin(ang1)
+        rho = np.sqrt(1.0 - ang2**2)
+        np.multiply(rho, np.cos(ang1, out=out[0]), out=out[0])
+        np.multiply(rho, np.sin(ang1, out=out[1]), out=out[1])
         out[2] = ang2
 
 
-_CHUNK = 1 << 16
+_CHUNK = 1 << 15
 _POOL = None
 
 
@@ -328,13 +329,6 @@ def _chunked(fn, n, threads):
 
         _POOL = ThreadPoolExecutor(max_workers=threads)
     list(_POOL.map(lambda b: fn(*b), bounds))
-
-
-def _sphere(dim, ang1, ang2, threads=1):
-    out = np.empty((dim, ang1.size))
-    _chunked(lambda lo, hi: _sphere_chunk(dim, ang1[lo:hi], None if ang2 is None else ang2[lo:hi], out[:, lo:hi]),
-             ang1.size, threads)
-    return out
 
 
 def _native_seed(seed):
@@ -490,29 +484,47 @@ def _prepare(model, mode_nos, seeds, sampling, device, nwalkers, burn_in, oversa
 
 def _finish(model, p):
     """Modes of a prepared group; MCMC groups on the device still lack their radii
-    (returned as (directions, z1, z2) for the caller to scale)."""
+    (returned as (directions, z1, z2) for the caller to scale).
+
+    The directions, the radii and their product are computed in one pass over
+    chunks of all the group's modes on the thread pool (the same numpy
+    operations per element as ``rad * sample_sphere(...)``); each realization
+    then gets its (dim, k) block of one buffer."""
     kind, dim, cuts = p["kind"], p["dim"], p["cuts"]
     if not p["n"].size:
         return []
     z1s, z2s = np.split(p["z1"], cuts), np.split(p["z2"], cuts)
     if kind == 0:
         return [(np.full((dim, k), 0.0), a, b) for k, a, b in zip(p["n"], z1s, z2s)]
-    threads = p["threads"]
-    sphs = np.split(_sphere(dim, p["ang1"], p["ang2"], threads), cuts, axis=1)
+    total = int(p["n"].sum())
+    ang1, ang2 = p["ang1"], p["ang2"]
     if kind == 1:
-        pdf, cdf, ppf = model.dist_func
-        rad = np.empty(p["u_inv"].size)
-
-        def inv(lo, hi):
-            rad[lo:hi] = np.asarray(ppf(p["u_inv"][lo:hi]), dtype=np.double) * 1.0 + 0.0
-
-        _chunked(inv, rad.size, threads)
-        rads = np.split(rad, cuts)
+        ppf = model.dist_func[2]
+        radius = lambda lo, hi: np.asarray(ppf(p["u_inv"][lo:hi]), dtype=np.double) * 1.0 + 0.0
     elif p["md"] is None:
-        rads = p["rads"]
+        rad = np.concatenate(p["rads"]) if p["rads"] else np.empty(0)
+        radius = lambda lo, hi: rad[lo:hi]
     else:
-        return list(zip(sphs, z1s, z2s))
-    return [(r * s, a, b) for r, s, a, b in zip(rads, sphs, z1s, z2s)]
+        radius = None
+    sph = np.empty((dim, total))
+
+    def chunk(lo, hi):
+        s = sph[:, lo:hi]
+        _sphere_chunk(dim, ang1[lo:hi], None if ang2 is None else ang2[lo:hi], s)
+        if radius is not None:
+            np.multiply(radius(lo, hi), s, out=s)
+
+    _chunked(chunk, total, p["threads"])
+    if radius is None:
+        return list(zip(np.split(sph, cuts, axis=1), z1s, z2s))
+    flat = np.empty(dim * total)
+    covs, o = [], 0
+    for k in p["n"].tolist():
+        c = flat[dim * o:dim * (o + k)].reshape(dim, k)
+        c[...] = sph[:, o:o + k]
+        covs.append(c)
+        o += k
+    return list(zip(covs, z1s, z2s))
 
 
 def _modes_python(model, mode_nos, seeds, kind, md, nwalkers, burn_in, oversampling_factor, threads):
