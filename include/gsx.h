@@ -450,7 +450,7 @@ typedef struct gsx_stretch_model {
 
 /* Device form of the stretch-move sampler of RNG.sample_ln_pdf
  * (src/gstools/random/rng.py:38-104) for R one-dimensional chains of
- * nwalkers <= 64 walkers: one wavefront per chain runs burn_steps steps on
+ * nwalkers <= 64 walkers: one block per chain runs burn_steps steps on
  * the chain's first MT19937 stream, then prod_steps[r] steps on its second,
  * recording the walkers after every production step in chain (R, max_steps,
  * nwalkers) (may be NULL).  models: DEVICE array of R gsx_stretch_model, the
