@@ -813,8 +813,9 @@ def _end_to_end(torch, specs, reps=3):
     """Sampling and sums of the 256 realizations, end to end: sampling then the
     sums, against :func:`gstools_amd.batch.sample_and_sum` (each group's sums
     launched as soon as its modes are final, beside the device chains).
-    Positions and axes are built on the device beforehand; both runs build
-    their items and batches inside the timing and launch eagerly."""
+    Positions, axes and the output buffers (one set per run kind) are on the
+    device beforehand, as in a loop that reuses them; both runs build their
+    items and batches inside the timing and launch eagerly."""
     from gstools_amd import Exponential
     from gstools_amd.batch import BatchItem, MixedBatch, sample_and_sum
     from gstools_amd.sampling import sample_modes_many
@@ -830,24 +831,41 @@ def _end_to_end(torch, specs, reps=3):
             pos = np.random.RandomState(seed).uniform(0.0, 1000.0, size=(3, size))
             geo[i] = torch.from_numpy(np.ascontiguousarray(expo.isometrize(pos))).to(dev)
         else:
-            geo[i] = [np.arange(size, dtype=np.float64)] * (2 if kind == "grid2d" else 3)
+            host = np.arange(size, dtype=np.float64)
+            nd = 2 if kind == "grid2d" else 3
+            geo[i] = ([torch.from_numpy(host).to(dev)] * nd, [host] * nd)  # device axes and their host copies
+
+    outs = {}
+    for run in ("seq", "ovl"):
+        for i, (kind, n, seed, size) in enumerate(specs):
+            npts = size if kind == "points3d" else size ** (2 if kind == "grid2d" else 3)
+            outs[run, i] = torch.empty((3 if kind == "incompr3d" else 1, npts), dtype=torch.float32, device=dev)
+    run_kind = ["seq"]
 
     def items_of(g, modes):
         # one upload per group: each realization's cov, z1, z2 as contiguous views of three flat buffers
         k = kinds[g]
-        flat = [torch.from_numpy(np.concatenate([np.ravel(m[c]) for m in modes])).to(dev) for c in range(3)]
+        # staged through pinned memory: a pageable copy would wait for the kernels of the other groups
+        flat = []
+        for c in range(3):
+            host = torch.empty(sum(m[c].size for m in modes), dtype=torch.float64, pin_memory=True)
+            np.concatenate([np.ravel(m[c]) for m in modes], out=host.numpy())
+            flat.append(host.to(dev, non_blocking=True))
         out, o, oz = [], 0, 0
         for i, (cov, z1, z2) in zip(idx[k], modes):
             d, n = cov.shape
             args = (flat[0][o:o + d * n].view(d, n), flat[1][oz:oz + n], flat[2][oz:oz + n])
             o, oz = o + d * n, oz + n
+            res = outs[run_kind[0], i]
             if k == "points3d":
-                out.append(BatchItem(*args, pos=geo[i]))
+                out.append(BatchItem(*args, pos=geo[i], out=res))
             else:
-                out.append(BatchItem(*args, axes=geo[i], incompr=k == "incompr3d"))
+                out.append(BatchItem(*args, axes=geo[i][0], axes_host=geo[i][1], incompr=k == "incompr3d",
+                                     out=res))
         return out
 
     def sequential():
+        run_kind[0] = "seq"
         modes = sample_modes_many(groups)
         its = [it for g in range(len(kinds)) for it in items_of(g, modes[g])]
         mb = MixedBatch(its, "bf16x3", tune=True)
@@ -856,6 +874,7 @@ def _end_to_end(torch, specs, reps=3):
         return its
 
     def overlapped():
+        run_kind[0] = "ovl"
         _, its, keep = sample_and_sum(groups, items_of)
         torch.cuda.synchronize()
         return [it for g in its for it in g]
@@ -873,8 +892,9 @@ def _end_to_end(torch, specs, reps=3):
     same = len(a) == len(b) and all(torch.equal(x.out, y.out) for x, y in zip(a, b))
     return {"sequential_ms": 1e3 * float(np.median(seq_s[1:])), "overlapped_ms": 1e3 * float(np.median(ovl_s[1:])),
             "outputs_equal": bool(same),
-            "note": "sampling (native draws, MCMC chains on the device) and the MixedBatch sums, eager launches; "
-                    "overlapped: the 2D grids (inversion sampling, drawn on the host) sum beside the chains"}
+            "note": "sampling (native draws, MCMC chains on the device) and the MixedBatch sums, eager launches, "
+                    "outputs preallocated; overlapped: each group's sums start as soon as its modes are final "
+                    "(the device groups first once their chains have run)"}
 
 
 def _grid_pos(torch, axes):

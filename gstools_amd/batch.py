@@ -22,9 +22,14 @@ TILE_VARIANTS = [(512, 128), (512, 256), (1024, 128), (1024, 256), (2048, 128), 
 
 
 class BatchItem:
-    """One realization: device tensors for modes, points or axes, and output."""
+    """One realization: device tensors for modes, points or axes, and output.
 
-    def __init__(self, cov, z1, z2, pos=None, axes=None, incompr=False, out=None):
+    Axes given as host arrays are uploaded in one copy and kept on the host
+    too (the uniform-axis check of the matrix-core plan reads them there);
+    axes already on the device may come with ``axes_host``, host copies of
+    the same values, for that check without a device round trip."""
+
+    def __init__(self, cov, z1, z2, pos=None, axes=None, incompr=False, out=None, axes_host=None):
         dev = torch.device("cuda", torch.cuda.current_device())
         as_dev = lambda a: (a if isinstance(a, torch.Tensor) else torch.as_tensor(np.asarray(a, dtype=np.float64))
                             ).to(dev, torch.float64).contiguous()
@@ -43,14 +48,21 @@ class BatchItem:
             self.shape = (int(self.pos.shape[1]), 1, 1)
             self.P = int(self.pos.shape[1])
         else:
-            # host copies of axes given as host arrays (checked without a device round trip)
-            self.axes_host = None if any(isinstance(a, torch.Tensor) for a in axes) else [
-                np.asarray(a, dtype=np.float64).reshape(-1) for a in axes]
-            ax = [as_dev(a).reshape(-1) for a in axes]
-            if len(ax) != self.d:
+            if any(isinstance(a, torch.Tensor) for a in axes):
+                ax = [as_dev(a).reshape(-1) for a in axes]
+                self.axes = torch.cat(ax)
+                dims = [int(a.numel()) for a in ax]
+                self.axes_host = None
+                if axes_host is not None:
+                    self.axes_host = [np.asarray(a, dtype=np.float64).reshape(-1) for a in axes_host]
+                    if [a.size for a in self.axes_host] != dims:
+                        raise ValueError("axes_host must match the axes' lengths")
+            else:
+                self.axes_host = [np.asarray(a, dtype=np.float64).reshape(-1) for a in axes]
+                dims = [int(a.size) for a in self.axes_host]
+                self.axes = torch.from_numpy(np.concatenate(self.axes_host)).to(dev)
+            if len(dims) != self.d:
                 raise ValueError("one axis per dimension")
-            self.axes = torch.cat(ax)
-            dims = [int(a.numel()) for a in ax]
             self.shape = tuple(dims + [1] * (3 - self.d))
             self.P = int(np.prod(dims))
         self.incompr = bool(incompr)

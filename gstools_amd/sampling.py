@@ -212,6 +212,12 @@ class _DeviceChains:
                                                            self.max_steps, float(a), _tp(models_t), _tp(init_t),
                                                            _tp(self.chain), None, _tp(self.status), _stream()),
                       "stretch_run_device")
+        self._done = torch.cuda.Event()
+        self._done.record()
+
+    def finished(self):
+        """True once the chains have run (does not wait)."""
+        return self._done.query()
 
     def pick(self, picks):
         """Chain r's recorded walkers at the flat indices picks[r] -> list of arrays."""
@@ -357,9 +363,13 @@ def sample_modes_many(groups, sampling="auto", nthreads=None, device=None, nwalk
     """:func:`sample_modes_batch` for several ``(model, mode_nos, seeds)`` groups:
     the MCMC chains of every group on the device run in one launch while the
     host draws and finishes all groups.  ``on_ready(i, modes)``, if given, is
-    called for each group as soon as its modes are final: the groups drawn on
-    the host before the device chains are read back, so the caller can start
-    GPU work on them while the chains still run."""
+    called for each group once its modes are final: the device groups'
+    directions and then the host groups' modes are computed while the chains
+    run; the host groups are then handed over in order, and the device
+    groups are read back and handed over before the first host group still
+    waiting once their chains have run (else after the last), so the
+    caller's GPU work on any group starts as early as the chains and the host
+    allow."""
     if sampling not in ("auto", "inversion", "mcmc"):
         raise ValueError("sample_modes_batch: sampling not in ['auto', 'inversion', 'mcmc']")
     threads = len(os.sched_getaffinity(0)) if nthreads is None else int(nthreads)
@@ -376,30 +386,42 @@ def sample_modes_many(groups, sampling="auto", nthreads=None, device=None, nwalk
         if first:
             chains, dev_groups = _launch(preps, burn_in)
     out = [None] * len(preps)
-    for i, ((model, mode_nos, seeds), p) in enumerate(zip(groups, preps)):
-        if p is None:
-            continue
-        if not p["n"].size:
-            out[i] = []
-            if on_ready is not None:
-                on_ready(i, out[i])
-            continue
-        if p["kind"] == 2 and p["md"] is None:
-            p["rads"] = _chains_host(model.ln_spectral_rad_pdf, p["init"], p["burn"], p["prod"], burn_in,
-                                     p["prod_steps"], p["picks"], nthreads=threads)
-        out[i] = _finish(model, p)
-        if on_ready is not None and p.get("md") is None:
-            on_ready(i, out[i])
-    if chains is not None:
+    on_dev = [p is not None and p.get("md") is not None and p["n"].size > 0 for p in preps]
+    # the directions of the device groups first, while the chains run
+    for i, ((model, _, _), p) in enumerate(zip(groups, preps)):
+        if on_dev[i]:
+            out[i] = _finish(model, p)
+
+    def device_groups():
         rads = chains.pick([pk for p in dev_groups for pk in p["picks"]])
         k = 0
         for i, p in enumerate(preps):
-            if p is not None and p.get("md") is not None and p["n"].size:
+            if on_dev[i]:
                 nr = p["init"].shape[0]
                 out[i] = [(r * s, a, b) for r, (s, a, b) in zip(rads[k:k + nr], out[i])]
                 k += nr
                 if on_ready is not None:
                     on_ready(i, out[i])
+
+    host = [i for i, p in enumerate(preps) if p is not None and not on_dev[i]]
+    for i in host:
+        model, p = groups[i][0], preps[i]
+        if not p["n"].size:
+            out[i] = []
+            continue
+        if p["kind"] == 2 and p["md"] is None:
+            p["rads"] = _chains_host(model.ln_spectral_rad_pdf, p["init"], p["burn"], p["prod"], burn_in,
+                                     p["prod_steps"], p["picks"], nthreads=threads)
+        out[i] = _finish(model, p)
+    pending = chains is not None
+    for i in host:
+        if pending and chains.finished():  # chains done: their groups before the next host group
+            device_groups()
+            pending = False
+        if on_ready is not None:
+            on_ready(i, out[i])
+    if pending:
+        device_groups()
     for i, ((model, mode_nos, seeds), p) in enumerate(zip(groups, preps)):
         if p is None:
             out[i] = _modes_python(model, [int(n) for n in mode_nos], list(seeds), _kind(model, sampling),

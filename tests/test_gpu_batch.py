@@ -163,6 +163,41 @@ def test_mixed_batch_branches_equal_one_branch():
         MixedBatch(items, "bf16x3", tune=False, branches=0)
 
 
+def test_batch_item_axes_forms_equal():
+    """Grid items with host axes, device axes, and device axes with their host
+    copies (``axes_host``) give the same outputs bit for bit (uniform and
+    non-uniform last axes, 2D and 3D)."""
+    _say("batch item axes forms")
+    rs = np.random.RandomState(11)
+    dev = torch.device("cuda", 0)
+    specs = []
+    for shape, uniform in [((300, 512), True), ((64, 300), False), ((40, 36, 300), True)]:
+        n = int(rs.choice([500, 1000]))
+        cov = rs.normal(size=(len(shape), n)) * 0.3
+        axes = [np.arange(m, dtype=np.float64) * 0.7 for m in shape]
+        if not uniform:
+            axes[-1] = np.sort(rs.uniform(0, 200, shape[-1]))
+        specs.append((cov, rs.normal(size=n), rs.normal(size=n), axes))
+    outs = []
+    for form in ("host", "device", "device+host"):
+        items = []
+        for cov, z1, z2, axes in specs:
+            if form == "host":
+                items.append(BatchItem(cov, z1, z2, axes=axes))
+            else:
+                dax = [torch.from_numpy(a).to(dev) for a in axes]
+                items.append(BatchItem(cov, z1, z2, axes=dax, axes_host=axes if form == "device+host" else None))
+        MixedBatch(items, "bf16x3", tune=False).launch(graph=False)
+        torch.cuda.synchronize()
+        outs.append([it.out.clone() for it in items])
+    for other in outs[1:]:
+        for a, b in zip(outs[0], other):
+            assert torch.equal(a, b)
+    cov, z1, z2, axes = specs[0]
+    with pytest.raises(ValueError):
+        BatchItem(cov, z1, z2, axes=[torch.from_numpy(a).to(dev) for a in axes], axes_host=[axes[0], axes[1][:-1]])
+
+
 def test_batch_empty_and_invalid():
     plan = BatchPlan([], 1024, 256)
     assert plan.tiles == 0

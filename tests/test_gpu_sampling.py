@@ -191,3 +191,25 @@ def test_sample_and_sum_overlapped_equals_sequential():
     assert len(got) == len(seq)
     for a, b in zip(got, seq):
         assert torch.equal(a.out, b.out)
+
+
+@pytest.mark.parametrize("chains_first", [True, False])
+def test_many_groups_ready_order(monkeypatch, chains_first):
+    """Both orders of ``sample_modes_many``'s finish (device groups read back
+    before the host groups when their chains have run, or after the last) give
+    the same modes, and ``on_ready`` sees every group once, in that order."""
+    from gstools_amd import sampling as smp
+
+    _say(f"ready order chains_first={chains_first}")
+    groups = [(Gaussian(dim=2, var=1.0, len_scale=10.0), [300, 50], [1, 2]),          # host (inversion)
+              (Exponential(dim=3, var=2.0, len_scale=5.0), [64, 200], [5, 6]),        # device chains
+              (Gaussian(dim=3, var=1.0, len_scale=10.0), [], []),                     # empty
+              (Matern(dim=3, var=1.0, len_scale=8.0, nu=1.5), [90], [7])]             # device chains
+    ref = smp.sample_modes_many(groups, device=False)
+    monkeypatch.setattr(smp._DeviceChains, "finished", lambda self: chains_first)
+    seen = []
+    got = smp.sample_modes_many(groups, on_ready=lambda i, m: seen.append(i))
+    assert sorted(seen) == [0, 1, 2, 3]
+    assert seen == ([1, 3, 0, 2] if chains_first else [0, 2, 1, 3])
+    for a, b in zip(ref, got):
+        _same(a, b)
