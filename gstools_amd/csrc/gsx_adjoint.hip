@@ -35,6 +35,7 @@ namespace gsx {
 
 constexpr int kAdjPts = 256;         // points per LDS stage
 constexpr int64_t kAdjTargetBlocks = 2048;
+constexpr int kAdjModesF32 = 2;        // modes per thread of the FP32 stage 1
 
 struct AdjPlan {
     int64_t mode_tiles, chunks, chunk_pts;
@@ -70,7 +71,11 @@ constexpr int adj_sums() {
     return F64 ? 2 + 2 * D : 2 + D;
 }
 
-template <int D, typename GT>
+// MPT modes per thread (threads t, t + kThreads, ... of the block's
+// kThreads * MPT modes): every staged point's LDS reads and the loop's
+// bookkeeping serve MPT modes; each mode sums its points in the same order
+// whatever MPT is.  The FP64 path takes MPT = 1.
+template <int D, typename GT, int MPT>
 __global__ __launch_bounds__(kThreads) void k_adjoint_stage1(int64_t N, int64_t P, int64_t chunk_pts,
                                                              const double* __restrict__ cov,
                                                              const double* __restrict__ z1,
@@ -79,26 +84,35 @@ __global__ __launch_bounds__(kThreads) void k_adjoint_stage1(int64_t N, int64_t 
                                                              const GT* __restrict__ g,
                                                              double* __restrict__ part) {
     constexpr bool F64 = sizeof(GT) == 8;
+    static_assert(!F64 || MPT == 1, "the FP64 stage takes one mode per thread");
     constexpr int Q = adj_sums<D, F64>();
     __shared__ double s_x[kAdjPts * D];
     __shared__ GT s_g[kAdjPts * (D + 1)];  // g, g x_0 .. g x_{D-1}
     __shared__ unsigned long long s_xmax;
 
-    const int64_t j = (int64_t)blockIdx.x * kThreads + threadIdx.x;
-    const bool active = j < N;
-    double k[D];
+    int64_t j[MPT];
+    bool active[MPT];
+    double k[MPT][D];
+    double mphi[MPT];
     double k1 = 0.0;
 #pragma unroll
-    for (int d = 0; d < D; ++d) {
-        k[d] = active ? cov[d * N + j] : 0.0;
-        if (!F64) k[d] *= kInv2Pi;
-        k1 += fabs(k[d]);
-    }
-    // FP32: -phi / 2pi, the phase shift in revolutions
-    const double mphi = (!F64 && active) ? -atan2(z2[j], z1[j]) * kInv2Pi : 0.0;
-    double acc[Q];
+    for (int u = 0; u < MPT; ++u) {
+        j[u] = (int64_t)blockIdx.x * kThreads * MPT + u * kThreads + threadIdx.x;
+        active[u] = j[u] < N;
 #pragma unroll
-    for (int q = 0; q < Q; ++q) acc[q] = 0.0;
+        for (int d = 0; d < D; ++d) {
+            k[u][d] = active[u] ? cov[d * N + j[u]] : 0.0;
+            if (!F64) k[u][d] *= kInv2Pi;
+            k1 += fabs(k[u][d]);
+        }
+        // FP32: -phi / 2pi, the phase shift in revolutions
+        mphi[u] = (!F64 && active[u]) ? -atan2(z2[j[u]], z1[j[u]]) * kInv2Pi : 0.0;
+    }
+    double acc[MPT][Q];
+#pragma unroll
+    for (int u = 0; u < MPT; ++u)
+#pragma unroll
+        for (int q = 0; q < Q; ++q) acc[u][q] = 0.0;
 
     const int64_t p0 = (int64_t)blockIdx.y * chunk_pts;
     const int64_t p1 = (p0 + chunk_pts < P) ? p0 + chunk_pts : P;
@@ -130,57 +144,74 @@ __global__ __launch_bounds__(kThreads) void k_adjoint_stage1(int64_t N, int64_t 
 #pragma unroll
                     for (int d = 0; d < D; ++d) x[d] = s_x[m * D + d];
                     double sn, cs;
-                    sincos_fast(phase_exact<D>(k, x), &sn, &cs);
+                    sincos_fast(phase_exact<D>(k[0], x), &sn, &cs);
                     const GT* gg = s_g + m * (D + 1);
-                    acc[0] = fma_(gg[0], cs, acc[0]);
-                    acc[1] = fma_(gg[0], sn, acc[1]);
+                    acc[0][0] = fma_(gg[0], cs, acc[0][0]);
+                    acc[0][1] = fma_(gg[0], sn, acc[0][1]);
 #pragma unroll
                     for (int d = 0; d < D; ++d) {
-                        acc[2 + d] = fma_(gg[1 + d], cs, acc[2 + d]);
-                        acc[2 + D + d] = fma_(gg[1 + d], sn, acc[2 + D + d]);
+                        acc[0][2 + d] = fma_(gg[1 + d], cs, acc[0][2 + d]);
+                        acc[0][2 + D + d] = fma_(gg[1 + d], sn, acc[0][2 + D + d]);
                     }
                 }
             } else {
                 for (int m = 0; m < cnt; ++m) {
                     double x[D];
                     for (int d = 0; d < D; ++d) x[d] = s_x[m * D + d];
-                    const double th = phase_exact<D>(k, x);
+                    const double th = phase_exact<D>(k[0], x);
                     double sn, cs;
                     sincos_any(th, &sn, &cs);
                     const GT* gg = s_g + m * (D + 1);
-                    acc[0] = fma_(gg[0], cs, acc[0]);
-                    acc[1] = fma_(gg[0], sn, acc[1]);
+                    acc[0][0] = fma_(gg[0], cs, acc[0][0]);
+                    acc[0][1] = fma_(gg[0], sn, acc[0][1]);
                     for (int d = 0; d < D; ++d) {
-                        acc[2 + d] = fma_(gg[1 + d], cs, acc[2 + d]);
-                        acc[2 + D + d] = fma_(gg[1 + d], sn, acc[2 + D + d]);
+                        acc[0][2 + d] = fma_(gg[1 + d], cs, acc[0][2 + d]);
+                        acc[0][2 + D + d] = fma_(gg[1 + d], sn, acc[0][2 + D + d]);
                     }
                 }
             }
         } else {
-            float pa[Q];
+            float pa[MPT][Q];
 #pragma unroll
-            for (int q = 0; q < Q; ++q) pa[q] = 0.0f;
+            for (int u = 0; u < MPT; ++u)
+#pragma unroll
+                for (int q = 0; q < Q; ++q) pa[u][q] = 0.0f;
             for (int m = 0; m < cnt; ++m) {
-                double rev = mphi;
+                double x[D];
 #pragma unroll
-                for (int d = 0; d < D; ++d) rev = fma_(k[d], s_x[m * D + d], rev);
-                const float f = (float)__builtin_amdgcn_fract(rev);
-                const float cs = __builtin_amdgcn_cosf(f);
-                const float sn = __builtin_amdgcn_sinf(f);
+                for (int d = 0; d < D; ++d) x[d] = s_x[m * D + d];
                 const GT* gg = s_g + m * (D + 1);
-                pa[0] = __builtin_fmaf(gg[0], cs, pa[0]);
-                pa[1] = __builtin_fmaf(gg[0], sn, pa[1]);
+                const float g0 = gg[0];
+                float gx[D];
 #pragma unroll
-                for (int d = 0; d < D; ++d) pa[2 + d] = __builtin_fmaf(gg[1 + d], sn, pa[2 + d]);
+                for (int d = 0; d < D; ++d) gx[d] = gg[1 + d];
+#pragma unroll
+                for (int u = 0; u < MPT; ++u) {
+                    double rev = mphi[u];
+#pragma unroll
+                    for (int d = 0; d < D; ++d) rev = fma_(k[u][d], x[d], rev);
+                    const float f = (float)__builtin_amdgcn_fract(rev);
+                    const float cs = __builtin_amdgcn_cosf(f);
+                    const float sn = __builtin_amdgcn_sinf(f);
+                    pa[u][0] = __builtin_fmaf(g0, cs, pa[u][0]);
+                    pa[u][1] = __builtin_fmaf(g0, sn, pa[u][1]);
+#pragma unroll
+                    for (int d = 0; d < D; ++d) pa[u][2 + d] = __builtin_fmaf(gx[d], sn, pa[u][2 + d]);
+                }
             }
 #pragma unroll
-            for (int q = 0; q < Q; ++q) acc[q] += (double)pa[q];
+            for (int u = 0; u < MPT; ++u)
+#pragma unroll
+                for (int q = 0; q < Q; ++q) acc[u][q] += (double)pa[u][q];
         }
     }
-    if (active) {
-        double* dst = part + (int64_t)blockIdx.y * Q * N;
 #pragma unroll
-        for (int q = 0; q < Q; ++q) dst[q * N + j] = acc[q];
+    for (int u = 0; u < MPT; ++u) {
+        if (active[u]) {
+            double* dst = part + (int64_t)blockIdx.y * Q * N;
+#pragma unroll
+            for (int q = 0; q < Q; ++q) dst[q * N + j[u]] = acc[u][q];
+        }
     }
 }
 
@@ -225,7 +256,9 @@ int adjoint_launch(const AdjPlan& pl, int64_t N, int64_t P, const double* cov, c
                    const double* z2, const double* pos, const GT* g, double* gz1, double* gz2,
                    double* gk, void* ws, hipStream_t s) {
     double* part = static_cast<double*>(ws);
-    hipLaunchKernelGGL((k_adjoint_stage1<D, GT>), dim3((unsigned)pl.mode_tiles, (unsigned)pl.chunks),
+    constexpr int mpt = sizeof(GT) == 8 ? 1 : kAdjModesF32;
+    hipLaunchKernelGGL((k_adjoint_stage1<D, GT, mpt>), dim3((unsigned)ceil_div(N, (int64_t)kThreads * mpt),
+                                                          (unsigned)pl.chunks),
                        dim3(kThreads), 0, s, N, P, pl.chunk_pts, cov, z1, z2, pos, g, part);
     int st = launch_status();
     if (st != GSX_OK) return st;
