@@ -265,8 +265,11 @@ def sample_and_sum(groups, items_of, precision="bf16x3", sampling="auto", nthrea
     returns the :class:`BatchItem` list of group i for its sampled modes.  Each
     group's sums (grids on the matrix cores, the rest in one persistent FP32
     launch) are launched on a stream of their own as soon as the group's modes
-    are final, so the groups drawn on the host run while the MCMC chains of the
-    others are still on the device.  Returns ``(modes, items, batches)``:
+    are final (the order in which ``sample_modes_many`` hands the groups over:
+    the device groups first once their chains have run, else the groups drawn
+    on the host while the chains still run).  An ``items_of`` that uploads
+    modes should stage them through pinned memory: a pageable copy waits for
+    the kernels of the groups already launched.  Returns ``(modes, items, batches)``:
     modes and items per group, and the :class:`MixedBatch` objects whose
     launches fill the items' outputs (keep them until the outputs are read).
     The outputs are complete once the caller's stream has synchronized (the
