@@ -902,6 +902,41 @@ def _grid_pos(torch, axes):
     return torch.stack([a.reshape(-1) for a in g]).contiguous()
 
 
+def bench_mfma_ceiling(torch):
+    """Matrix-core ceilings on this chip: nothing but independent back-to-back
+    MFMAs (``gsx_probe_mfma``, 16 accumulators per wave, two waves per SIMD),
+    with near-constant and with random operands.  Random operands toggle the
+    multipliers' inputs as real data does; the extra power lowers the clock,
+    so that rate, not the published peak, is what a GEMM-shaped kernel on
+    real data can approach."""
+    import ctypes
+
+    from gstools_amd import _native
+
+    log("bench: matrix-core ceilings")
+    dev = torch.device("cuda", 0)
+    blocks = 2 * torch.cuda.get_device_properties(0).multi_processor_count
+    out = torch.empty(blocks * 256, dtype=torch.float64, device=dev)
+    res = {"blocks": blocks, "note": "16 independent accumulators per wave, two waves per SIMD, no memory traffic"}
+    for name, kind, flops, iters, peak in (("bf16", _native.GSX_PROBE_BF16, 16384.0, 16000, 2500.0),
+                                           ("f64", _native.GSX_PROBE_F64, 2048.0, 4000, 78.6)):
+        for rnd in (0, 1):
+            def call():
+                stream = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+                _native.check(_native.lib().gsx_probe_mfma(kind, rnd, iters, blocks,
+                                                           ctypes.c_void_p(out.data_ptr()), stream), "probe_mfma")
+
+            _, ker = time_kernel(call, 5, 2, torch)
+            tf = blocks * 4 * iters * 16 * flops / ker / 1e12
+            tag = f"{name}_{'random' if rnd else 'constant'}"
+            res[tag + "_tflops"] = tf
+            res[tag + "_frac_of_peak"] = tf / peak
+            res[tag + "_ms"] = ker * 1e3
+    if not bool(torch.isfinite(out).all()):
+        raise RuntimeError("probe_mfma: non-finite checksums")
+    return res
+
+
 def bench_copy(torch):
     """A plain 1 GiB device copy: the known-size reference for HBM byte counters."""
     n = 1 << 27  # 2^27 doubles = 1 GiB
@@ -930,7 +965,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--extras", type=int, default=1, help="also run the other configurations")
     ap.add_argument("--only", choices=["fp32", "fp64", "config1", "mfma", "incompr", "fourier", "krige", "variogram",
-                                       "adjoint", "batch", "copy"],
+                                       "adjoint", "batch", "copy", "ceiling"],
                     help="run one section only (for profiling) and print its JSON")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -952,7 +987,8 @@ def main():
     dev = torch.device("cuda", 0)
     k_ext = max(3, args.steps // 4)
     w_ext = max(1, min(args.warmup, 3))  # the extra sections run long kernels
-    if args.only in ("config1", "mfma", "incompr", "fourier", "krige", "variogram", "adjoint", "batch", "copy"):
+    if args.only in ("config1", "mfma", "incompr", "fourier", "krige", "variogram", "adjoint", "batch", "copy",
+                     "ceiling"):
         fn = {"config1": lambda: bench_config1(torch, k_ext, w_ext),
               "mfma": lambda: bench_mfma_grid(torch, k_ext, w_ext),
               "incompr": lambda: bench_incompr_grid(torch, k_ext, w_ext),
@@ -961,7 +997,8 @@ def main():
               "fourier": lambda: bench_fourier(torch, k_ext, w_ext),
               "adjoint": lambda: bench_adjoint(torch, k_ext, w_ext),
               "batch": lambda: bench_batch(torch, k_ext, w_ext, *batch_inputs),
-              "copy": lambda: bench_copy(torch)}[args.only]
+              "copy": lambda: bench_copy(torch),
+              "ceiling": lambda: bench_mfma_ceiling(torch)}[args.only]
         print(json.dumps({args.only: fn()}), flush=True)
         return
 
@@ -1046,7 +1083,8 @@ def main():
                     ("krige", bench_krige),
                     ("variogram", bench_variogram),
                     ("adjoint", bench_adjoint),
-                    ("batch", lambda t, k, w: bench_batch(t, k, w, *batch_inputs)))
+                    ("batch", lambda t, k, w: bench_batch(t, k, w, *batch_inputs)),
+                    ("mfma_ceiling", lambda t, k, w: bench_mfma_ceiling(t)))
         for name, fn in sections:
             try:
                 extras[name] = fn(torch, k_ext, w_ext)

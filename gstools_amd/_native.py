@@ -19,6 +19,8 @@ GSX_E_NO_DEVICE = -3
 GSX_E_WORKSPACE = -4
 GSX_E_UNSUPPORTED = -5
 GSX_E_SCRATCH = -6
+GSX_PROBE_BF16 = 0
+GSX_PROBE_F64 = 1
 NOALLOC_SCRATCH_BYTES = 32 << 20  # GSX_NOALLOC_SCRATCH_BYTES
 MFMA_BF16X3 = 0
 MFMA_FP16X3 = 1
@@ -120,6 +122,7 @@ EXPORTS = {
     "gsx_batch_plan_launch_graph": (_i32, [_vp, _vp]),
     "gsx_batch_plan_tiles": (_i64, [_vp]),
     "gsx_batch_plan_destroy": (None, [_vp]),
+    "gsx_probe_mfma": (_i32, [_i32, _i32, _i64, _i64, _vp, _vp]),
 }
 
 _lock = threading.Lock()

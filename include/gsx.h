@@ -511,6 +511,20 @@ int gsx_batch_plan_launch_graph(gsx_batch_plan* plan, void* stream);
 int64_t gsx_batch_plan_tiles(const gsx_batch_plan* plan);
 void gsx_batch_plan_destroy(gsx_batch_plan* plan);
 
+/* Measurement aid, no counterpart in the reference: the matrix-core rate
+ * the chip attains with nothing but independent back-to-back MFMAs (16
+ * accumulators per wave, no memory traffic in the loop), for the roofline
+ * figures of the GEMM-shaped kernels.  kind GSX_PROBE_BF16 runs
+ * v_mfma_f32_16x16x32_bf16 (16384 flops each), GSX_PROBE_F64
+ * v_mfma_f64_16x16x4_f64 (2048 flops each); every wave of `blocks` blocks
+ * of 256 threads issues 16 * iters of them.  random_operands != 0 gives
+ * each lane random operands (toggling inputs, as real data: more power, a
+ * lower clock), 0 near-constant ones.  out: DEVICE array of blocks * 256
+ * floats (bf16) or doubles (f64), one checksum per thread. */
+#define GSX_PROBE_BF16 0
+#define GSX_PROBE_F64 1
+int gsx_probe_mfma(int kind, int random_operands, int64_t iters, int64_t blocks, void* out, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

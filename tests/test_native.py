@@ -55,6 +55,11 @@ def test_invalid_arguments_are_rejected_on_the_host():
     assert lib.gsx_summate_adjoint_workspace(2, 2048, 10**6) > 0
     shape = (ctypes.c_int64 * 3)(10, 10, 1)
     assert lib.gsx_summate_grid_f64(2, 5, shape, null, null, null, null, null, null) == -1
+    # matrix-core probe: unknown kind, no iterations or blocks, null output
+    assert lib.gsx_probe_mfma(2, 0, 10, 8, null, null) == -1
+    assert lib.gsx_probe_mfma(0, 0, 0, 8, null, null) == -1
+    assert lib.gsx_probe_mfma(1, 1, 10, 0, null, null) == -1
+    assert lib.gsx_probe_mfma(1, 0, 10, 8, null, null) == -1
     # allocation-free adjoint: bad d, and more modes than its static array holds
     args = [null] * 8 + [None]
     assert lib.gsx_summate_adjoint_f32_noalloc(4, 10, 10, *args) == -1
