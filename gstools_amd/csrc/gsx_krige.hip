@@ -55,15 +55,24 @@ constexpr int kKrApad = kKrK + 1;
 // (rows 144 doubles apart) fall in disjoint LDS banks
 constexpr int kKrBpad = kKrPts + 16;
 
-// M2 of the variance form (row tile r keeps its columns from 128 r on; the
-// entries left of the diagonal block are never read and not written)
-__global__ __launch_bounds__(256) void k_krige_sym(int64_t K, const double* __restrict__ M,
+// M2 of the variance form, padded to Rp = 128 * row tiles rows and Kp (K
+// rounded up to a multiple of 16) columns, row stride Kp: row tile r keeps its
+// columns from 128 r on, zero beyond K (the GEMM's A loads then need no
+// bounds checks); the entries left of the diagonal block are never read and
+// not written
+inline int64_t krige_kp(int64_t K) { return ceil_div(K, (int64_t)kKrK) * kKrK; }
+
+__global__ __launch_bounds__(256) void k_krige_sym(int64_t K, int64_t Rp, int64_t Kp, const double* __restrict__ M,
                                                    double* __restrict__ M2) {
     const int64_t idx = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    if (idx >= K * K) return;
-    const int64_t i = idx / K, l = idx - i * K;
+    if (idx >= Rp * Kp) return;
+    const int64_t i = idx / Kp, l = idx - i * Kp;
     const int64_t b0 = (i / kKrRows) * kKrRows;
     if (l < b0) return;
+    if (i >= K || l >= K) {
+        M2[idx] = 0.0;
+        return;
+    }
     const double s = M[i * K + l] + M[l * K + i];
     M2[idx] = l < b0 + kKrRows ? 0.5 * s : s;
 }
@@ -71,11 +80,16 @@ __global__ __launch_bounds__(256) void k_krige_sym(int64_t K, const double* __re
 // W = M2 V over the columns from the row tile on (K steps in descending
 // order); per-row-tile partial sums of V o W.  Row tile r runs about K - 128 r
 // K steps, so each block takes the pair (u, R - 1 - u): equal work per block
-// (about K + 128 steps), and the long tile first.
-__global__ __launch_bounds__(256, 2) void k_krige_gemm(int64_t K, int64_t P, int64_t row_tiles, int64_t col_tiles,
-                                                    const double* __restrict__ M,
+// (about K + 128 steps), and the long tile first.  The staging loads are
+// 16 bytes wide: A (the padded M2, row stride Kp) four per thread with no
+// bounds checks; B as point pairs (kPairB: P even and V 16-byte aligned),
+// else one point of eight rows per thread.
+template <bool kPairB>
+__global__ __launch_bounds__(256, 2) void k_krige_gemm(int64_t K, int64_t Kp, int64_t P, int64_t row_tiles,
+                                                    int64_t col_tiles, const double* __restrict__ M,
                                                     const double* __restrict__ V,
                                                     double* __restrict__ part_e) {
+    typedef __attribute__((ext_vector_type(2))) double f64x2;
     __shared__ double As[kKrRows * kKrApad];
     __shared__ double Bs[kKrK * kKrBpad];
     __shared__ double red[2][kKrPts];  // [row half][point]
@@ -105,33 +119,55 @@ __global__ __launch_bounds__(256, 2) void k_krige_gemm(int64_t K, int64_t P, int
         const int64_t rt_idx = ti == 0 ? u_idx : R - 1 - u_idx;
         const int64_t i0 = rt_idx * kKrRows;
 
-        // staging assignment: A kKrRows x kKrK (kAU consecutive per thread); B kKrK x 128
-        // (kKrK / 2 per thread: point t % 128 of rows t / 128 + 2u, so a wave's loads
-        // and LDS stores are unit-stride and the stores conflict-free)
-        constexpr int kAT = 256 / kKrRows;  // threads per A row
-        constexpr int kAU = kKrK / kAT, kBU = kKrK / 2;
-        const int ar = t / kAT, ac = (t % kAT) * kAU;
-        const int br = t >> 7, bc = t & 127;
-        double pa[kAU], pb[kBU];
+        // staging assignment: A kKrRows x kKrK, 8 consecutive doubles of one row per
+        // thread (four 16-byte loads from the padded M2); B kKrK x 128, either point
+        // pair (t % 64) * 2 of rows t / 64 + 4u (16-byte loads and LDS stores) or
+        // point t % 128 of rows t / 128 + 2u (unit-stride, conflict-free stores)
+        static_assert(kKrRows == 128 && kKrK == 16, "staging assignment");
+        const int ar = t >> 1, ac = (t & 1) * 8;
+        const f64x2* arow = reinterpret_cast<const f64x2*>(M + (i0 + ar) * Kp + ac);
+        const int br = kPairB ? t >> 6 : t >> 7;
+        const int bc = kPairB ? (t & 63) * 2 : t & 127;
+        f64x2 pa[4];
+        f64x2 pb2[4];    // point pairs: loads from clamped (valid) addresses, zeroed at the store (bit u of mb)
+        double pb1[8];   // single points
+        unsigned mb = 0;
         auto load = [&](int64_t j0) {
-            const int64_t row = i0 + ar;
 #pragma unroll
-            for (int u = 0; u < kAU; ++u) {
-                const int64_t col = j0 + ac + u;
-                pa[u] = (row < K && col < K) ? M[row * K + col] : 0.0;
-            }
+            for (int v = 0; v < 4; ++v) pa[v] = arow[(j0 >> 1) + v];
             const int64_t p = p0 + bc;
+            if constexpr (kPairB) {
+                const int64_t pc = p < P ? p : P - 2;  // P even: a pair is whole or out of range
+                mb = 0;
 #pragma unroll
-            for (int u = 0; u < kBU; ++u) {
-                const int64_t j = j0 + br + 2 * u;
-                pb[u] = (j < K && p < P) ? V[j * P + p] : 0.0;
+                for (int u = 0; u < 4; ++u) {
+                    const int64_t j = j0 + br + 4 * u;
+                    pb2[u] = *reinterpret_cast<const f64x2*>(V + (j < K ? j : K - 1) * P + pc);
+                    mb |= (j < K && p < P) ? (1u << u) : 0u;
+                }
+            } else {
+#pragma unroll
+                for (int u = 0; u < 8; ++u) {
+                    const int64_t j = j0 + br + 2 * u;
+                    pb1[u] = (j < K && p < P) ? V[j * P + p] : 0.0;
+                }
             }
         };
         auto store = [&]() {
 #pragma unroll
-            for (int u = 0; u < kAU; ++u) As[ar * kKrApad + ac + u] = pa[u];
+            for (int v = 0; v < 4; ++v) {
+                As[ar * kKrApad + ac + 2 * v] = pa[v].x;
+                As[ar * kKrApad + ac + 2 * v + 1] = pa[v].y;
+            }
+            if constexpr (kPairB) {
 #pragma unroll
-            for (int u = 0; u < kBU; ++u) Bs[(br + 2 * u) * kKrBpad + bc] = pb[u];
+                for (int u = 0; u < 4; ++u)
+                    *reinterpret_cast<f64x2*>(&Bs[(br + 4 * u) * kKrBpad + bc]) =
+                        ((mb >> u) & 1u) ? pb2[u] : f64x2{0.0, 0.0};
+            } else {
+#pragma unroll
+                for (int u = 0; u < 8; ++u) Bs[(br + 2 * u) * kKrBpad + bc] = pb1[u];
+            }
         };
 
         f64x4 acc[kKrRF][4];
@@ -343,10 +379,14 @@ __global__ void k_fill_zero_f64(double* __restrict__ p, int64_t n) {
 inline int64_t krige_row_tiles(int64_t K) { return ceil_div(K, (int64_t)kKrRows); }
 
 // field: the weight vector (K) and its row-chunk partials (kWChunks x K);
-// variance form: also the row-tile partials (tiles x P) and M2 (K x K)
+// variance form: also the row-tile partials (tiles x P) and the padded M2
+// (128 tiles x Kp), 16-byte aligned after the others (K even or padded by one)
 inline int64_t krige_ws_bytes(int64_t K, int64_t P, bool variance) {
     const int64_t field = (1 + kWChunks) * K;
-    if (variance) return (krige_row_tiles(K) * P + field + K * K) * (int64_t)sizeof(double);
+    if (variance) {
+        const int64_t head = krige_row_tiles(K) * P + field;
+        return ((head + 1) / 2 * 2 + krige_row_tiles(K) * kKrRows * krige_kp(K)) * (int64_t)sizeof(double);
+    }
     return field * (int64_t)sizeof(double);
 }
 
@@ -377,7 +417,8 @@ int gsx_krige_f64(int64_t K, int64_t P, const double* krig_mat, const double* kr
     double* ws = static_cast<double*>(workspace);
     const int64_t tiles = gsx::krige_row_tiles(K);
     const int64_t col_blocks = gsx::ceil_div(P, (int64_t)gsx::kKrPts);
-    if (variance && (tiles * (col_blocks + 8) > 0x7fffffffLL || gsx::ceil_div(K * K, (int64_t)256) > 0x7fffffffLL))
+    const int64_t Kp = gsx::krige_kp(K), Rp = tiles * gsx::kKrRows;
+    if (variance && (tiles * (col_blocks + 8) > 0x7fffffffLL || gsx::ceil_div(Rp * Kp, (int64_t)256) > 0x7fffffffLL))
         return GSX_E_INVALID;
     double* part_e = ws;
     double* a = variance ? ws + tiles * P : ws;
@@ -401,15 +442,24 @@ int gsx_krige_f64(int64_t K, int64_t P, const double* krig_mat, const double* kr
                            P, a, krig_vecs, field);
     st = gsx::launch_status();
     if (st != GSX_OK || !variance) return st;
-    // error = v^T M2 v over the block upper triangle
-    double* m2 = wpart + gsx::kWChunks * K;
-    hipLaunchKernelGGL(gsx::k_krige_sym, dim3((unsigned)gsx::ceil_div(K * K, (int64_t)256)), dim3(256), 0, s, K,
-                       krig_mat, m2);
+    // error = v^T M2 v over the block upper triangle (M2 16-byte aligned: the
+    // workspace is, and its offset is rounded up to an even number of doubles)
+    const int64_t head = tiles * P + (1 + gsx::kWChunks) * K;
+    double* m2 = ws + (head + 1) / 2 * 2;
+    if (reinterpret_cast<uintptr_t>(m2) % 16 != 0) return GSX_E_INVALID;
+    hipLaunchKernelGGL(gsx::k_krige_sym, dim3((unsigned)gsx::ceil_div(Rp * Kp, (int64_t)256)), dim3(256), 0, s, K, Rp,
+                       Kp, krig_mat, m2);
     st = gsx::launch_status();
     if (st != GSX_OK) return st;
     const int64_t col8 = gsx::ceil_div(col_blocks, (int64_t)8) * 8;
-    hipLaunchKernelGGL(gsx::k_krige_gemm, dim3((unsigned)(col8 * ((tiles + 1) / 2))), dim3(256), 0, s, K, P, tiles,
-                       col_blocks, m2, krig_vecs, part_e);
+    const dim3 grid((unsigned)(col8 * ((tiles + 1) / 2)));
+    const bool pair_b = P % 2 == 0 && reinterpret_cast<uintptr_t>(krig_vecs) % 16 == 0;
+    if (pair_b)
+        hipLaunchKernelGGL(gsx::k_krige_gemm<true>, grid, dim3(256), 0, s, K, Kp, P, tiles, col_blocks, m2, krig_vecs,
+                           part_e);
+    else
+        hipLaunchKernelGGL(gsx::k_krige_gemm<false>, grid, dim3(256), 0, s, K, Kp, P, tiles, col_blocks, m2,
+                           krig_vecs, part_e);
     st = gsx::launch_status();
     if (st != GSX_OK) return st;
     hipLaunchKernelGGL(gsx::k_krige_reduce, dim3((unsigned)gsx::ceil_div(P, (int64_t)256)), dim3(256), 0, s, tiles,

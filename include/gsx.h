@@ -299,7 +299,8 @@ int gsx_summate_adjoint_f32_noalloc(int64_t d, int64_t N, int64_t P,
  * computes the field only (calc_field_krige), otherwise both
  * (calc_field_krige_and_variance; the caller forms max(sill - error, 0)).
  * Replaces the external kernels called at src/gstools/krige/base.py:42-61.
- * Workspace bytes from gsx_krige_workspace(K, P, error != NULL). */
+ * Workspace bytes from gsx_krige_workspace(K, P, error != NULL), 16-byte
+ * aligned (GSX_E_INVALID otherwise when error != NULL). */
 int64_t gsx_krige_workspace(int64_t K, int64_t P, int variance);
 /* Covariance rows of krig_vecs: out[i * ld_out + p] = C(|c_i - x_p|) for
  * conditioning points cond_pos (dim, K) and targets pos (dim, P), both

@@ -34,7 +34,8 @@ def _bounds(m, v, c):
     return np.abs(c) @ scale, np.sum(np.abs(v) * scale, axis=0)
 
 
-@pytest.mark.parametrize("k,p", [(1, 1), (5, 7), (64, 128), (65, 129), (33, 130), (130, 1000), (600, 3001), (17, 0)])
+@pytest.mark.parametrize("k,p", [(1, 1), (5, 7), (64, 128), (65, 129), (33, 130), (130, 1000), (256, 258), (600, 3001),
+                                 (17, 0)])
 def test_kernel_matches_oracle(k, p):
     _say(f"krige K={k} P={p}")
     rs = np.random.RandomState(k * 7 + p)
