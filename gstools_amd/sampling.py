@@ -29,8 +29,8 @@ device is present): ``gsx_stretch_run_device`` advances every chain in one
 launch, one block of two wavefronts per chain (the stream wave draws the
 next step while the walker wave applies the current one), drawing from the
 chain's own MT19937 streams in the same order and evaluating the
-closed-form log-pdf on the device; the final ``choice`` indices come from numpy and are gathered on the
-device (``gsx_stretch_pick_device``).  Accept decisions compare the same
+closed-form log-pdf on the device; the final ``choice`` indices come from
+numpy and are gathered on the device (``gsx_stretch_pick_device``).  Accept decisions compare the same
 quantities up to the last bits of exp / log, so the modes come out bit for
 bit equal to the host sampler's unless a test lands within an ulp.
 """
