@@ -1,0 +1,107 @@
+"""Parity at the full sizes of the benchmark configurations.
+
+The oracle finishes only samples of these, so the checks are of two kinds:
+sampled points against the oracle or the FP64 kernel, and properties that
+hold at any size -- a point's sum does not depend on the launch it belongs
+to, and the sum is linear in the amplitudes (z1, z2).
+
+* 3D Exponential (var 2, len [50, 20, 5], angles [0.5, 0.2, 0.1]), N 1024
+  MCMC modes (seed 20170519), 4e6 points uniform in [0, 1000]^3
+  (RandomState(19970221)): the headline workload.
+* 3D Matern (nu 1.5, len 8), N 4096 MCMC modes (seed 20170519) on the
+  structured 512^3 grid with spacing 0.5, split bf16 on the matrix cores.
+"""
+import sys
+
+import numpy as np
+import pytest
+import torch
+
+import oracle
+from gstools_amd import Exponential, Matern, RandMeth
+from gstools_amd.geometry import generate_grid
+from gstools_amd.summate import summate_device, summate_grid_device
+
+pytestmark = pytest.mark.gpu
+
+FP64_TOL = 1e-12
+FP32_TOL = 1e-5
+
+
+def _say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def test_headline_workload_full_size():
+    _say("full size: 3D Exponential, 4e6 points, N 1024")
+    model = Exponential(dim=3, var=2.0, len_scale=[50, 20, 5], angles=[0.5, 0.2, 0.1])
+    rm = RandMeth(model, mode_no=1024, seed=20170519)
+    cov, z1, z2 = rm._cov_sample, rm._z_1, rm._z_2
+    pos = np.random.RandomState(19970221).uniform(0.0, 1000.0, size=(3, 4_000_000))
+    iso = np.ascontiguousarray(model.isometrize(pos))
+    dev_pos = torch.from_numpy(iso).cuda()
+    u64 = summate_device(cov, z1, z2, dev_pos, "fp64")
+    u32 = summate_device(cov, z1, z2, dev_pos, "fp32")
+    torch.cuda.synchronize()
+    # FP32 against FP64 over every point
+    ref = u64.cpu().numpy()
+    rel_max, rel_rms = oracle.rel_errors(u32.cpu().numpy().astype(np.float64), ref)
+    _say(f"  fp32 vs fp64 over 4e6 points: rel_max {rel_max:.3e} rel_rms {rel_rms:.3e}")
+    assert rel_max <= FP32_TOL and rel_rms <= FP32_TOL
+    # FP64 against the oracle on a strided sample spanning the whole set
+    idx = np.arange(0, 4_000_000, 977)
+    sample = np.ascontiguousarray(iso[:, idx])
+    want = oracle.summate_c(cov, z1, z2, sample)
+    assert oracle.rel_errors(ref[idx], want)[0] <= FP64_TOL
+    # a point's sum does not depend on the launch it belongs to
+    for precision, whole in (("fp64", u64), ("fp32", u32)):
+        part = summate_device(cov, z1, z2, torch.from_numpy(sample).cuda(), precision).cpu().numpy()
+        assert np.array_equal(part, whole.cpu().numpy()[idx]), precision
+
+
+def test_matern_grid_full_size_split_bf16():
+    _say("full size: 3D Matern 512^3, N 4096, bf16x3")
+    model = Matern(dim=3, var=1.0, len_scale=8.0, nu=1.5)
+    rm = RandMeth(model, mode_no=4096, seed=20170519)
+    cov, z1, z2 = rm._cov_sample, rm._z_1, rm._z_2
+    ax = np.arange(512) * 0.5
+    axes = [ax, ax, ax]
+    u = summate_grid_device(cov, z1, z2, axes, "bf16x3").view(512, 512, 512)
+    # three full axis-0 planes against the FP64 grid kernel
+    for i0 in (0, 255, 511):
+        ref = summate_grid_device(cov, z1, z2, [ax[i0:i0 + 1], ax, ax], "fp64").view(512, 512)
+        got = u[i0].to(torch.float64)
+        err = (got - ref).flatten()
+        rel_max = float(err.abs().max() / ref.abs().max())
+        rel_rms = float(err.norm() / ref.norm())
+        _say(f"  plane {i0}: rel_max {rel_max:.3e} rel_rms {rel_rms:.3e}")
+        assert rel_rms <= FP32_TOL and rel_max <= 5 * FP32_TOL
+    # the oracle on a few points of the grid
+    pts = [(0, 0, 0), (511, 511, 511), (100, 7, 300), (300, 450, 2)]
+    want = oracle.summate_c(cov, z1, z2, np.array([[ax[i], ax[j], ax[k]] for i, j, k in pts]).T.copy())
+    got = np.array([float(u[i, j, k]) for i, j, k in pts])
+    assert np.abs(got - want).max() <= 5 * FP32_TOL * np.abs(ref.cpu().numpy()).max()
+    # linear in the amplitudes at full size: u(z) + u(z') = u(z + z') to the split rounding
+    rs = np.random.RandomState(5)
+    y1, y2 = rs.normal(size=z1.shape), rs.normal(size=z2.shape)
+    v = summate_grid_device(cov, y1, y2, axes, "bf16x3")
+    w = summate_grid_device(cov, z1 + y1, z2 + y2, axes, "bf16x3")
+    lin = (u.flatten().to(torch.float64) + v.to(torch.float64)) - w.to(torch.float64)
+    rel = float(lin.norm() / w.to(torch.float64).norm())
+    _say(f"  linearity over 512^3: rel_rms {rel:.3e}")
+    assert rel <= 2 * FP32_TOL
+
+
+def test_grid_full_size_generate_grid_consistent():
+    """The structured sum equals the sum over the materialised meshgrid
+    (generate_grid, the reference's Field.pre_pos route) at a full-size axis
+    length: one 512-point axis 0 row of the 512^3 grid, FP64, bit for bit."""
+    model = Matern(dim=3, var=1.0, len_scale=8.0, nu=1.5)
+    rm = RandMeth(model, mode_no=256, seed=20170519)
+    ax = np.arange(512) * 0.5
+    axes = [ax, ax[200:201], ax[17:18]]
+    grid = summate_grid_device(rm._cov_sample, rm._z_1, rm._z_2, axes, "fp64").cpu().numpy()
+    flat = summate_device(rm._cov_sample, rm._z_1, rm._z_2, np.ascontiguousarray(generate_grid(axes)),
+                          "fp64").cpu().numpy()
+    assert np.array_equal(grid, flat)
