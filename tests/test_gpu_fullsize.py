@@ -11,6 +11,7 @@ to, and the sum is linear in the amplitudes (z1, z2).
 * 3D Matern (nu 1.5, len 8), N 4096 MCMC modes (seed 20170519) on the
   structured 512^3 grid with spacing 0.5, split bf16 on the matrix cores.
 * Ordinary kriging, 1001 x 1e6 (field and variance on the matrix cores).
+* Variogram pair binning of 1e6 points and axis lags of a 1024 x 100000 array.
 """
 import sys
 
@@ -141,3 +142,40 @@ def test_kriging_full_size():
     assert np.all(np.abs(ee - e_ref) <= 1e-13 * np.sum(np.abs(vs) * scale, axis=0) + 1e-300)
     f2, e2 = gkrige.krige_sum_device(mat, vs_dev, cond, variance=True)
     assert np.array_equal(f2.cpu().numpy(), fe) and np.array_equal(e2.cpu().numpy(), ee)
+
+
+def test_variogram_full_size():
+    """Unstructured 3D variogram, 1e6 points uniform in [0, 1000]^3, one field,
+    20 bins to 577, Matheron: one bin over [0, inf) counts every one of the
+    5e11 pairs exactly once, the 20 bins plus the tail bin [577, inf) count
+    them all too, and a 3000-point subset matches the oracle (counts exactly).
+    Axis lags of a 1024 x 100000 array: counts exact at every lag, spot lags
+    against numpy."""
+    from gstools_amd import variogram as gv
+
+    _say("full size: variogram 1e6 points, axis lags 1024 x 100000")
+    rs = np.random.RandomState(19970221)
+    n = 1_000_000
+    pos = rs.uniform(0, 1000, size=(3, n))
+    field = rs.normal(size=(1, n))
+    edges = np.linspace(0, 577.0, 21)
+    pt, ft = torch.from_numpy(pos).cuda(), torch.from_numpy(field).cuda()
+    dev_edges = lambda e: torch.tensor(e, dtype=torch.float64, device="cuda")
+    _, c20 = gv._pairs(ft, dev_edges(edges), pt, "m")
+    _, c_all = gv._pairs(ft, dev_edges([0.0, np.inf]), pt, "m")
+    _, c_tail = gv._pairs(ft, dev_edges([577.0, np.inf]), pt, "m")
+    pairs = n * (n - 1) // 2
+    assert int(c_all.sum().item()) == pairs
+    assert int(c20.sum().item()) + int(c_tail.sum().item()) == pairs
+    sub = rs.choice(n, 3000, replace=False)
+    g_sub, c_sub = gv.unstructured(field[:, sub], edges, pos[:, sub])
+    g_ref, c_ref = oracle.vario_unstructured_c(field[:, sub], edges, pos[:, sub])
+    assert np.array_equal(c_sub, c_ref)
+    assert np.max(np.abs(g_sub - g_ref) / np.maximum(np.abs(g_ref), 1e-300)) <= 1e-12
+    lag_np = rs.normal(size=(1024, 100_000))
+    lg, lc = (t.cpu().numpy() for t in gv._lags(torch.from_numpy(lag_np).cuda(), None, "m"))
+    assert np.array_equal(lc[1:], (1024 - np.arange(1, 1024)) * 100_000)
+    for k in (1, 2, 7, 31, 32, 33, 500, 1023):
+        d = lag_np[k:] - lag_np[:-k]
+        ref = np.sum(d * d) / (2.0 * d.size)
+        assert abs(lg[k] - ref) <= 1e-12 * ref, k
