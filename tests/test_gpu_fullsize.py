@@ -12,6 +12,7 @@ to, and the sum is linear in the amplitudes (z1, z2).
   structured 512^3 grid with spacing 0.5, split bf16 on the matrix cores.
 * Ordinary kriging, 1001 x 1e6 (field and variance on the matrix cores).
 * Variogram pair binning of 1e6 points and axis lags of a 1024 x 100000 array.
+* The Fourier method on the 4096^2 grid with 256^2 modes.
 """
 import sys
 
@@ -179,3 +180,34 @@ def test_variogram_full_size():
         d = lag_np[k:] - lag_np[:-k]
         ref = np.sum(d * d) / (2.0 * d.size)
         assert abs(lg[k] - ref) <= 1e-12 * ref, k
+
+
+def test_fourier_full_size():
+    """Periodic 2D Gaussian (var 1, len 200) on the 4096^2 grid, 256^2 modes
+    (Fourier method, seed 1681903): the per-axis FP64 contraction over the
+    whole grid against the direct FP64 sum on five full rows (1e-12) and the
+    oracle on points of those rows; split bf16 over the whole grid against
+    the same rows (1e-5)."""
+    from gstools_amd import Fourier, Gaussian, summate_fourier_grid_device, summate_fourier_tensor_device
+
+    _say("full size: Fourier 4096^2, 256^2 modes")
+    side, m = 4096, 256
+    gen = Fourier(Gaussian(dim=2, var=1.0, len_scale=200.0), period=float(side), mode_no=m, seed=1681903)
+    fac, modes, z1, z2 = (torch.from_numpy(np.ascontiguousarray(a)).cuda()
+                          for a in (gen._spectrum_factor, gen._modes, gen._z_1, gen._z_2))
+    ax = np.linspace(0.0, float(side), side, endpoint=False)
+    axes = [torch.from_numpy(ax).cuda()] * 2
+    rows = np.array([0, 1, 1000, 2047, 4095])
+    ref = summate_fourier_grid_device(fac, modes, z1, z2, [torch.from_numpy(ax[rows]).cuda(), axes[1]], "fp64")
+    ref = ref.reshape(len(rows), side).cpu().numpy()
+    cols = np.array([0, 17, 2048, 4095])
+    pts = np.stack(np.meshgrid(ax[rows], ax[cols], indexing="ij")).reshape(2, -1)
+    ora = oracle.summate_fourier_c(gen._spectrum_factor, gen._modes, gen._z_1, gen._z_2, pts)
+    assert oracle.rel_errors(ref[:, cols].ravel(), ora)[0] <= FP64_TOL
+    kax = [torch.from_numpy(k).cuda() for k in gen._kaxes]
+    ten = summate_fourier_tensor_device(fac, kax, z1, z2, axes).reshape(side, side)[rows].cpu().numpy()
+    assert oracle.rel_errors(ten, ref)[0] <= FP64_TOL
+    bf = summate_fourier_grid_device(fac, modes, z1, z2, axes, "bf16x3").reshape(side, side)[rows]
+    rel_max, rel_rms = oracle.rel_errors(bf.cpu().numpy().astype(np.float64), ref)
+    _say(f"  bf16x3 on five rows: rel_max {rel_max:.3e} rel_rms {rel_rms:.3e}")
+    assert rel_rms <= FP32_TOL and rel_max <= 5 * FP32_TOL
