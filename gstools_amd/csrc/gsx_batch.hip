@@ -15,14 +15,18 @@
 // sequence can be captured in a HIP graph), then a persistent kernel with
 // (CUs x resident blocks) blocks pulls tiles with one atomicAdd per tile until
 // the list is exhausted; every block exits when the counter passes the end.
-// Tile work is the FP32 path of gsx_summate.hip: FP64 phase in revolutions,
-// v_fract_f64, v_cos_f32, blocked FP32 -> FP64 mode sums, modes staged through
-// LDS in chunks of CHUNK (both PPT and CHUNK are autotuned variants).
+// Tile work is the FP32 sum of gsx_fp32sum.h, the one the single launches run:
+// FP64 phase in revolutions, magic-addition or fract reduction, v_cos_f32,
+// blocked FP32 -> FP64 mode sums, modes staged through LDS in chunks of CHUNK
+// (both PPT and CHUNK are autotuned variants).  Its segment and mode-group
+// rules do not depend on PPT or CHUNK, so every variant, and the single
+// launch, give a point the same bits.
 #include <stdlib.h>
 
 #include <algorithm>
 #include <vector>
 
+#include "gsx_fp32sum.h"
 #include "gsx_internal.h"
 
 namespace gsx {
@@ -33,22 +37,22 @@ struct BatchTile {
     int64_t p0;
 };
 
-constexpr int kBatchSub = 64;
-
 template <int D, bool INCOMPR, int PPT, int CHUNK, class Pts>
 __device__ __forceinline__ void batch_tile(const gsx_batch_item& it, Pts pts, int64_t p0, double* tabd,
-                                           float* tabf) {
+                                           float* tabf, double* s_wmax, int* s_nmagic) {
     constexpr int NOUT = INCOMPR ? D : 1;
-    constexpr int kWd = D + 1;
-    const int64_t N = it.N, P = it.P;
+    const int64_t P = it.P;
     double x[PPT][D];
     double acc[PPT][NOUT];
+    double own = 0.0;
     const int64_t base = p0 + threadIdx.x;
 #pragma unroll
     for (int p = 0; p < PPT; ++p) {
         const int64_t i = base + (int64_t)p * kThreads;
         if (i < P) {
             pts.get(i, x[p]);
+#pragma unroll
+            for (int d = 0; d < D; ++d) own = coord_bound(own, x[p][d]);
         } else {
 #pragma unroll
             for (int d = 0; d < D; ++d) x[p][d] = 0.0;
@@ -56,64 +60,9 @@ __device__ __forceinline__ void batch_tile(const gsx_batch_item& it, Pts pts, in
 #pragma unroll
         for (int c = 0; c < NOUT; ++c) acc[p][c] = 0.0;
     }
-    const double* __restrict__ cov = it.cov;
-    for (int64_t j0 = 0; j0 < N; j0 += CHUNK) {
-        const int cnt = (int)(N - j0 < CHUNK ? N - j0 : CHUNK);
-        __syncthreads();
-        for (int m = threadIdx.x; m < cnt; m += kThreads) {
-            const int64_t j = j0 + m;
-            double k[D];
-            double k2 = 0.0;
-#pragma unroll
-            for (int d = 0; d < D; ++d) {
-                k[d] = cov[d * N + j];
-                tabd[m * kWd + d] = k[d] * kInv2Pi;
-                k2 += k[d] * k[d];
-            }
-            const double u1 = it.z1[j], u2 = it.z2[j];
-            tabd[m * kWd + D] = -atan2(u2, u1) * kInv2Pi;
-            const double a = hypot(u1, u2);
-            if constexpr (INCOMPR) {
-#pragma unroll
-                for (int c = 0; c < D; ++c)
-                    tabf[m * NOUT + c] = (float)(((c == 0 ? 1.0 : 0.0) - k[c] * k[0] / k2) * a);
-            } else {
-                tabf[m] = (float)a;
-            }
-        }
-        __syncthreads();
-        for (int m0 = 0; m0 < cnt; m0 += kBatchSub) {
-            const int m1 = (m0 + kBatchSub < cnt) ? m0 + kBatchSub : cnt;
-            float part[PPT][NOUT];
-#pragma unroll
-            for (int p = 0; p < PPT; ++p)
-#pragma unroll
-                for (int c = 0; c < NOUT; ++c) part[p][c] = 0.0f;
-#pragma unroll 2
-            for (int m = m0; m < m1; ++m) {
-                double kr[D];
-#pragma unroll
-                for (int d = 0; d < D; ++d) kr[d] = tabd[m * kWd + d];
-                const double mphi = tabd[m * kWd + D];
-                float w[NOUT];
-#pragma unroll
-                for (int c = 0; c < NOUT; ++c) w[c] = tabf[m * NOUT + c];
-#pragma unroll
-                for (int p = 0; p < PPT; ++p) {
-                    double rev = mphi;
-#pragma unroll
-                    for (int d = 0; d < D; ++d) rev = fma_(kr[d], x[p][d], rev);
-                    const float cs = __builtin_amdgcn_cosf((float)__builtin_amdgcn_fract(rev));
-#pragma unroll
-                    for (int c = 0; c < NOUT; ++c) part[p][c] = __builtin_fmaf(w[c], cs, part[p][c]);
-                }
-            }
-#pragma unroll
-            for (int p = 0; p < PPT; ++p)
-#pragma unroll
-                for (int c = 0; c < NOUT; ++c) acc[p][c] += (double)part[p][c];
-        }
-    }
+    const double xseg = segment_xmax<D, PPT>(pts, P, p0, own, s_wmax);
+    fp32_item_sum<D, PPT, INCOMPR, CHUNK>(it.N, it.cov, it.z1, it.z2, nullptr, xseg, x, acc, tabd, tabf,
+                                          s_nmagic);
 #pragma unroll
     for (int p = 0; p < PPT; ++p) {
         const int64_t i = base + (int64_t)p * kThreads;
@@ -126,16 +75,18 @@ __device__ __forceinline__ void batch_tile(const gsx_batch_item& it, Pts pts, in
 
 template <int D, bool INCOMPR, int PPT, int CHUNK>
 __device__ __forceinline__ void batch_dispatch_pts(const gsx_batch_item& it, int64_t p0, double* tabd,
-                                                   float* tabf) {
+                                                   float* tabf, double* s_wmax, int* s_nmagic) {
     if (it.axes) {
         const double* a0 = it.axes;
         const double* a1 = D > 1 ? a0 + it.shape[0] : nullptr;
         const double* a2 = D > 2 ? a1 + it.shape[1] : nullptr;
         const uint32_t n1 = D > 1 ? (uint32_t)it.shape[1] : 1u;
         const uint32_t n2 = D > 2 ? (uint32_t)it.shape[2] : 1u;
-        batch_tile<D, INCOMPR, PPT, CHUNK>(it, GridPoints<D>{a0, a1, a2, n1, n2}, p0, tabd, tabf);
+        batch_tile<D, INCOMPR, PPT, CHUNK>(it, GridPoints<D>{a0, a1, a2, n1, n2}, p0, tabd, tabf, s_wmax,
+                                           s_nmagic);
     } else {
-        batch_tile<D, INCOMPR, PPT, CHUNK>(it, UnstructuredPoints<D>{it.pos, it.P}, p0, tabd, tabf);
+        batch_tile<D, INCOMPR, PPT, CHUNK>(it, UnstructuredPoints<D>{it.pos, it.P}, p0, tabd, tabf, s_wmax,
+                                           s_nmagic);
     }
 }
 
@@ -149,6 +100,8 @@ __global__ __launch_bounds__(kThreads) void k_batch(const gsx_batch_item* __rest
                                                     unsigned* __restrict__ counter) {
     __shared__ double tabd[CHUNK * 4];
     __shared__ float tabf[CHUNK * 3];
+    __shared__ double s_wmax[kWaves];
+    __shared__ int s_nmagic[kWaves];
     __shared__ unsigned s_tile;
     for (;;) {
         if (threadIdx.x == 0) s_tile = atomicAdd(counter, 1u);
@@ -159,12 +112,12 @@ __global__ __launch_bounds__(kThreads) void k_batch(const gsx_batch_item* __rest
         const BatchTile tl = tiles[t];
         const gsx_batch_item it = items[tl.item];
         if (it.kind == 1) {
-            if (it.d == 2) batch_dispatch_pts<2, true, PPT, CHUNK>(it, tl.p0, tabd, tabf);
-            else batch_dispatch_pts<3, true, PPT, CHUNK>(it, tl.p0, tabd, tabf);
+            if (it.d == 2) batch_dispatch_pts<2, true, PPT, CHUNK>(it, tl.p0, tabd, tabf, s_wmax, s_nmagic);
+            else batch_dispatch_pts<3, true, PPT, CHUNK>(it, tl.p0, tabd, tabf, s_wmax, s_nmagic);
         } else {
-            if (it.d == 1) batch_dispatch_pts<1, false, PPT, CHUNK>(it, tl.p0, tabd, tabf);
-            else if (it.d == 2) batch_dispatch_pts<2, false, PPT, CHUNK>(it, tl.p0, tabd, tabf);
-            else batch_dispatch_pts<3, false, PPT, CHUNK>(it, tl.p0, tabd, tabf);
+            if (it.d == 1) batch_dispatch_pts<1, false, PPT, CHUNK>(it, tl.p0, tabd, tabf, s_wmax, s_nmagic);
+            else if (it.d == 2) batch_dispatch_pts<2, false, PPT, CHUNK>(it, tl.p0, tabd, tabf, s_wmax, s_nmagic);
+            else batch_dispatch_pts<3, false, PPT, CHUNK>(it, tl.p0, tabd, tabf, s_wmax, s_nmagic);
         }
     }
 }

@@ -57,6 +57,31 @@ struct GridPoints {
     }
 };
 
+// FP32 phase reduction by magic addition.  A phase chain started at
+// kPhaseMagic + kPhaseRound - phi/2pi and summed by FMAs keeps a phase t (in
+// revolutions, |t| < 2^19) inside [2^20, 2^21), where the FP64 ulp is 2^-32:
+// the low 32 bits of the sum hold frac(t + 2^-24) in 32-bit fixed point.  Their
+// top 23 bits under the exponent of 1.0 (one v_alignbit_b32) are the float
+// 1 + frac(t) rounded to 2^-23, and since v_cos_f32 / v_sin_f32 take
+// revolutions, cos(2 pi (1 + f)) = cos(2 pi t).  This replaces v_fract_f64 and
+// v_cvt_f32_f64 by one integer op; the phase error is at most 6e-8 revolutions
+// (3e-8 for the fract-and-convert form).  Callers check |k/2pi|_1 |x|_inf <=
+// kPhaseMagicMax per chunk of modes and otherwise use the fract form.
+constexpr double kPhaseMagic = 0x1.8p+20;
+constexpr double kPhaseRound = 0x1.0p-24;
+constexpr double kPhaseMagicMax = 0x1.0p+18;
+
+__device__ __forceinline__ float magic_rev_to_unit(double rev) {
+    const uint32_t lo = (uint32_t)__double_as_longlong(rev);
+    return __uint_as_float(__builtin_amdgcn_alignbit(0x7Fu, lo, 9u));
+}
+
+// Running bound max(xm, |v|), infinite once v is NaN or infinite.
+__device__ __forceinline__ double coord_bound(double xm, double v) {
+    const double a = fabs(v);
+    return a <= 0x1.fffffffffffffp+1023 ? fmax(xm, a) : __builtin_inf();
+}
+
 // Phase k.x accumulated in dimension order with every product rounded
 // before the add, as a plain C loop without fused multiply-add computes it.
 template <int D>

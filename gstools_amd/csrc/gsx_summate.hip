@@ -26,18 +26,22 @@
 // whose |x|_inf * max_j |k_j|_1 may exceed 2^45 takes a per-term guarded path
 // that falls back to the library sincos for |t| > 2^46.
 //
-// FP32 path: the phase is computed in revolutions in FP64 (t = k/2pi . x -
-// phi/2pi, three FMAs), reduced exactly with v_fract_f64, converted to FP32
-// and fed to the hardware cosine (v_cos_f32 takes revolutions).  The mode sum
+// FP32 path (gsx_fp32sum.h): the phase is computed in revolutions in FP64
+// (t = k/2pi . x - phi/2pi, three FMAs), reduced by magic addition (the low
+// word of the sum is the fraction in fixed point, one v_alignbit_b32 makes it
+// a float in [1, 2)) for the modes whose |k/2pi|_1 times the largest |x|_inf
+// of the point's 2048-point segment stays within 2^18 revolutions, else
+// exactly with v_fract_f64 and a conversion, and fed to the hardware cosine
+// (v_cos_f32 takes revolutions).  The mode sum
 // runs in FP32 for blocks of kSub modes and is then added into an FP64
 // accumulator (blocked compensation: FP32 rounding error grows only over
 // kSub terms).
+#include "gsx_fp32sum.h"
 #include "gsx_internal.h"
 
 namespace gsx {
 
 constexpr int kChunk = 256;  // modes per LDS chunk (== kThreads: one mode per thread to build)
-constexpr int kSub = 64;     // FP32 terms per FP32 partial sum
 constexpr double kSafeK1X = 0x1.0p+45;  // |k|_1 |x|_inf bound of the FP64 fast path
 
 // cos(theta - phi) at any |theta|: the fast reduction where it is exact, else the
@@ -193,26 +197,32 @@ __global__ __launch_bounds__(kThreads) void k_forward_f64(Pts pts, int64_t N, in
 }
 
 // -------------------------------------------------------------- FP32 --------
+// One block: kThreads * PPT points (a divisor of the kSeg-point segment), all
+// modes through fp32_item_sum (gsx_fp32sum.h).  At least 3 waves per SIMD: the
+// two inlined mode loops would otherwise take the 3D sum to 198 VGPRs (2
+// waves); bounded, it fits in 168.
 template <int D, int PPT, bool INCOMPR, class Pts>
-__global__ __launch_bounds__(kThreads) void k_forward_f32(Pts pts, int64_t N, int64_t P,
-                                                          const double* __restrict__ cov,
-                                                          const double* __restrict__ z1,
-                                                          const double* __restrict__ z2,
-                                                          const double* __restrict__ fac,
-                                                          float* __restrict__ out) {
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(3))) void k_forward_f32(
+    Pts pts, int64_t N, int64_t P, const double* __restrict__ cov, const double* __restrict__ z1,
+    const double* __restrict__ z2, const double* __restrict__ fac, float* __restrict__ out) {
     constexpr int NOUT = INCOMPR ? D : 1;
-    constexpr int kWd = D + 1;  // k/2pi [D], -phi/2pi
-    __shared__ double tabd[kChunk * kWd];
+    __shared__ double tabd[kChunk * (D + 1)];
     __shared__ float tabf[kChunk * NOUT];  // a (scalar) or p(k) * a (incompr)
+    __shared__ double s_wmax[kWaves];
+    __shared__ int s_nmagic[kWaves];
 
     double x[PPT][D];
     double acc[PPT][NOUT];
-    const int64_t base = (int64_t)blockIdx.x * (kThreads * PPT) + threadIdx.x;
+    double own = 0.0;
+    const int64_t p0 = (int64_t)blockIdx.x * (kThreads * PPT);
+    const int64_t base = p0 + threadIdx.x;
 #pragma unroll
     for (int p = 0; p < PPT; ++p) {
         const int64_t i = base + (int64_t)p * kThreads;
         if (i < P) {
             pts.get(i, x[p]);
+#pragma unroll
+            for (int d = 0; d < D; ++d) own = coord_bound(own, x[p][d]);
         } else {
 #pragma unroll
             for (int d = 0; d < D; ++d) x[p][d] = 0.0;
@@ -220,66 +230,8 @@ __global__ __launch_bounds__(kThreads) void k_forward_f32(Pts pts, int64_t N, in
 #pragma unroll
         for (int c = 0; c < NOUT; ++c) acc[p][c] = 0.0;
     }
-
-    for (int64_t j0 = 0; j0 < N; j0 += kChunk) {
-        const int cnt = (int)(N - j0 < kChunk ? N - j0 : kChunk);
-        __syncthreads();
-        if ((int)threadIdx.x < cnt) {
-            const int64_t j = j0 + threadIdx.x;
-            double k[D];
-            double k2 = 0.0;
-#pragma unroll
-            for (int d = 0; d < D; ++d) {
-                k[d] = cov[d * N + j];
-                tabd[threadIdx.x * kWd + d] = k[d] * kInv2Pi;
-                k2 += k[d] * k[d];
-            }
-            const double u1 = z1[j], u2 = z2[j];
-            tabd[threadIdx.x * kWd + D] = -atan2(u2, u1) * kInv2Pi;
-            const double a = fac ? fac[j] * hypot(u1, u2) : hypot(u1, u2);
-            if constexpr (INCOMPR) {
-#pragma unroll
-                for (int c = 0; c < D; ++c)
-                    tabf[threadIdx.x * NOUT + c] = (float)(((c == 0 ? 1.0 : 0.0) - k[c] * k[0] / k2) * a);
-            } else {
-                tabf[threadIdx.x] = (float)a;
-            }
-        }
-        __syncthreads();
-        for (int m0 = 0; m0 < cnt; m0 += kSub) {
-            const int m1 = (m0 + kSub < cnt) ? m0 + kSub : cnt;
-            float part[PPT][NOUT];
-#pragma unroll
-            for (int p = 0; p < PPT; ++p)
-#pragma unroll
-                for (int c = 0; c < NOUT; ++c) part[p][c] = 0.0f;
-#pragma unroll 2
-            for (int m = m0; m < m1; ++m) {
-                const double* t = tabd + m * kWd;
-                double kr[D];
-#pragma unroll
-                for (int d = 0; d < D; ++d) kr[d] = t[d];
-                const double mphi = t[D];
-                float w[NOUT];
-#pragma unroll
-                for (int c = 0; c < NOUT; ++c) w[c] = tabf[m * NOUT + c];
-#pragma unroll
-                for (int p = 0; p < PPT; ++p) {
-                    double rev = mphi;
-#pragma unroll
-                    for (int d = 0; d < D; ++d) rev = fma_(kr[d], x[p][d], rev);
-                    const float f = (float)__builtin_amdgcn_fract(rev);
-                    const float cs = __builtin_amdgcn_cosf(f);
-#pragma unroll
-                    for (int c = 0; c < NOUT; ++c) part[p][c] = __builtin_fmaf(w[c], cs, part[p][c]);
-                }
-            }
-#pragma unroll
-            for (int p = 0; p < PPT; ++p)
-#pragma unroll
-                for (int c = 0; c < NOUT; ++c) acc[p][c] += (double)part[p][c];
-        }
-    }
+    const double xseg = segment_xmax<D, PPT>(pts, P, p0, own, s_wmax);
+    fp32_item_sum<D, PPT, INCOMPR, kChunk>(N, cov, z1, z2, fac, xseg, x, acc, tabd, tabf, s_nmagic);
 
 #pragma unroll
     for (int p = 0; p < PPT; ++p) {

@@ -36,6 +36,11 @@ def _mixed_items(seed=0):
     specs.append((2, 700, True, rs.uniform(0, 50, size=(2, 3000)), None))
     specs.append((1, 129, False, rs.uniform(-5, 5, size=(1, 999)), None))
     specs.append((2, 64, False, rs.uniform(0, 10, size=(2, 1)), None))
+    # the second 2048-point segment reaches |x| ~ 3e6: there the large modes of
+    # every 128-mode group take the fract reduction, the small ones the magic
+    far = rs.uniform(0, 1000, size=(3, 6000))
+    far[:, 2048:4096] *= 3000.0
+    specs.append((3, 300, False, far, None))
     items, refs = [], []
     for d, n, incompr, pos, axes in specs:
         cov = rs.normal(size=(d, n)) * 0.3

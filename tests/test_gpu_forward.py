@@ -82,6 +82,32 @@ def test_fp32_within_1e5(d):
     assert rel_max <= FP32_TOL and rel_rms <= FP32_TOL, (rel_max, rel_rms)
 
 
+@pytest.mark.parametrize("incompr", [False, True])
+def test_fp32_phase_reductions_both_regimes(incompr):
+    """The FP32 kernels reduce the phase by magic addition where a wave's
+    |k/2pi|_1 |x|_inf stays below 2^18 revolutions and by fract-and-convert
+    beyond.  Points at |x| ~ 3e6 (phases ~1e6 revolutions) next to points at
+    |x| ~ 100 put both in one launch, over 5 mode chunks; both must meet the
+    FP32 bar, and a wave of small points gives the same sums whichever
+    regime its neighbours in other waves take."""
+    _say(f"fp32 magic and fract reductions incompr={incompr}")
+    cov, z1, z2, pos = _case(3, 1100, 8192, 91, 1.0, 100.0)
+    big = pos.copy()
+    big[:, 2048:4096] *= 3e4  # the second 256-point block: all of its waves take the fract form
+    fn = summate_incompr_device if incompr else summate_device
+    ref_fn = oracle.summate_incompr_c if incompr else oracle.summate_c
+    bound = np.abs(cov).sum(axis=0).max() / (2 * np.pi) * np.abs(big[:, 2048:4096]).max()
+    assert bound > 2.0**18 and np.abs(cov).sum(axis=0).max() / (2 * np.pi) * 100.0 < 2.0**18
+    got = fn(cov, z1, z2, big, precision="fp32").cpu().numpy()
+    ref = ref_fn(cov, z1, z2, big)
+    for sl in (np.s_[..., 2048:4096], np.s_[..., :2048]):
+        rel_max, rel_rms = oracle.rel_errors(got[sl], ref[sl])
+        assert rel_max <= FP32_TOL and rel_rms <= FP32_TOL, (sl, rel_max, rel_rms)
+    small = fn(cov, z1, z2, pos, precision="fp32").cpu().numpy()
+    assert np.array_equal(small[..., :2048], got[..., :2048])
+    assert np.array_equal(small[..., 4096:], got[..., 4096:])
+
+
 @pytest.mark.parametrize("incompr,p_big", [(False, 4_500_000), (True, 2_500_000)])
 def test_fp32_full_chip_launch_matches_prefix(incompr, p_big):
     """A launch of millions of points (thousands of blocks, every compute unit
