@@ -18,7 +18,9 @@
 //
 // FP64: exact reference phase and sincos_fast (two-term reduction) with a
 // library fallback for waves whose phases may exceed 2^46.  FP32: phase in
-// revolutions in FP64, v_fract_f64, hardware v_sin_f32 / v_cos_f32, FP32
+// revolutions in FP64, reduced by magic addition (gsx_internal.h) where the
+// wave's |k/2pi|_1 times the stage's largest |x|_inf stays within 2^18
+// revolutions, else by v_fract_f64, hardware v_sin_f32 / v_cos_f32, FP32
 // partial sums per LDS chunk folded into FP64 accumulators.  The FP32 path
 // sums over the shifted phase psi_ij = t_ij - phi_j, z1 + i z2 = a e^{i phi}
 // (the forward kernel's amplitude-phase form), which needs 2 + D sums per
@@ -75,6 +77,42 @@ constexpr int adj_sums() {
 // kThreads * MPT modes): every staged point's LDS reads and the loop's
 // bookkeeping serve MPT modes; each mode sums its points in the same order
 // whatever MPT is.  The FP64 path takes MPT = 1.
+// One LDS stage of cnt points into the FP32 partial sums of this thread's MPT
+// modes.  MAGIC: the phase chain starts at kPhaseMagic + kPhaseRound -
+// phi/2pi and is reduced by magic_rev_to_unit (the caller has checked the
+// stage's bound); else by v_fract_f64 and a conversion.
+template <int D, int MPT, int Q, typename GT, bool MAGIC>
+__device__ __forceinline__ void adj_stage_f32(int cnt, const double* s_x, const GT* s_g,
+                                              const double (&k)[MPT][D], const double (&mphi)[MPT],
+                                              float (&pa)[MPT][Q]) {
+    double c0[MPT];
+#pragma unroll
+    for (int u = 0; u < MPT; ++u) c0[u] = MAGIC ? (kPhaseMagic + kPhaseRound) + mphi[u] : mphi[u];
+    for (int m = 0; m < cnt; ++m) {
+        double x[D];
+#pragma unroll
+        for (int d = 0; d < D; ++d) x[d] = s_x[m * D + d];
+        const GT* gg = s_g + m * (D + 1);
+        const float g0 = gg[0];
+        float gx[D];
+#pragma unroll
+        for (int d = 0; d < D; ++d) gx[d] = gg[1 + d];
+#pragma unroll
+        for (int u = 0; u < MPT; ++u) {
+            double rev = c0[u];
+#pragma unroll
+            for (int d = 0; d < D; ++d) rev = fma_(k[u][d], x[d], rev);
+            const float f = MAGIC ? magic_rev_to_unit(rev) : (float)__builtin_amdgcn_fract(rev);
+            const float cs = __builtin_amdgcn_cosf(f);
+            const float sn = __builtin_amdgcn_sinf(f);
+            pa[u][0] = __builtin_fmaf(g0, cs, pa[u][0]);
+            pa[u][1] = __builtin_fmaf(g0, sn, pa[u][1]);
+#pragma unroll
+            for (int d = 0; d < D; ++d) pa[u][2 + d] = __builtin_fmaf(gx[d], sn, pa[u][2 + d]);
+        }
+    }
+}
+
 template <int D, typename GT, int MPT>
 __global__ __launch_bounds__(kThreads) void k_adjoint_stage1(int64_t N, int64_t P, int64_t chunk_pts,
                                                              const double* __restrict__ cov,
@@ -131,7 +169,7 @@ __global__ __launch_bounds__(kThreads) void k_adjoint_stage1(int64_t N, int64_t 
                 const double xv = pos[d * P + i];
                 s_x[threadIdx.x * D + d] = xv;
                 s_g[threadIdx.x * (D + 1) + 1 + d] = (GT)(gi * xv);
-                xm = fmax(xm, fabs(xv));
+                xm = coord_bound(xm, xv);
             }
             atomicMax(&s_xmax, (unsigned long long)__double_as_longlong(xm));
         }
@@ -176,28 +214,13 @@ __global__ __launch_bounds__(kThreads) void k_adjoint_stage1(int64_t N, int64_t 
             for (int u = 0; u < MPT; ++u)
 #pragma unroll
                 for (int q = 0; q < Q; ++q) pa[u][q] = 0.0f;
-            for (int m = 0; m < cnt; ++m) {
-                double x[D];
-#pragma unroll
-                for (int d = 0; d < D; ++d) x[d] = s_x[m * D + d];
-                const GT* gg = s_g + m * (D + 1);
-                const float g0 = gg[0];
-                float gx[D];
-#pragma unroll
-                for (int d = 0; d < D; ++d) gx[d] = gg[1 + d];
-#pragma unroll
-                for (int u = 0; u < MPT; ++u) {
-                    double rev = mphi[u];
-#pragma unroll
-                    for (int d = 0; d < D; ++d) rev = fma_(k[u][d], x[d], rev);
-                    const float f = (float)__builtin_amdgcn_fract(rev);
-                    const float cs = __builtin_amdgcn_cosf(f);
-                    const float sn = __builtin_amdgcn_sinf(f);
-                    pa[u][0] = __builtin_fmaf(g0, cs, pa[u][0]);
-                    pa[u][1] = __builtin_fmaf(g0, sn, pa[u][1]);
-#pragma unroll
-                    for (int d = 0; d < D; ++d) pa[u][2 + d] = __builtin_fmaf(gx[d], sn, pa[u][2 + d]);
-                }
+            // magic-addition reduction when every mode of the wave keeps
+            // |k/2pi|_1 |x|_inf within 2^18 revolutions over this stage
+            const double xmax = __longlong_as_double((long long)s_xmax);
+            if (__all(k1 * xmax <= kPhaseMagicMax)) {
+                adj_stage_f32<D, MPT, Q, GT, true>(cnt, s_x, s_g, k, mphi, pa);
+            } else {
+                adj_stage_f32<D, MPT, Q, GT, false>(cnt, s_x, s_g, k, mphi, pa);
             }
 #pragma unroll
             for (int u = 0; u < MPT; ++u)

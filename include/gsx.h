@@ -107,7 +107,12 @@ int gsx_summate_f64(int64_t d, int64_t N, int64_t P, const double* cov,
 /* FP32 summation of the same sum, out (P,) float32.  Phase in FP64 (range
  * reduction is the precision-critical step), cosine and mode sum in FP32 with
  * blocked FP32 -> FP64 accumulation.  Replaces summate where an FP32 field is
- * acceptable (error <= 1e-5 relative to FP64). */
+ * acceptable (error <= 1e-5 relative to FP64).  The phase is reduced by magic
+ * addition for the modes whose |k/2pi|_1 times the largest |x|_inf of the
+ * point's aligned 2048-point segment stays within 2^18 revolutions, by
+ * fract-and-convert for the others: a point's bits depend on its segment's
+ * largest coordinate as well as its own (the batch plan applies the same
+ * rule, so its items equal their single launches bit for bit). */
 int gsx_summate_f32(int64_t d, int64_t N, int64_t P, const double* cov,
                     const double* z1, const double* z2, const double* pos,
                     float* out, void* stream);

@@ -84,16 +84,17 @@ def test_fp32_within_1e5(d):
 
 @pytest.mark.parametrize("incompr", [False, True])
 def test_fp32_phase_reductions_both_regimes(incompr):
-    """The FP32 kernels reduce the phase by magic addition where a wave's
-    |k/2pi|_1 |x|_inf stays below 2^18 revolutions and by fract-and-convert
-    beyond.  Points at |x| ~ 3e6 (phases ~1e6 revolutions) next to points at
-    |x| ~ 100 put both in one launch, over 5 mode chunks; both must meet the
-    FP32 bar, and a wave of small points gives the same sums whichever
-    regime its neighbours in other waves take."""
+    """The FP32 kernels reduce the phase by magic addition for the modes whose
+    |k/2pi|_1 times the largest |x|_inf of the point's 2048-point segment stays
+    within 2^18 revolutions and by fract-and-convert for the others.  A
+    segment of points at |x| ~ 3e6 (phases ~1e6 revolutions) between
+    segments at |x| ~ 100 puts both in one launch, over 5 mode chunks; both
+    must meet the FP32 bar, and the small segments give the same sums as a
+    launch without the far points."""
     _say(f"fp32 magic and fract reductions incompr={incompr}")
     cov, z1, z2, pos = _case(3, 1100, 8192, 91, 1.0, 100.0)
     big = pos.copy()
-    big[:, 2048:4096] *= 3e4  # the second 256-point block: all of its waves take the fract form
+    big[:, 2048:4096] *= 3e4  # the second 2048-point segment: its large modes take the fract form
     fn = summate_incompr_device if incompr else summate_device
     ref_fn = oracle.summate_incompr_c if incompr else oracle.summate_c
     bound = np.abs(cov).sum(axis=0).max() / (2 * np.pi) * np.abs(big[:, 2048:4096]).max()
@@ -113,7 +114,10 @@ def test_fp32_full_chip_launch_matches_prefix(incompr, p_big):
     """A launch of millions of points (thousands of blocks, every compute unit
     busy) and a small launch of its first 30001 points agree with each other and
     with the oracle.  N = 1000 leaves a ragged last chunk of 232 modes; 30001 is
-    ragged against the 2048- (or 1024-) point block."""
+    ragged against the 2048- (or 1024-) point block.  Bit for bit because every
+    mode here takes the magic-addition reduction in both launches (|k/2pi|_1
+    |x|_inf stays far below 2^18 revolutions); the test with both reductions
+    covers segments that differ."""
     _say(f"fp32 full-chip launch vs prefix incompr={incompr} P={p_big}")
     cov, z1, z2, pos = _case(3, 1000, p_big, 77, 2.0, 500.0)
     fn = summate_incompr_device if incompr else summate_device
