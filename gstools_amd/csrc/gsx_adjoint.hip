@@ -88,6 +88,9 @@ __device__ __forceinline__ void adj_stage_f32(int cnt, const double* s_x, const 
     double c0[MPT];
 #pragma unroll
     for (int u = 0; u < MPT; ++u) c0[u] = MAGIC ? (kPhaseMagic + kPhaseRound) + mphi[u] : mphi[u];
+    // four points per iteration: their LDS reads are issued together and one
+    // wait serves them (the compiler unrolls only two with both forms inlined)
+#pragma unroll 4
     for (int m = 0; m < cnt; ++m) {
         double x[D];
 #pragma unroll
