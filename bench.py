@@ -1054,7 +1054,7 @@ def main():
                         "mode_no 1024 (MCMC modes, seed 20170519), 4e6 points uniform in [0,1000]^3",
             "points": P_HEADLINE,
             "modes": N_HEADLINE,
-            "path": "fp32 (fp64 phase, v_cos_f32, blocked fp32->fp64 sum)",
+            "path": "fp32 (fp64 phase reduced by magic addition, fract form for modes beyond the bound, v_cos_f32, blocked fp32->fp64 sum)",
         },
         "roofline": {
             "bound": "valu",
