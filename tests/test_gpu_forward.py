@@ -109,6 +109,28 @@ def test_fp32_phase_reductions_both_regimes(incompr):
     assert np.array_equal(small[..., 4096:], got[..., 4096:])
 
 
+@pytest.mark.parametrize("precision", ["fp32", "fp64"])
+def test_non_finite_coordinates_propagate(precision):
+    """A NaN or infinite coordinate gives NaN at its point, as the reference's
+    loop does (cos of NaN or inf), in both precisions; in FP32 it makes its
+    2048-point segment's bound infinite, so the segment's other points take
+    the fract-form reduction and still meet the FP32 bar."""
+    _say(f"non-finite coordinates {precision}")
+    cov, z1, z2, pos = _case(3, 300, 5000, 17, 1.0, 100.0)
+    bad = pos.copy()
+    bad[1, 100] = np.nan
+    bad[2, 2500] = np.inf
+    bad[0, 4999] = -np.inf
+    got = summate_device(cov, z1, z2, bad, precision=precision).cpu().numpy().astype(np.float64)
+    hit = np.zeros(5000, bool)
+    hit[[100, 2500, 4999]] = True
+    assert np.all(np.isnan(got[hit]))
+    assert np.all(np.isfinite(got[~hit]))
+    ref = oracle.summate_c(cov, z1, z2, pos)
+    rel_max, rel_rms = oracle.rel_errors(got[~hit], ref[~hit])
+    assert rel_max <= (FP32_TOL if precision == "fp32" else FP64_TOL), (rel_max, rel_rms)
+
+
 @pytest.mark.parametrize("incompr,p_big", [(False, 4_500_000), (True, 2_500_000)])
 def test_fp32_full_chip_launch_matches_prefix(incompr, p_big):
     """A launch of millions of points (thousands of blocks, every compute unit
